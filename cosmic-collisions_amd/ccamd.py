@@ -1,0 +1,275 @@
+"""Python binding of the C ABI in include/cc_hip.h (libcc_hip.so), over ctypes.
+
+A thin layer: numpy arrays in, numpy arrays out, a Python exception for every
+negative return code with the library's own message.  It is what the tests
+and the benchmark drive; the reference-shaped engine object is the
+JavaScript one in js/nbody.js.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcc_hip.so")
+_LIB = None
+
+FL = 6.0
+FH = 2 * FL / 64
+PP_CAP = 8192
+
+
+class Params(C.Structure):
+    _fields_ = [("N", C.c_int), ("W", C.c_int), ("H", C.c_int), ("G", C.c_int),
+                ("m", C.c_double), ("a", C.c_double), ("cell", C.c_double), ("dt", C.c_double), ("k", C.c_double),
+                ("c", C.c_double), ("creep", C.c_double), ("damp", C.c_double), ("tK", C.c_double),
+                ("kappa", C.c_double), ("cpIron", C.c_double), ("seatSlack", C.c_double),
+                ("spin0", C.c_double * 3), ("spin1", C.c_double * 3), ("field", C.c_int), ("gField", C.c_double * 3),
+                ("frozenOn", C.c_int), ("frozen", C.c_double), ("meshEvery", C.c_int)]
+
+
+class SimArgs(C.Structure):
+    _fields_ = [("settle", C.c_int), ("kick", C.c_int), ("gdt", C.c_double), ("bulk0", C.c_double * 3),
+                ("bulk1", C.c_double * 3), ("cen0", C.c_double * 3), ("cen1", C.c_double * 3)]
+
+
+EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_upload_state", "cc_download_state",
+           "cc_set_boxes", "cc_set_mono", "cc_slots", "cc_read_seats", "cc_deposit", "cc_coarse", "cc_cell", "cc_mesh",
+           "cc_read_mesh", "cc_write_mesh", "cc_set_pp_table", "cc_set_pp", "cc_pp", "cc_read_pp", "cc_sim", "cc_step",
+           "cc_step_n", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_diag", "cc_morton", "cc_books", "cc_sync",
+           "cc_last_step_ms"]
+
+
+class CCError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CCError("libcc_hip.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        dp, fp, ip, up = (C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                          C.POINTER(C.c_uint32))
+        sig = {
+            "cc_create": ([C.POINTER(Params), C.POINTER(P)], C.c_int), "cc_destroy": ([P], C.c_int),
+            "cc_last_error": ([P], C.c_char_p), "cc_set_params": ([P, C.POINTER(Params)], C.c_int),
+            "cc_upload_state": ([P, fp, fp, fp, fp], C.c_int), "cc_download_state": ([P, fp, fp, fp, fp, fp], C.c_int),
+            "cc_set_boxes": ([P, dp, dp, C.c_int], C.c_int), "cc_set_mono": ([P, C.c_int, fp], C.c_int),
+            "cc_slots": ([P], C.c_int), "cc_read_seats": ([P, up, up], C.c_int),
+            "cc_deposit": ([P, C.c_int], C.c_int), "cc_coarse": ([P, C.c_int], C.c_int),
+            "cc_cell": ([P, C.c_int], C.c_int), "cc_mesh": ([P], C.c_int),
+            "cc_read_mesh": ([P, C.c_int, C.c_int, fp], C.c_int), "cc_write_mesh": ([P, C.c_int, C.c_int, fp], C.c_int),
+            "cc_set_pp_table": ([P, fp], C.c_int), "cc_set_pp": ([P, ip, C.c_int], C.c_int), "cc_pp": ([P], C.c_int),
+            "cc_read_pp": ([P, fp, fp, fp], C.c_int), "cc_sim": ([P, C.POINTER(SimArgs)], C.c_int),
+            "cc_step": ([P, C.c_int, C.c_int, C.POINTER(SimArgs)], C.c_int),
+            "cc_step_n": ([P, C.c_int, C.POINTER(SimArgs), dp, C.c_int], C.c_int),
+            "cc_get_step_no": ([P, ip], C.c_int), "cc_set_step_no": ([P, C.c_int], C.c_int),
+            "cc_rigid": ([P, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int), "cc_diag": ([P, fp], C.c_int),
+            "cc_morton": ([P, dp, ip, ip], C.c_int), "cc_books": ([P, dp], C.c_int), "cc_sync": ([P], C.c_int),
+            "cc_last_step_ms": ([P, dp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _ptr(a, t=C.c_float):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def _d3(v):
+    return (C.c_double * len(v))(*[float(x) for x in v])
+
+
+def params(N, W, G, m, a, cell, dt, k, c, creep=0.12, damp=None, tK=None, kappa=0.3, spin0=(0, 0, 0),
+           spin1=(0, 0, 0), field=False, gField=(0, 0, 0), frozen=None, meshEvery=8, cpIron=450.0 / 1200.0,
+           seatSlack=0.6):
+    p = Params()
+    p.N, p.W, p.H, p.G = N, W, N // W, G
+    p.m, p.a, p.cell, p.dt, p.k, p.c, p.creep = m, a, cell, dt, k, c, creep
+    p.damp = float(np.exp(-dt * 12)) if damp is None else damp
+    p.tK = 0.5 * 6.256e7 / 1200.0 / m if tK is None else tK
+    p.kappa, p.cpIron, p.seatSlack = kappa, cpIron, seatSlack
+    p.spin0[:] = list(spin0)
+    p.spin1[:] = list(spin1)
+    p.field = 1 if field else 0
+    p.gField[:] = list(gField)
+    p.frozenOn = 0 if frozen is None else 1
+    p.frozen = 0.0 if frozen is None else float(frozen)
+    p.meshEvery = meshEvery
+    return p
+
+
+def sim_args(settle=False, kick=False, gdt=0.0, bulk0=(0, 0, 0), bulk1=(0, 0, 0), cen0=(0, 0, 0), cen1=(0, 0, 0)):
+    s = SimArgs()
+    s.settle, s.kick, s.gdt = int(bool(settle)), int(bool(kick)), float(gdt)
+    s.bulk0[:], s.bulk1[:], s.cen0[:], s.cen1[:] = list(bulk0), list(bulk1), list(cen0), list(cen1)
+    return s
+
+
+class Context:
+    """One simulation instance on the GPU (cc_ctx)."""
+
+    def __init__(self, p):
+        self.p = p
+        self.N = p.N
+        self.G = p.G
+        sx = 16 if p.G == 128 else 8          # src/gl.js:73-83: SX slices across, G/SX down
+        self.AW = sx * p.G
+        self.AH = (p.G // sx) * p.G
+        h = C.c_void_p()
+        r = lib().cc_create(C.byref(p), C.byref(h))
+        if r != 0:
+            raise CCError("cc_create: %s" % lib().cc_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, r, what):
+        if r != 0:
+            raise CCError("%s: %s" % (what, lib().cc_last_error(self.h).decode()))
+
+    def close(self):
+        if self.h:
+            lib().cc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        self._chk(lib().cc_set_params(self.h, C.byref(p)), "cc_set_params")
+        self.p = p
+
+    def upload(self, pos=None, vel=None, mat=None, q=None):
+        arrs = [None if x is None else _f(x) for x in (pos, vel, mat, q)]
+        self._chk(lib().cc_upload_state(self.h, *[_ptr(a) for a in arrs]), "cc_upload_state")
+
+    def download(self):
+        N = self.N
+        pos, vel, aux = (np.empty((N, 4), np.float32) for _ in range(3))
+        q, mat = np.empty(N, np.float32), np.empty(N, np.float32)
+        self._chk(lib().cc_download_state(self.h, _ptr(pos), _ptr(vel), _ptr(aux), _ptr(q), _ptr(mat)),
+                  "cc_download_state")
+        return pos, vel, aux, q, mat
+
+    def set_boxes(self, LA, LB, boxB):
+        self._chk(lib().cc_set_boxes(self.h, _d3(LA), _d3(LB), int(bool(boxB))), "cc_set_boxes")
+
+    def set_mono(self, box, mono):
+        self._chk(lib().cc_set_mono(self.h, box, _ptr(_f(mono))), "cc_set_mono")
+
+    def slots(self):
+        self._chk(lib().cc_slots(self.h), "cc_slots")
+
+    def read_seats(self):
+        a = np.empty((self.AH, self.AW, 4), np.uint32)
+        b = np.empty((self.AH, self.AW, 4), np.uint32)
+        self._chk(lib().cc_read_seats(self.h, _ptr(a, C.c_uint32), _ptr(b, C.c_uint32)), "cc_read_seats")
+        return a, b
+
+    def deposit(self, box=0):
+        self._chk(lib().cc_deposit(self.h, box), "cc_deposit")
+
+    def coarse(self, box=0):
+        self._chk(lib().cc_coarse(self.h, box), "cc_coarse")
+
+    def cell(self, box=0):
+        self._chk(lib().cc_cell(self.h, box), "cc_cell")
+
+    def mesh(self):
+        self._chk(lib().cc_mesh(self.h), "cc_mesh")
+
+    SHAPES = {0: (513, 512, 4), 1: (64, 64, 4), 2: (4,), 3: (64, 64, 4), 4: (64, 64, 4), 5: (64, 64, 4),
+              6: (513, 512, 4)}
+    NAMES = {"fine": 0, "coarse": 1, "mono": 2, "farA": 3, "farT0": 4, "farT1": 5, "force": 6}
+
+    def read_mesh(self, box, which):
+        w = self.NAMES.get(which, which)
+        out = np.empty(self.SHAPES[w], np.float32)
+        self._chk(lib().cc_read_mesh(self.h, box, w, _ptr(out)), "cc_read_mesh")
+        return out
+
+    def write_mesh(self, box, which, arr):
+        w = self.NAMES.get(which, which)
+        a = _f(arr).reshape(self.SHAPES[w])
+        self._chk(lib().cc_write_mesh(self.h, box, w, _ptr(a)), "cc_write_mesh")
+
+    def set_pp_table(self, tab):
+        self._chk(lib().cc_set_pp_table(self.h, _ptr(_f(tab).reshape(64, 2))), "cc_set_pp_table")
+
+    def set_pp(self, lst):
+        a = np.ascontiguousarray(lst, np.int32)
+        self._chk(lib().cc_set_pp(self.h, _ptr(a, C.c_int32), len(a)), "cc_set_pp")
+
+    def pp(self):
+        self._chk(lib().cc_pp(self.h), "cc_pp")
+
+    def read_pp(self):
+        lst = np.empty((PP_CAP, 4), np.float32)
+        box = np.empty(PP_CAP, np.float32)
+        force = np.empty((4, PP_CAP, 4), np.float32)
+        self._chk(lib().cc_read_pp(self.h, _ptr(lst), _ptr(box), _ptr(force)), "cc_read_pp")
+        return lst, box, force
+
+    def sim(self, args):
+        self._chk(lib().cc_sim(self.h, C.byref(args)), "cc_sim")
+
+    def step(self, settle, kick, args):
+        self._chk(lib().cc_step(self.h, int(settle), int(kick), C.byref(args)), "cc_step")
+
+    def step_n(self, n, args, boxL=None):
+        if boxL is None:
+            self._chk(lib().cc_step_n(self.h, n, C.byref(args), None, 0), "cc_step_n")
+        else:
+            b = np.ascontiguousarray(boxL, np.float64).reshape(-1, 7)
+            self._chk(lib().cc_step_n(self.h, n, C.byref(args), _ptr(b, C.c_double), len(b)), "cc_step_n")
+        ms = C.c_double()
+        lib().cc_last_step_ms(self.h, C.byref(ms))
+        return ms.value
+
+    @property
+    def step_no(self):
+        v = C.c_int32()
+        self._chk(lib().cc_get_step_no(self.h, C.byref(v)), "cc_get_step_no")
+        return v.value
+
+    @step_no.setter
+    def step_no(self, n):
+        self._chk(lib().cc_set_step_no(self.h, int(n)), "cc_set_step_no")
+
+    def rigid(self, bulk0, bulk1, cen0, cen1, rot0, rot1, dtF):
+        self._chk(lib().cc_rigid(self.h, _d3(bulk0), _d3(bulk1), _d3(cen0), _d3(cen1), _d3(rot0), _d3(rot1),
+                                 float(dtF)), "cc_rigid")
+
+    def diag(self):
+        out = np.empty((self.N, 4), np.float32)
+        self._chk(lib().cc_diag(self.h, _ptr(out)), "cc_diag")
+        return out
+
+    def morton(self, core):
+        perm = np.empty(self.N, np.int32)
+        into = np.empty(self.N, np.int32)
+        self._chk(lib().cc_morton(self.h, _d3(core), _ptr(perm, C.c_int32), _ptr(into, C.c_int32)), "cc_morton")
+        return perm, into
+
+    BOOKS = ("M", "px", "py", "pz", "Jx", "Jy", "Jz", "KE", "PE", "Q", "iron", "unseated", "full")
+
+    def books(self):
+        out = np.empty(13, np.float64)
+        self._chk(lib().cc_books(self.h, _ptr(out, C.c_double)), "cc_books")
+        return dict(zip(self.BOOKS, out.tolist()))
+
+    def sync(self):
+        self._chk(lib().cc_sync(self.h), "cc_sync")

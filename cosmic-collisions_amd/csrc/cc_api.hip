@@ -1,0 +1,540 @@
+// The C ABI (include/cc_hip.h): device buffers of one simulation instance,
+// uniform derivation as the reference's setStaticUniforms does it, and the
+// pass sequence of its step() -- all queued on one HIP stream.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cc_hip.h"
+#include "cc_internal.h"
+
+using namespace cc;
+
+struct cc_ctx {
+  cc_params p;
+  hipStream_t s = nullptr;
+  int N = 0;
+  Grid grid;
+  size_t cells = 0;
+  float4* pos[2] = { nullptr, nullptr };
+  float4* vel[2] = { nullptr, nullptr };
+  float4* aux[2] = { nullptr, nullptr };
+  float* q[2] = { nullptr, nullptr };
+  float* mat = nullptr;
+  float* matTmp = nullptr;
+  int cur = 0;
+  GridBuf gb;
+  MeshBuf mesh[2];
+  int* ppIndex = nullptr;
+  float4* ppList = nullptr;
+  float* ppBox = nullptr;
+  float4* ppForce = nullptr;
+  float* ppSlot = nullptr;
+  float2* ppTab = nullptr;
+  int ppCount = 0;
+  float4* diag = nullptr;
+  int *mkeys = nullptr, *mslot = nullptr, *mhist = nullptr, *mblock = nullptr, *mperm = nullptr, *minto = nullptr;
+  double *bpart = nullptr, *bout = nullptr;
+  Box box;
+  int stepNo = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double lastMs = 0.0;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+static std::string g_err;
+
+#define FAIL(ctx, code, ...)                                      \
+  do {                                                            \
+    char _b[512];                                                 \
+    snprintf(_b, sizeof _b, __VA_ARGS__);                         \
+    if (ctx) (ctx)->err = _b; else g_err = _b;                    \
+    return code;                                                  \
+  } while (0)
+#define HIPCHK(ctx, call)                                                                  \
+  do {                                                                                     \
+    hipError_t _e = (call);                                                                \
+    if (_e != hipSuccess) FAIL(ctx, CC_EDEVICE, "%s: %s", #call, hipGetErrorString(_e));  \
+  } while (0)
+#define LAUNCHCHK(ctx)                                                                              \
+  do {                                                                                              \
+    hipError_t _e = hipGetLastError();                                                              \
+    if (_e != hipSuccess) FAIL(ctx, CC_EDEVICE, "kernel launch: %s", hipGetErrorString(_e));       \
+  } while (0)
+
+static int dalloc(cc_ctx* c, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) FAIL(c, CC_EDEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  return 0;
+}
+#define ALLOC(ptr, bytes) \
+  do { int _r = dalloc(c, (void**)&(ptr), (bytes)); if (_r) return _r; } while (0)
+
+static void grid_make(int G, Grid& g) {   // src/gl.js:73-83
+  g.G = G;
+  g.GShift = G == 128 ? 7 : 6;
+  g.GMask = G - 1;
+  g.SX = G == 128 ? 16 : 8;
+  g.SXShift = g.SX == 16 ? 4 : 3;
+  g.SXMask = g.SX - 1;
+  g.AW = g.SX * G;
+  g.AH = (G / g.SX) * G;
+}
+
+static int check_params(cc_ctx* c, const cc_params* p) {
+  if (!p) FAIL(c, CC_EINVAL, "params is NULL");
+  if (p->N <= 0 || p->W <= 0 || p->H <= 0 || (long long)p->W * p->H != p->N)
+    FAIL(c, CC_EINVAL, "N must equal W*H (N=%d W=%d H=%d)", p->N, p->W, p->H);
+  if (p->N > (1 << 18))
+    FAIL(c, CC_EINVAL, "N=%d exceeds the seat word's 18-bit particle index (262144)", p->N);
+  if (p->G != 64 && p->G != 128) FAIL(c, CC_EINVAL, "contact grid G must be 64 or 128, got %d", p->G);
+  if (p->meshEvery <= 0) FAIL(c, CC_EINVAL, "meshEvery must be positive");
+  if (!(p->m > 0) || !(p->a > 0) || !(p->cell > 0) || !(p->dt > 0)) FAIL(c, CC_EINVAL, "m, a, cell, dt must be > 0");
+  return 0;
+}
+
+// the step pass's uniforms as setStaticUniforms + passSim set them
+static SimU sim_uniforms(const cc_ctx* c, const cc_sim_args* a) {
+  const cc_params& p = c->p;
+  SimU u;
+  memset(&u, 0, sizeof u);
+  u.N = c->N;
+  u.invCell = (float)(1.0 / p.cell);
+  u.cell = (float)p.cell;
+  u.dt = (float)p.dt;
+  u.touch = (float)(2.0 * p.a);
+  u.link2 = (float)(p.cell * p.cell);
+  double reach = p.cell + p.seatSlack * p.a;
+  u.reach2 = (float)(reach * reach);
+  u.k = (float)p.k;
+  u.c = (float)p.c;
+  u.invM = (float)(1.0 / p.m);
+  u.damp = (float)p.damp;
+  u.creep = (float)p.creep;
+  u.tK = (float)p.tK;
+  u.kappa = (float)p.kappa;
+  u.cpIron = (float)p.cpIron;
+  for (int k = 0; k < 3; k++) { u.spin0[k] = (float)p.spin0[k]; u.spin1[k] = (float)p.spin1[k]; }
+  u.grid = c->grid;
+  u.box = c->box;
+  u.gm = (float)p.m;
+  u.eps2 = (float)(FH * FH);
+  u.field = p.field;
+  for (int k = 0; k < 3; k++) u.gField[k] = (float)p.gField[k];
+  u.frozenOn = p.frozenOn;
+  u.frozen = (float)p.frozen;
+  if (a) {
+    u.settle = a->settle ? 1.0f : 0.0f;
+    u.kick = a->kick ? 1.0f : 0.0f;
+    u.gdt = (float)a->gdt;
+    for (int k = 0; k < 3; k++) {
+      u.bulk0[k] = (float)a->bulk0[k]; u.bulk1[k] = (float)a->bulk1[k];
+      u.cen0[k] = (float)a->cen0[k]; u.cen1[k] = (float)a->cen1[k];
+    }
+  }
+  return u;
+}
+
+extern "C" {
+
+const char* cc_last_error(cc_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int cc_create(const cc_params* p, cc_ctx** out) {
+  if (!out) FAIL((cc_ctx*)nullptr, CC_EINVAL, "out is NULL");
+  *out = nullptr;
+  int r = check_params(nullptr, p);
+  if (r) return r;
+  cc_ctx* c = new cc_ctx();
+  c->p = *p;
+  c->N = p->N;
+  grid_make(p->G, c->grid);
+  c->cells = (size_t)c->grid.AW * c->grid.AH;
+  hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
+  if (e != hipSuccess) { g_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete c; return CC_EDEVICE; }
+  size_t N = (size_t)c->N;
+  int rr = 0;
+  auto step = [&]() -> int {
+    for (int b = 0; b < 2; b++) {
+      ALLOC(c->pos[b], N * 16); ALLOC(c->vel[b], N * 16); ALLOC(c->aux[b], N * 16); ALLOC(c->q[b], N * 4);
+    }
+    ALLOC(c->mat, N * 4); ALLOC(c->matTmp, N * 4);
+    ALLOC(c->gb.seats, c->cells * SLOTS * 4);
+    ALLOC(c->gb.cnt, c->cells * 4);
+    ALLOC(c->gb.bucket, c->cells * BUCKET_CAP * 4);
+    ALLOC(c->gb.occ[0], N * 4); ALLOC(c->gb.occ[1], N * 4);
+    ALLOC(c->gb.occN, 2 * 4);
+    ALLOC(c->gb.ovf, N * 8); ALLOC(c->gb.ovfN, 4);
+    ALLOC(c->gb.word, N * 4);
+    for (int b = 0; b < 2; b++) {
+      MeshBuf& m = c->mesh[b];
+      ALLOC(m.acc, (size_t)FINE_TEXELS * 4 * 8);
+      ALLOC(m.fine, (size_t)FINE_TEXELS * 16); ALLOC(m.force, (size_t)FINE_TEXELS * 16);
+      ALLOC(m.coarse, 4096 * 16); ALLOC(m.farA, 4096 * 16); ALLOC(m.farT0, 4096 * 16); ALLOC(m.farT1, 4096 * 16);
+      ALLOC(m.mono, 16);
+      ALLOC(m.occ, (size_t)SCRATCH * 4); ALLOC(m.occN, 4);
+    }
+    ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
+    ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
+    ALLOC(c->diag, N * 16);
+    ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
+    ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4);
+    ALLOC(c->bpart, 256 * 13 * 8); ALLOC(c->bout, 13 * 8);
+    // the empty state: seats empty, counters and accumulators zero, meshes zero
+    launch_fill_u32(c->gb.seats, c->cells * SLOTS, EMPTY_SEAT, c->s);
+    HIPCHK(c, hipMemsetAsync(c->gb.cnt, 0, c->cells * 4, c->s));
+    HIPCHK(c, hipMemsetAsync(c->gb.occN, 0, 8, c->s));
+    HIPCHK(c, hipMemsetAsync(c->gb.ovfN, 0, 4, c->s));
+    for (int b = 0; b < 2; b++) {
+      MeshBuf& m = c->mesh[b];
+      HIPCHK(c, hipMemsetAsync(m.acc, 0, (size_t)FINE_TEXELS * 32, c->s));
+      HIPCHK(c, hipMemsetAsync(m.fine, 0, (size_t)FINE_TEXELS * 16, c->s));
+      HIPCHK(c, hipMemsetAsync(m.force, 0, (size_t)FINE_TEXELS * 16, c->s));
+      for (float4* t : { m.coarse, m.farA, m.farT0, m.farT1 }) HIPCHK(c, hipMemsetAsync(t, 0, 4096 * 16, c->s));
+      HIPCHK(c, hipMemsetAsync(m.mono, 0, 16, c->s));
+      HIPCHK(c, hipMemsetAsync(m.occN, 0, 4, c->s));
+    }
+    for (int b = 0; b < 2; b++) {
+      HIPCHK(c, hipMemsetAsync(c->aux[b], 0, N * 16, c->s));
+      HIPCHK(c, hipMemsetAsync(c->q[b], 0, N * 4, c->s));
+    }
+    launch_fill_u32(reinterpret_cast<uint32_t*>(c->ppIndex), PP_CAP, 0xFFFFFFFFu, c->s);
+    launch_fill_f32(c->ppSlot, N, -1.0f, c->s);
+    HIPCHK(c, hipMemsetAsync(c->ppForce, 0, 4 * PP_CAP * 16, c->s));
+    HIPCHK(c, hipMemsetAsync(c->ppList, 0, PP_CAP * 16, c->s));
+    HIPCHK(c, hipMemsetAsync(c->ppTab, 0, 64 * 8, c->s));
+    HIPCHK(c, hipMemsetAsync(c->diag, 0, N * 16, c->s));
+    LAUNCHCHK(c);
+    HIPCHK(c, hipEventCreate(&c->ev0));
+    HIPCHK(c, hipEventCreate(&c->ev1));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+  };
+  rr = step();
+  if (rr) { g_err = c->err; cc_destroy(c); return rr; }
+  memset(&c->box, 0, sizeof c->box);
+  c->box.invH = (float)(1.0 / FH);
+  *out = c;
+  return 0;
+}
+
+int cc_destroy(cc_ctx* c) {
+  if (!c) return 0;
+  if (c->s) hipStreamSynchronize(c->s);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+  return 0;
+}
+
+int cc_set_params(cc_ctx* c, const cc_params* p) {
+  if (!c) return CC_EINVAL;
+  int r = check_params(c, p);
+  if (r) return r;
+  if (p->N != c->p.N || p->W != c->p.W || p->G != c->p.G) FAIL(c, CC_EINVAL, "N, W and G are fixed at create");
+  c->p = *p;
+  return 0;
+}
+
+int cc_upload_state(cc_ctx* c, const float* pos, const float* vel, const float* mat, const float* q) {
+  if (!c) return CC_EINVAL;
+  size_t N = (size_t)c->N;
+  if (pos) HIPCHK(c, hipMemcpyAsync(c->pos[c->cur], pos, N * 16, hipMemcpyHostToDevice, c->s));
+  if (vel) HIPCHK(c, hipMemcpyAsync(c->vel[c->cur], vel, N * 16, hipMemcpyHostToDevice, c->s));
+  if (mat) HIPCHK(c, hipMemcpyAsync(c->mat, mat, N * 4, hipMemcpyHostToDevice, c->s));
+  if (q) HIPCHK(c, hipMemcpyAsync(c->q[c->cur], q, N * 4, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_download_state(cc_ctx* c, float* pos, float* vel, float* aux, float* q, float* mat) {
+  if (!c) return CC_EINVAL;
+  size_t N = (size_t)c->N;
+  if (pos) HIPCHK(c, hipMemcpyAsync(pos, c->pos[c->cur], N * 16, hipMemcpyDeviceToHost, c->s));
+  if (vel) HIPCHK(c, hipMemcpyAsync(vel, c->vel[c->cur], N * 16, hipMemcpyDeviceToHost, c->s));
+  if (aux) HIPCHK(c, hipMemcpyAsync(aux, c->aux[c->cur], N * 16, hipMemcpyDeviceToHost, c->s));
+  if (q) HIPCHK(c, hipMemcpyAsync(q, c->q[c->cur], N * 4, hipMemcpyDeviceToHost, c->s));
+  if (mat) HIPCHK(c, hipMemcpyAsync(mat, c->mat, N * 4, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_set_boxes(cc_ctx* c, const double LA[3], const double LB[3], int boxB) {
+  if (!c || !LA || !LB) return CC_EINVAL;
+  for (int k = 0; k < 3; k++) { c->box.LA[k] = (float)LA[k]; c->box.LB[k] = (float)LB[k]; }
+  c->box.boxB = boxB ? 1.0f : 0.0f;
+  c->box.invH = (float)(1.0 / FH);
+  return 0;
+}
+
+static float4* mesh_ptr(cc_ctx* c, int box, int which, size_t* bytes) {
+  MeshBuf& m = c->mesh[box];
+  switch (which) {
+    case 0: *bytes = (size_t)FINE_TEXELS * 16; return m.fine;
+    case 1: *bytes = 4096 * 16; return m.coarse;
+    case 2: *bytes = 16; return m.mono;
+    case 3: *bytes = 4096 * 16; return m.farA;
+    case 4: *bytes = 4096 * 16; return m.farT0;
+    case 5: *bytes = 4096 * 16; return m.farT1;
+    case 6: *bytes = (size_t)FINE_TEXELS * 16; return m.force;
+    default: return nullptr;
+  }
+}
+
+int cc_set_mono(cc_ctx* c, int box, const float mono[4]) { return cc_write_mesh(c, box, 2, mono); }
+
+int cc_read_mesh(cc_ctx* c, int box, int which, float* out) {
+  if (!c || !out || box < 0 || box > 1) return CC_EINVAL;
+  size_t bytes;
+  float4* p = mesh_ptr(c, box, which, &bytes);
+  if (!p) FAIL(c, CC_EINVAL, "unknown mesh texture %d", which);
+  HIPCHK(c, hipMemcpyAsync(out, p, bytes, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_write_mesh(cc_ctx* c, int box, int which, const float* in) {
+  if (!c || !in || box < 0 || box > 1) return CC_EINVAL;
+  size_t bytes;
+  float4* p = mesh_ptr(c, box, which, &bytes);
+  if (!p) FAIL(c, CC_EINVAL, "unknown mesh texture %d", which);
+  HIPCHK(c, hipMemcpyAsync(p, in, bytes, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  if (which == 0) {   // a written fine mesh: its occupied list as a deposit would leave it
+    std::vector<int> occ;
+    const float* f = in;
+    for (int t = 0; t < SCRATCH; t++) if (f[t * 4 + 3] > 0.0f) occ.push_back(t);
+    int n = (int)occ.size();
+    if (n) HIPCHK(c, hipMemcpyAsync(c->mesh[box].occ, occ.data(), occ.size() * 4, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->mesh[box].occN, &n, 4, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+  }
+  return 0;
+}
+
+int cc_slots(cc_ctx* c) {
+  if (!c) return CC_EINVAL;
+  launch_slots(c->pos[c->cur], c->N, (float)(1.0 / c->p.cell), c->grid, c->gb, c->s);
+  LAUNCHCHK(c);
+  return 0;
+}
+
+int cc_read_seats(cc_ctx* c, uint32_t* seatA, uint32_t* seatB) {
+  if (!c || !seatA || !seatB) return CC_EINVAL;
+  std::vector<uint32_t> s(c->cells * SLOTS);
+  HIPCHK(c, hipMemcpyAsync(s.data(), c->gb.seats, s.size() * 4, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  // a cell's eight words are its even seats (atlas A's four channels) then its odd ones (atlas B's)
+  for (size_t h = 0; h < c->cells; h++)
+    for (int k = 0; k < 4; k++) { seatA[h * 4 + k] = s[h * 8 + k]; seatB[h * 4 + k] = s[h * 8 + 4 + k]; }
+  return 0;
+}
+
+int cc_deposit(cc_ctx* c, int box) {
+  if (!c || box < 0 || box > 1) return CC_EINVAL;
+  launch_deposit(c->pos[c->cur], c->mat, c->N, c->box, box, (float)(FH * FH), c->mesh[box], c->s);
+  LAUNCHCHK(c);
+  return 0;
+}
+int cc_coarse(cc_ctx* c, int box) {
+  if (!c || box < 0 || box > 1) return CC_EINVAL;
+  launch_coarse(c->mesh[box], c->s);
+  LAUNCHCHK(c);
+  return 0;
+}
+int cc_cell(cc_ctx* c, int box) {
+  if (!c || box < 0 || box > 1) return CC_EINVAL;
+  launch_cell(c->mesh[box], (float)(0.5 * FH * FH), (float)c->p.m, c->s);
+  LAUNCHCHK(c);
+  return 0;
+}
+static int mesh_all(cc_ctx* c) {
+  int r;
+  if ((r = cc_deposit(c, 0)) || (r = cc_coarse(c, 0)) || (r = cc_cell(c, 0))) return r;
+  if (c->box.boxB > 0.5f && ((r = cc_deposit(c, 1)) || (r = cc_coarse(c, 1)) || (r = cc_cell(c, 1)))) return r;
+  return 0;
+}
+int cc_mesh(cc_ctx* c) {
+  if (!c) return CC_EINVAL;
+  return mesh_all(c);
+}
+
+int cc_set_pp_table(cc_ctx* c, const float* tab) {
+  if (!c || !tab) return CC_EINVAL;
+  HIPCHK(c, hipMemcpyAsync(c->ppTab, tab, 64 * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_pp(cc_ctx* c) {
+  if (!c) return CC_EINVAL;
+  if (!c->ppCount) return 0;
+  launch_pp(c->pos[c->cur], c->mat, c->box, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s);
+  LAUNCHCHK(c);
+  return 0;
+}
+
+int cc_set_pp(cc_ctx* c, const int32_t* list, int count) {
+  if (!c || count < 0 || count > PP_CAP || (count > 0 && !list)) FAIL(c, CC_EINVAL, "loose list of %d (cap %d)", count, PP_CAP);
+  std::vector<int32_t> idx(PP_CAP, -1);
+  for (int k = 0; k < count; k++) {
+    if (list[k] < 0 || list[k] >= c->N) FAIL(c, CC_EINVAL, "loose list entry %d = %d out of range", k, list[k]);
+    idx[k] = list[k];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->ppIndex, idx.data(), PP_CAP * 4, hipMemcpyHostToDevice, c->s));
+  launch_pp_slot(c->N, c->ppIndex, count, c->ppSlot, c->s);
+  LAUNCHCHK(c);
+  c->ppCount = count;
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return count ? cc_pp(c) : 0;
+}
+
+int cc_read_pp(cc_ctx* c, float* list, float* box, float* force) {
+  if (!c) return CC_EINVAL;
+  if (list) HIPCHK(c, hipMemcpyAsync(list, c->ppList, PP_CAP * 16, hipMemcpyDeviceToHost, c->s));
+  if (box) HIPCHK(c, hipMemcpyAsync(box, c->ppBox, PP_CAP * 4, hipMemcpyDeviceToHost, c->s));
+  if (force) HIPCHK(c, hipMemcpyAsync(force, c->ppForce, 4 * PP_CAP * 16, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+static int sim_pass(cc_ctx* c, const cc_sim_args* a) {
+  SimU u = sim_uniforms(c, a);
+  int s = c->cur, d = c->cur ^ 1;
+  launch_sim(u, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot, c->ppForce,
+             c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
+  LAUNCHCHK(c);
+  c->cur = d;
+  return 0;
+}
+
+int cc_sim(cc_ctx* c, const cc_sim_args* a) {
+  if (!c || !a) return CC_EINVAL;
+  return sim_pass(c, a);
+}
+
+static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
+  int n = c->stepNo++;
+  int r;
+  if (n % 2 == 0 && (r = cc_slots(c))) return r;
+  bool mesh = n % c->p.meshEvery == 0;
+  if (mesh && (r = mesh_all(c))) return r;
+  if (mesh && (r = cc_pp(c))) return r;
+  cc_sim_args b = *a;
+  b.settle = settle;
+  b.kick = kick;
+  b.gdt = mesh ? c->p.meshEvery * c->p.dt : 0.0;
+  return sim_pass(c, &b);
+}
+
+int cc_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
+  if (!c || !a) return CC_EINVAL;
+  return one_step(c, settle, kick, a);
+}
+
+int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nBoxL) {
+  if (!c || !a || n < 0) return CC_EINVAL;
+  HIPCHK(c, hipEventRecord(c->ev0, c->s));
+  int used = 0;
+  for (int i = 0; i < n; i++) {
+    if (c->stepNo % c->p.meshEvery == 0 && boxL && used < nBoxL) {
+      const double* L = boxL + 7 * used++;
+      cc_set_boxes(c, L, L + 3, L[6] > 0.5);
+    }
+    int r = one_step(c, a->settle, a->kick, a);
+    if (r) return r;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->s));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->lastMs = ms;
+  return 0;
+}
+
+int cc_get_step_no(cc_ctx* c, int* stepNo) {
+  if (!c || !stepNo) return CC_EINVAL;
+  *stepNo = c->stepNo;
+  return 0;
+}
+int cc_set_step_no(cc_ctx* c, int stepNo) {
+  if (!c) return CC_EINVAL;
+  c->stepNo = stepNo;
+  return 0;
+}
+
+int cc_rigid(cc_ctx* c, const double bulk0[3], const double bulk1[3], const double cen0[3], const double cen1[3],
+             const double rot0[9], const double rot1[9], double dtF) {
+  if (!c) return CC_EINVAL;
+  float b0[3], b1[3], c0[3], c1[3], s0[3], s1[3], r0[9], r1[9];
+  for (int k = 0; k < 3; k++) {
+    b0[k] = (float)bulk0[k]; b1[k] = (float)bulk1[k]; c0[k] = (float)cen0[k]; c1[k] = (float)cen1[k];
+    s0[k] = (float)c->p.spin0[k]; s1[k] = (float)c->p.spin1[k];
+  }
+  for (int k = 0; k < 9; k++) { r0[k] = (float)rot0[k]; r1[k] = (float)rot1[k]; }
+  int s = c->cur, d = c->cur ^ 1;
+  launch_rigid(c->N, c->pos[s], c->vel[s], c->aux[s], c->q[s], b0, b1, c0, c1, s0, s1, r0, r1, (float)dtF, c->pos[d],
+               c->vel[d], c->aux[d], c->q[d], c->s);
+  LAUNCHCHK(c);
+  c->cur = d;
+  return 0;
+}
+
+int cc_diag(cc_ctx* c, float* out) {
+  if (!c) return CC_EINVAL;
+  SimU u = sim_uniforms(c, nullptr);
+  launch_diag(u, c->pos[c->cur], c->q[c->cur], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot, c->ppForce, c->diag,
+              c->s);
+  LAUNCHCHK(c);
+  if (out) {
+    HIPCHK(c, hipMemcpyAsync(out, c->diag, (size_t)c->N * 16, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+  }
+  return 0;
+}
+
+int cc_morton(cc_ctx* c, const double core[3], int32_t* perm, int32_t* into) {
+  if (!c || !core) return CC_EINVAL;
+  int s = c->cur, d = c->cur ^ 1;
+  launch_morton(c->pos[s], c->N, core[0], core[1], core[2], c->mkeys, c->mslot, c->mhist, c->mblock, c->mperm,
+                c->minto, c->s);
+  launch_permute(c->N, c->mperm, c->pos[s], c->vel[s], c->aux[s], c->q[s], c->mat, c->pos[d], c->vel[d], c->aux[d],
+                 c->q[d], c->matTmp, c->s);
+  LAUNCHCHK(c);
+  std::swap(c->mat, c->matTmp);
+  c->cur = d;
+  int r = cc_slots(c);   // applyPerm rebuilds the seats: the old ones name particles by their old places
+  if (r) return r;
+  if (perm) HIPCHK(c, hipMemcpyAsync(perm, c->mperm, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->s));
+  if (into) HIPCHK(c, hipMemcpyAsync(into, c->minto, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_books(cc_ctx* c, double* out) {
+  if (!c || !out) return CC_EINVAL;
+  launch_books(c->N, c->p.m, c->pos[c->cur], c->vel[c->cur], c->mat, c->diag, c->bpart, c->bout, c->s);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(out, c->bout, 13 * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_sync(cc_ctx* c) {
+  if (!c) return CC_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_last_step_ms(cc_ctx* c, double* ms) {
+  if (!c || !ms) return CC_EINVAL;
+  *ms = c->lastMs;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Internal types shared by the kernels (cc_kernels.hip) and the C-ABI
+// (cc_api.hip). Device layouts (HBM):
+//   particle state : pos/vel/aux float4[N], q float[N] -- two copies (src/dst),
+//                    mat float[N]; index i = y*W + x of the reference's W x H
+//                    state textures.
+//   contact grid   : seats uint32[cells*8] -- the eight seat words of hashed
+//                    cell h (h = atlas texel y*AW + x) contiguous, so one cell
+//                    is two 16-byte loads; cnt int[cells], bucket int[cells*CAP]
+//                    (arrival lists of the current rebuild), occupied-cell
+//                    lists (two, alternating), an overflow list.
+//   mesh (per box) : acc int64[513*512*4] fixed-point deposit accumulator,
+//                    fine/force float4[513*512] in the reference atlas layout
+//                    (8x8 slices of 64x64, scratch texel (0,512)),
+//                    coarse/farA/farT0/farT1 float4[64*64], mono float4.
+//   loose list     : index int[8192], list float4[8192], box float[8192],
+//                    force float4[4*8192] (slice s of seat j at s*8192+j),
+//                    slot float[N] (seat of particle, or -1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cc {
+
+constexpr int SLOTS = 8;             // seats per contact cell
+constexpr int BUCKET_CAP = 16;       // arrivals kept per cell before the overflow list
+constexpr int PP_CAP = 8192;
+constexpr int FINE_TEXELS = 513 * 512;
+constexpr int SCRATCH = 512 * 512;   // texel (0, 512)
+constexpr uint32_t EMPTY_SEAT = 0xBF800000u;
+constexpr double FL = 6.0;
+constexpr double FH = 2.0 * FL / 64.0;
+// fixed-point scales of the deposit accumulator: the mesh sums (weights up to
+// a few thousand, mass-weighted positions up to a few 1e4) and the strays'
+// pull in the scratch texel (bounded by N * 2.5 / (0.385 FH^2) < 2e7)
+constexpr double DEP_SCALE = 1099511627776.0;     // 2^40
+constexpr double SCR_SCALE = 16777216.0;          // 2^24
+
+struct Grid {
+  int G, GShift, GMask, SX, SXShift, SXMask, AW, AH;
+};
+
+struct Box {            // the two mesh boxes as the shaders see them
+  float invH;
+  float LA[3], LB[3];
+  float boxB;
+};
+
+struct SimU {           // the step pass's uniforms, already FP32
+  int N;
+  float invCell, cell, dt, gdt, touch, link2, reach2, k, c, invM, damp, creep, tK, kappa, cpIron;
+  float settle, kick;
+  float bulk0[3], bulk1[3], cen0[3], cen1[3], spin0[3], spin1[3];
+  Grid grid;
+  Box box;
+  float gm, eps2;       // G m, monopole softening FH^2
+  int field;            // 1: constant gravity gField
+  float gField[3];
+  int frozenOn;
+  float frozen;
+};
+
+struct MeshBuf {
+  unsigned long long* acc;   // int64 two's complement, [FINE_TEXELS*4]
+  float4* fine;
+  float4* force;
+  float4* coarse;
+  float4* farA;
+  float4* farT0;
+  float4* farT1;
+  float4* mono;
+  int* occ;                  // occupied fine texels of the last deposit
+  int* occN;
+};
+
+struct GridBuf {
+  uint32_t* seats;           // [cells*8]
+  int* cnt;                  // [cells]
+  int* bucket;               // [cells*BUCKET_CAP]
+  int* occ[2];               // occupied cells, two lists used in turn
+  int* occN;                 // [2]
+  int2* ovf;                 // (cell, particle) past a cell's bucket
+  int* ovfN;
+  uint32_t* word;            // [N] seat word of each particle
+  int cur;                   // which occupied list is the current one
+};
+
+// ---- launches (cc_kernels.hip) ----
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
+void launch_deposit(const float4* pos, const float* mat, int N, const Box& b, int box, float eps2,
+                    MeshBuf& m, hipStream_t s);
+void launch_coarse(MeshBuf& m, hipStream_t s);
+void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s);
+void launch_pp(const float4* pos, const float* mat, const Box& b, const int* index, int count, float gm,
+               float cut, float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s);
+void launch_sim(const SimU& u, const float4* pos, const float4* vel, const float* mat, const float* q,
+                const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
+                const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
+void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux, const float* q,
+                  const float* bulk0, const float* bulk1, const float* cen0, const float* cen1,
+                  const float* spin0, const float* spin1, const float* rot0, const float* rot1, float dt,
+                  float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
+void launch_diag(const SimU& u, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
+                 const MeshBuf& m1, const float* ppSlot, const float4* ppForce, float4* out, hipStream_t s);
+void launch_pp_slot(int N, const int* index, int count, float* slot, hipStream_t s);
+// Morton order: keys from FP64 arithmetic, a counting sort over 2^21 keys made
+// stable by ranking each key's members by index; perm[dst] = src, into[src] = dst
+void launch_morton(const float4* pos, int N, double cx, double cy, double cz, int* keys, int* slot, int* hist,
+                   int* blockSums, int* perm, int* into, hipStream_t s);
+void launch_permute(int N, const int* perm, const float4* pos, const float4* vel, const float4* aux, const float* q,
+                    const float* mat, float4* oPos, float4* oVel, float4* oAux, float* oQ, float* oMat,
+                    hipStream_t s);
+void launch_books(int N, double mref, const float4* pos, const float4* vel, const float* mat, const float4* diag,
+                  double* partial, double* out, hipStream_t s);
+
+}  // namespace cc

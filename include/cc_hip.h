@@ -1,0 +1,128 @@
+/* cc_hip.h -- C ABI of the MI355X (gfx950) simulation step of cosmic-collisions.
+ *
+ * The reference drives its step as WebGL2 passes from the engine object
+ * CC.nbody (src/nbody.js:1095-1118).  This library is what that engine's
+ * GL layer (src/gl.js + src/glsl-sim.js) becomes: every pass is an entry point
+ * here, and a host engine (the JavaScript engine object in
+ * cosmic-collisions_amd/js/, over the N-API addon) keeps the reference's
+ * schedule, body setup and read-out around it.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative CC_E* code; the
+ *    message is cc_last_error(ctx) (ctx may be NULL for create-time errors).
+ *  - Host arrays are owned by the caller and copied; device memory by the ctx.
+ *  - Particle arrays use the reference's texture order (index i = y*W + x):
+ *    pos/vel/aux float[N*4], q/mat float[N].  Scalars that the reference
+ *    computes in JavaScript numbers are passed as double and rounded to
+ *    float exactly where WebGL's uniform upload rounds them.
+ *  - Calls on one ctx are not thread-safe; all work is queued on the ctx's
+ *    stream, and functions that return data wait for it.
+ */
+#ifndef CC_HIP_H
+#define CC_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CC_OK 0
+#define CC_EINVAL -1     /* bad argument or size */
+#define CC_EDEVICE -2    /* HIP runtime error */
+#define CC_ESTATE -3     /* call out of order */
+
+typedef struct cc_ctx cc_ctx;
+
+/* The run's constants: sim fields of build()/soften() (src/nbody.js:114-250,
+ * :545-550) and the static uniforms setStaticUniforms() derives from them
+ * (src/nbody.js:255-336). */
+typedef struct {
+  int N, W, H;             /* particles; state texture width/height (N = W*H) */
+  int G;                   /* contact grid: 64 or 128 cells a side (src/nbody.js:209) */
+  double m;                /* mass of a mantle particle (uGm, 1/uInvM) */
+  double a;                /* particle radius (uTouch = 2a, uA2 = a^2) */
+  double cell;             /* contact cell edge, 2.6a */
+  double dt, k, c, creep, damp, tK, kappa;
+  double cpIron;           /* CP_IRON / CP_ROCK */
+  double seatSlack;        /* SEAT_SLACK = 0.6 radii (src/nbody.js:41) */
+  double spin0[3], spin1[3];
+  int field;               /* 1: constant gravity gField (chicxulub.html:235) */
+  double gField[3];
+  int frozenOn;            /* 1: material `frozen` held still */
+  double frozen;
+  int meshEvery;           /* MESH_EVERY (src/nbody.js:538), 8 */
+} cc_params;
+
+/* The per-pass host values of passSim (src/nbody.js:458-484). */
+typedef struct {
+  int settle, kick;
+  double gdt;
+  double bulk0[3], bulk1[3], cen0[3], cen1[3];
+} cc_sim_args;
+
+/* src/nbody.js:1032-1092 init + build's allocations; replaces CC.gl's
+ * simTarget/makeGrid/floatTex (src/gl.js:29-83) for the physics. */
+int cc_create(const cc_params* p, cc_ctx** out);
+int cc_destroy(cc_ctx* ctx);
+const char* cc_last_error(cc_ctx* ctx);
+/* setStaticUniforms after soften() (src/nbody.js:545-550, :255-336) */
+int cc_set_params(cc_ctx* ctx, const cc_params* p);
+/* texImage2D/texSubImage2D of the state textures (src/nbody.js:217-219) */
+int cc_upload_state(cc_ctx* ctx, const float* pos, const float* vel, const float* mat, const float* q);
+/* readPixels of the state (src/nbody.js:987-990); any pointer may be NULL */
+int cc_download_state(cc_ctx* ctx, float* pos, float* vel, float* aux, float* q, float* mat);
+/* boxes() (src/nbody.js:342-350): uLA, uLB as computed there, uBoxB */
+int cc_set_boxes(cc_ctx* ctx, const double LA[3], const double LB[3], int boxB);
+/* placeBoxes' priming of the second box's mono texel (src/nbody.js:365-367); also any mesh texture by name */
+int cc_set_mono(cc_ctx* ctx, int box, const float mono[4]);
+/* passSlots (src/nbody.js:379-399, SLOT_VS/FS src/glsl-sim.js:118-156) */
+int cc_slots(cc_ctx* ctx);
+/* the two seat atlases in the reference's RGBA32F layout, AW*AH*4 words each */
+int cc_read_seats(cc_ctx* ctx, uint32_t* seatA, uint32_t* seatB);
+/* passDeposit / passCoarse / passCell / passMesh (src/nbody.js:402-455) */
+int cc_deposit(cc_ctx* ctx, int box);
+int cc_coarse(cc_ctx* ctx, int box);
+int cc_cell(cc_ctx* ctx, int box);
+int cc_mesh(cc_ctx* ctx);
+/* mesh textures: which = 0 fine (513x512x4), 1 coarse (64x64x4), 2 mono (4),
+ * 3 farA, 4 farT0, 5 farT1 (64x64x4), 6 force (513x512x4) */
+int cc_read_mesh(cc_ctx* ctx, int box, int which, float* out);
+int cc_write_mesh(cc_ctx* ctx, int box, int which, const float* in);
+/* PP_TAB (src/nbody.js:109, :1080-1082): 64 x {force r^2, potential r} */
+int cc_set_pp_table(cc_ctx* ctx, const float* tab);
+/* setPP (src/nbody.js:957-966): the loose list (particle indices), count <= 8192; runs passPP if count > 0 */
+int cc_set_pp(cc_ctx* ctx, const int32_t* list, int count);
+/* passPP (src/nbody.js:511-528) */
+int cc_pp(cc_ctx* ctx);
+/* pp.list (8192x4), pp.box (8192), pp.force (4 slices x 8192 x 4, slice-major); any may be NULL */
+int cc_read_pp(cc_ctx* ctx, float* list, float* box, float* force);
+/* passSim (src/nbody.js:458-484): one step pass, state src -> dst, swap */
+int cc_sim(cc_ctx* ctx, const cc_sim_args* a);
+/* step (src/nbody.js:551-558): seats every 2nd step, mesh + pairwise every
+ * meshEvery-th with gravity as one meshEvery*dt impulse.  box_L is the
+ * boxes() uniform for the mesh step (NULL: keep the last); stepNo advances. */
+int cc_step(cc_ctx* ctx, int settle, int kick, const cc_sim_args* a);
+/* n steps without settle or kick; boxL[n_mesh][7] = {LA, LB, boxB} for each mesh step in order (NULL: keep) */
+int cc_step_n(cc_ctx* ctx, int n, const cc_sim_args* a, const double* boxL, int nBoxL);
+int cc_get_step_no(cc_ctx* ctx, int* stepNo);
+int cc_set_step_no(cc_ctx* ctx, int stepNo);
+/* rigidStep's GPU half (RIGID_FS, src/glsl-sim.js:537-570): rot column-major */
+int cc_rigid(cc_ctx* ctx, const double bulk0[3], const double bulk1[3], const double cen0[3], const double cen1[3],
+             const double rot0[9], const double rot1[9], double dtF);
+/* DIAG_FS alone (src/glsl-sim.js:608-667) on the current seats/mesh/pp; out N*4 (NULL: keep on device) */
+int cc_diag(cc_ctx* ctx, float* out);
+/* Morton order (src/nbody.js:871-887) from the analysis' core centre, applied
+ * to the state (PERM_FS + applyPerm, src/nbody.js:932-954) and the seats
+ * rebuilt; perm[dst] = src and into[src] = dst returned (either may be NULL) */
+int cc_morton(cc_ctx* ctx, const double core[3], int32_t* perm, int32_t* into);
+/* the FP64 books over the state and the last DIAG (src/nbody.js:831-850):
+ * out[13] = M, px, py, pz, Jx, Jy, Jz, KE, PE, Q, iron mass, unseated count, full count */
+int cc_books(cc_ctx* ctx, double* out);
+/* wait for the ctx's stream */
+int cc_sync(cc_ctx* ctx);
+/* device time of the last cc_step_n call, ms (HIP events on the ctx's stream) */
+int cc_last_step_ms(cc_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
