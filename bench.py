@@ -142,14 +142,18 @@ TWINS_OPT = {"nIdx": 4, "q": 1.0, "dens": 0.75, "core": 0.325, "spin0": 0.18, "s
              "vfac": 1.05, "rest": 0.3}
 
 
-def prepare_twins(out_dir, extra_steps=0):
+def prepare_twins(out_dir, extra_steps=0, reuse=False):
     """The twins run through the JavaScript engine's own phases (build, settle,
-    approach, kick, a look with the Morton sort) on this GPU, written to out_dir."""
-    cmd = ["node", os.path.join(PKG, "js", "prep_state.js"), json.dumps(TWINS_OPT), out_dir, str(extra_steps)]
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-    sys.stderr.write(r.stdout[-3000:])
-    if r.returncode != 0:
-        raise RuntimeError("state preparation failed:\n" + r.stderr[-4000:])
+    approach, kick, a look with the Morton sort) on this GPU, written to out_dir
+    (or read from there when reuse is set and a run for these options is there)."""
+    have = os.path.exists(os.path.join(out_dir, "run.json")) and \
+        json.load(open(os.path.join(out_dir, "run.json"))).get("opt") == TWINS_OPT
+    if not (reuse and have):
+        cmd = ["node", os.path.join(PKG, "js", "prep_state.js"), json.dumps(TWINS_OPT), out_dir, str(extra_steps)]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        sys.stderr.write(r.stdout[-3000:])
+        if r.returncode != 0:
+            raise RuntimeError("state preparation failed:\n" + r.stderr[-4000:])
     run = json.load(open(os.path.join(out_dir, "run.json")))
     N = run["sim"]["N"]
     st = {k: np.fromfile(os.path.join(out_dir, "state.%s.f32" % k), np.float32) for k in ("pos", "vel", "q", "mat")}
@@ -217,13 +221,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--state-dir", default=os.path.join(REPO, "gpurun_out", "twins_state"))
+    ap.add_argument("--reuse-state", action="store_true", help="use a prepared state in --state-dir if there is one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)
         return 2
     import ccamd
 
-    run, st = prepare_twins(args.state_dir)
+    run, st = prepare_twins(args.state_dir, reuse=args.reuse_state)
     import torch
     R = Runner(ccamd, run, st)
     N = run["sim"]["N"]

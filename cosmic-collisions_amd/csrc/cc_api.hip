@@ -30,6 +30,7 @@ struct cc_ctx {
   float4* ppList = nullptr;
   float* ppBox = nullptr;
   float4* ppForce = nullptr;
+  float4* ppPartial = nullptr;
   float* ppSlot = nullptr;
   float2* ppTab = nullptr;
   int ppCount = 0;
@@ -178,6 +179,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     }
     ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
+    ALLOC(c->ppPartial, (size_t)4 * (PP_CAP / 4 / 256) * PP_CAP * 16);
     ALLOC(c->diag, N * 16);
     ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
     ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4);
@@ -374,7 +376,7 @@ int cc_pp(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   if (!c->ppCount) return 0;
   launch_pp(c->pos[c->cur], c->mat, c->box, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
-            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s);
+            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppPartial, c->ppForce, c->s);
   LAUNCHCHK(c);
   return 0;
 }

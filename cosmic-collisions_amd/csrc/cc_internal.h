@@ -93,7 +93,8 @@ void launch_deposit(const float4* pos, const float* mat, int N, const Box& b, in
 void launch_coarse(MeshBuf& m, hipStream_t s);
 void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s);
 void launch_pp(const float4* pos, const float* mat, const Box& b, const int* index, int count, float gm,
-               float cut, float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s);
+               float cut, float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force,
+               hipStream_t s);
 void launch_sim(const SimU& u, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
                 const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);

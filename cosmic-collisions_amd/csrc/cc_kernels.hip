@@ -14,6 +14,7 @@
 //   * the loops that one fragment ran alone (the far field, the per-cell
 //     near field) are spread over the 64 lanes of a wavefront and reduced
 //     with cross-lane shuffles.
+#include <cmath>
 #include <cstring>
 
 #include "cc_internal.h"
@@ -209,23 +210,17 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
 
 // One particle per lane: its eight cloud-in-cell corners into its own box's
 // mesh, or -- a stray in neither box -- its pull on the mesh into the scratch
-// texel (src/glsl-sim.js:168-219).  In Morton order the 64 particles of a
-// wavefront mostly share one base cell: the lanes of a base cell are summed
-// across the wavefront (fixed point, so the sums are exact and independent
-// of order) and one lane makes the 32 atomic adds for them; after a few
-// distinct base cells the rest go lane by lane.
-constexpr int DEP_SEGMENTS = 4;
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ float corner_w(int k, float tx, float ty, float tz, float mr) {
-  int o0 = k & 1, o1 = (k >> 1) & 1, o2 = k >> 2;
-  return gmix(1.0f - tx, tx, (float)o0) * gmix(1.0f - ty, ty, (float)o1) * gmix(1.0f - tz, tz, (float)o2) * mr;
-}
+// texel (src/glsl-sim.js:168-219).  The work-group's corners are privatised
+// in an LDS tile when its particles fall in a small block of cells (Morton
+// order makes that the common case); each touched cell is flushed to the
+// mesh with one atomic add per channel.  Fixed-point int64 throughout, so
+// the sums are exact and independent of the order the adds land in.
+constexpr int TILE = 8;
 __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos, const float* __restrict__ mat, int N,
                                                  Box b, int which, const float4* __restrict__ mono, float eps2,
                                                  unsigned long long* acc) {
+  __shared__ unsigned long long tile[TILE * TILE * TILE * 4];
+  __shared__ int lo[3], hi[3];
   int i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < N;
   float4 p = live ? pos[i] : make_float4(0, 0, 0, 0);
@@ -240,43 +235,36 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
     tx = gx - fx; ty = gy - fy; tz = gz - fz;
     cx = (int)fx; cy = (int)fy; cz = (int)fz;
   }
-  int key = dep ? (cz * 64 + cy) * 64 + cx : -1;
-  int lane = (int)__lane_id();
-  bool todo = dep;
-  for (int seg = 0; seg < DEP_SEGMENTS; seg++) {
-    unsigned long long mask = __ballot(todo);
-    if (mask == 0ull) break;
-    int leader = __ffsll((long long)mask) - 1;
-    int lkey = __shfl(key, leader, 64);
-    bool mt = todo && key == lkey;
-    int l0 = lkey & 63, l1 = (lkey >> 6) & 63, l2 = lkey >> 12;
+  if (threadIdx.x < 3) { lo[threadIdx.x] = 1 << 20; hi[threadIdx.x] = -(1 << 20); }
+  for (int t = threadIdx.x; t < TILE * TILE * TILE * 4; t += blockDim.x) tile[t] = 0ull;
+  __syncthreads();
+  if (dep) {
+    atomicMin(&lo[0], cx); atomicMin(&lo[1], cy); atomicMin(&lo[2], cz);
+    atomicMax(&hi[0], cx); atomicMax(&hi[1], cy); atomicMax(&hi[2], cz);
+  }
+  __syncthreads();
+  int ox = lo[0], oy = lo[1], oz = lo[2];
+  // corners span [lo, hi+1]: the tile holds them when that is within TILE
+  bool tiled = hi[0] + 1 - ox < TILE && hi[1] + 1 - oy < TILE && hi[2] + 1 - oz < TILE;
+  if (dep) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      int c0 = l0 + (k & 1), c1 = l1 + ((k >> 1) & 1), c2 = l2 + (k >> 2);
-      float w = mt ? corner_w(k, tx, ty, tz, mr) : 0.0f;
-      unsigned long long v0 = wave_sum_u64(mt ? to_fixed(p.x * w, DEP_SCALE) : 0ull);
-      unsigned long long v1 = wave_sum_u64(mt ? to_fixed(p.y * w, DEP_SCALE) : 0ull);
-      unsigned long long v2 = wave_sum_u64(mt ? to_fixed(p.z * w, DEP_SCALE) : 0ull);
-      unsigned long long v3 = wave_sum_u64(mt ? to_fixed(1.0f * w, DEP_SCALE) : 0ull);
-      if (lane == 0 && c0 <= 63 && c1 <= 63 && c2 <= 63) {
+      int o0 = k & 1, o1 = (k >> 1) & 1, o2 = k >> 2;
+      int c0 = cx + o0, c1 = cy + o1, c2 = cz + o2;
+      if (c0 > 63 || c1 > 63 || c2 > 63) continue;
+      float w = gmix(1.0f - tx, tx, (float)o0) * gmix(1.0f - ty, ty, (float)o1) * gmix(1.0f - tz, tz, (float)o2) * mr;
+      unsigned long long v0 = to_fixed(p.x * w, DEP_SCALE), v1 = to_fixed(p.y * w, DEP_SCALE);
+      unsigned long long v2 = to_fixed(p.z * w, DEP_SCALE), v3 = to_fixed(1.0f * w, DEP_SCALE);
+      if (tiled) {
+        int l = ((c2 - oz) * TILE + (c1 - oy)) * TILE + (c0 - ox);
+        atomicAdd(&tile[l * 4], v0); atomicAdd(&tile[l * 4 + 1], v1);
+        atomicAdd(&tile[l * 4 + 2], v2); atomicAdd(&tile[l * 4 + 3], v3);
+      } else {
         unsigned long long* d = acc + (size_t)fine_at(c0, c1, c2) * 4;
         atomicAdd(d, v0); atomicAdd(d + 1, v1); atomicAdd(d + 2, v2); atomicAdd(d + 3, v3);
       }
     }
-    todo = todo && !mt;
-  }
-  if (todo) {
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      int c0 = cx + (k & 1), c1 = cy + ((k >> 1) & 1), c2 = cz + (k >> 2);
-      if (c0 > 63 || c1 > 63 || c2 > 63) continue;
-      float w = corner_w(k, tx, ty, tz, mr);
-      unsigned long long* d = acc + (size_t)fine_at(c0, c1, c2) * 4;
-      atomicAdd(d, to_fixed(p.x * w, DEP_SCALE)); atomicAdd(d + 1, to_fixed(p.y * w, DEP_SCALE));
-      atomicAdd(d + 2, to_fixed(p.z * w, DEP_SCALE)); atomicAdd(d + 3, to_fixed(1.0f * w, DEP_SCALE));
-    }
-  }
-  if (live && mine < 0) {
+  } else if (live && mine < 0) {
     float4 mo = mono[0];
     float rx = mo.x / gmax(mo.w, 1e-6f) - p.x, ry = mo.y / gmax(mo.w, 1e-6f) - p.y, rz = mo.z / gmax(mo.w, 1e-6f) - p.z;
     float inv = rsq(dot3(rx, ry, rz, rx, ry, rz) + eps2);
@@ -284,6 +272,16 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
     unsigned long long* d = acc + (size_t)SCRATCH * 4;
     atomicAdd(d, to_fixed(rx * s, SCR_SCALE)); atomicAdd(d + 1, to_fixed(ry * s, SCR_SCALE));
     atomicAdd(d + 2, to_fixed(rz * s, SCR_SCALE)); atomicAdd(d + 3, to_fixed(mr, SCR_SCALE));
+  }
+  __syncthreads();
+  if (!tiled) return;
+  for (int l = threadIdx.x; l < TILE * TILE * TILE; l += blockDim.x) {
+    unsigned long long w = tile[l * 4 + 3], a0 = tile[l * 4], a1 = tile[l * 4 + 1], a2 = tile[l * 4 + 2];
+    if ((w | a0 | a1 | a2) == 0ull) continue;
+    int c0 = ox + l % TILE, c1 = oy + (l / TILE) % TILE, c2 = oz + l / (TILE * TILE);
+    if (c0 > 63 || c1 > 63 || c2 > 63) continue;
+    unsigned long long* d = acc + (size_t)fine_at(c0, c1, c2) * 4;
+    atomicAdd(d, a0); atomicAdd(d + 1, a1); atomicAdd(d + 2, a2); atomicAdd(d + 3, w);
   }
 }
 
@@ -467,33 +465,40 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
   return t * t * (3.0f - 2.0f * t);
 }
 
-// Seat x slice: the slice's quarter of the list, in list order, staged
-// through LDS 256 entries at a time (src/glsl-sim.js:712-754).
+// Each reference slice (a quarter of the list, src/glsl-sim.js:712-754) is cut
+// into chunks of PP_CHUNK entries: a thread sums one seat against one chunk,
+// staged through LDS, and k_pp_sum adds a slice's chunks in order.  Four
+// slices alone would be 512 wavefronts; chunks make it up to 4096.
+constexpr int PP_CHUNK = 256;
+constexpr int PP_MAXCHUNK = PP_CAP / 4 / PP_CHUNK;   // 8
 __global__ void __launch_bounds__(256) k_pp_force(const float4* __restrict__ list, const float* __restrict__ box,
-                                                  int count, float gm, float cut, float a2,
-                                                  const float2* __restrict__ tab, float4* force) {
-  __shared__ float4 sl[256];
-  __shared__ float sb[256];
+                                                  int count, float cut, float a2, const float2* __restrict__ tab,
+                                                  float4* partial) {
+  __shared__ float4 sl[PP_CHUNK];
+  __shared__ float sb[PP_CHUNK];
   __shared__ float2 st[64];
-  int slice = blockIdx.y;
+  int slice = blockIdx.y, chunk = blockIdx.z;
   int me = blockIdx.x * blockDim.x + threadIdx.x;
   if (threadIdx.x < 64) st[threadIdx.x] = tab[threadIdx.x];
   int share = (count + 3) >> 2, j0 = slice * share, j1 = min(j0 + share, count);
+  int c0 = j0 + chunk * PP_CHUNK, c1 = min(c0 + PP_CHUNK, j1);
+  float4* out = partial + ((size_t)slice * PP_MAXCHUNK + chunk) * PP_CAP;
+  if (c0 >= c1) {   // past the slice's end: an empty partial
+    if (me < PP_CAP) out[me] = make_float4(0, 0, 0, 0);
+    return;
+  }
   float4 P = me < PP_CAP ? list[me] : make_float4(0, 0, 0, 0);
   bool active = me < count && P.w != 0.0f;
   float myBox = me < PP_CAP ? box[me] : -1.0f;
+  int j = c0 + threadIdx.x;
+  if (j < c1) { sl[threadIdx.x] = list[j]; sb[threadIdx.x] = box[j]; }
+  __syncthreads();
   float cut2 = cut * cut, a = sqrtf(a2);
   float a0 = 0, a1 = 0, a2s = 0, pot = 0;
-  for (int base = j0; base < j1; base += 256) {
-    __syncthreads();
-    int j = base + threadIdx.x;
-    if (j < j1) { sl[threadIdx.x] = list[j]; sb[threadIdx.x] = box[j]; }
-    __syncthreads();
-    int nt = min(256, j1 - base);
-    if (!active) continue;
+  if (active) {
+    int nt = c1 - c0;
     for (int q = 0; q < nt; q++) {
-      int jj = base + q;
-      if (jj == me) continue;
+      if (c0 + q == me) continue;
       float4 Q = sl[q];
       float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
       float r2 = dot3(d0, d1, d2, d0, d1, d2);
@@ -513,14 +518,32 @@ __global__ void __launch_bounds__(256) k_pp_force(const float4* __restrict__ lis
       pot += mj * (-invN - Ty / gmax(r, a)) * taper;
     }
   }
-  if (me < PP_CAP)
-    force[slice * PP_CAP + me] = active ? make_float4(a0 * gm, a1 * gm, a2s * gm, pot * gm) : make_float4(0, 0, 0, 0);
+  if (me < PP_CAP) out[me] = make_float4(a0, a1, a2s, pot);
+}
+// a slice's chunks in order, times G m
+__global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ partial, const float4* __restrict__ list,
+                                                int count, float gm, float4* force) {
+  int me = blockIdx.x * blockDim.x + threadIdx.x, slice = blockIdx.y;
+  if (me >= PP_CAP) return;
+  bool active = me < count && list[me].w != 0.0f;
+  float4 s = make_float4(0, 0, 0, 0);
+  if (active) {
+    for (int c = 0; c < PP_MAXCHUNK; c++) {
+      float4 v = partial[((size_t)slice * PP_MAXCHUNK + c) * PP_CAP + me];
+      s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+    s = make_float4(s.x * gm, s.y * gm, s.z * gm, s.w * gm);
+  }
+  force[slice * PP_CAP + me] = s;
 }
 
 void launch_pp(const float4* pos, const float* mat, const Box& b, const int* index, int count, float gm, float cut,
-               float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s) {
+               float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force, hipStream_t s) {
   hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, b, index, list, box);
-  hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / 256, 4), dim3(256), 0, s, list, box, count, gm, cut, a2, tab, force);
+  int tb = (count + 255) / 256;   // seats past count only need the zero that k_pp_sum writes
+  if (tb < 1) tb = 1;
+  hipLaunchKernelGGL(k_pp_force, dim3(tb, 4, PP_MAXCHUNK), dim3(256), 0, s, list, box, count, cut, a2, tab, partial);
+  hipLaunchKernelGGL(k_pp_sum, dim3(PP_CAP / 256, 4), dim3(256), 0, s, partial, list, count, gm, force);
 }
 
 __global__ void k_pp_slot(int N, const int* __restrict__ index, int count, float* slot) {
@@ -562,102 +585,138 @@ __device__ __forceinline__ void cross_at(float4 mo, float4 mm, float gm, float e
 // particle's box (or the monopoles, or a constant field), then symplectic
 // Euler (src/glsl-sim.js:362-533).
 //
-// One lane per particle: its contact sums accumulate in the reference's own
-// order (cells dz, dy, dx; seats ascending), which keeps every partial sum
-// -- and so the result -- the reference's.  The pass is bound by the latency
-// of dependent fetches (seat -> position -> velocity), so each cell is done
-// in three stages: the seats' candidates are chosen and all their positions
-// requested together, then all the touching pairs' velocities and masses,
-// then the sums run in seat order.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_sim(SimU u, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                             const float* __restrict__ mat, const float* __restrict__ q,
-                                             const uint32_t* __restrict__ seats, const float4* __restrict__ force0,
-                                             const float4* __restrict__ mono0, const float4* __restrict__ force1,
-                                             const float4* __restrict__ mono1, const float* __restrict__ ppSlot,
-                                             const float4* __restrict__ ppForce, float4* oPos, float4* oVel,
-                                             float4* oAux, float* oQ) {
-  __shared__ float seatpos[16];   // where a seat's particle sat, in cells: (q + 0.5) / 15
+// One lane per particle, its contact sums in the reference's order (cells dz,
+// dy, dx; seats ascending), which keeps every partial sum the reference's.
+// A wavefront runs the union of its lanes' work, so the contacts are found
+// in three compacted phases instead of eight predicated seat slots per cell:
+//   A  the seats that pass the cell and seat-place prefilters, in order,
+//      into the lane's LDS list;
+//   B  their positions: linked pairs counted, touching pairs kept in order
+//      (r^2 < touch2r: the r^2 below which sqrt(r^2) rounds under the contact
+//      distance -- the reference's touch test without its square root);
+//   C  the touching pairs' velocities and masses, and the sums.
+// A lane whose list would overflow finishes alone on the direct path.
+constexpr int SIM_BLOCK = 64;
+constexpr int CAND_CAP = 40;
+
+struct Contact {
+  float pushx, pushy, pushz, nb, lost, flow;
+};
+
+// one candidate pair's share, exactly as the step pass computes it
+__device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, float pz, float vx, float vy, float vz,
+                                             float Ti, float ri, float ci, float4 Pj, float4 Vj, float rj,
+                                             Contact& a) {
+  float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
+  float r = sqrtf(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
+  float rr = gmax(r, 1e-6f);
+  float nx = ddx / rr, ny = ddy / rr, nz = ddz / rr;
+  float ov = u.touch - r;
+  float cij = u.c * sqrtf(2.0f * ri * rj / (ri + rj));
+  float vn = dot3(Vj.x - vx, Vj.y - vy, Vj.z - vz, nx, ny, nz);
+  float fs = u.k * ov - cij * vn, fn = gmax(fs, 0.0f), on = fs < 0.0f ? 0.0f : 1.0f;
+  a.pushx -= nx * fn; a.pushy -= ny * fn; a.pushz -= nz * fn;
+  a.lost += cij * vn * vn * on + u.k * ov * vn * (1.0f - on);
+  float cj = rj * (mod4(Pj.w) < 0.5f ? u.cpIron : 1.0f);
+  a.flow += (Vj.w - Ti) * (2.0f * ci * cj / (ci + cj));
+}
+
+__device__ __forceinline__ bool seat_passes(uint32_t sv, int me, float lox, float loy, float loz, float px, float py,
+                                            float pz, const float* seatpos, const SimU& u) {
+  int j = (int)(sv & 0x3FFFFu);
+  float dqx = lox + seatpos[(sv >> 18) & 15u] * u.cell - px, dqy = loy + seatpos[(sv >> 22) & 15u] * u.cell - py,
+        dqz = loz + seatpos[(sv >> 26) & 15u] * u.cell - pz;
+  return j != me && !(dot3(dqx, dqy, dqz, dqx, dqy, dqz) > u.reach2);
+}
+
+__global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, float touch2r, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const float* __restrict__ mat,
+                                                   const float* __restrict__ q, const uint32_t* __restrict__ seats,
+                                                   const float4* __restrict__ force0, const float4* __restrict__ mono0,
+                                                   const float4* __restrict__ force1, const float4* __restrict__ mono1,
+                                                   const float* __restrict__ ppSlot,
+                                                   const float4* __restrict__ ppForce, float4* oPos, float4* oVel,
+                                                   float4* oAux, float* oQ) {
+  __shared__ int cand[CAND_CAP][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts
+  __shared__ float seatpos[16];               // where a seat's particle sat, in cells: (q + 0.5) / 15
   if (threadIdx.x < 16) seatpos[threadIdx.x] = ((float)threadIdx.x + 0.5f) / 15.0f;
   __syncthreads();
-  int me = xcd_block() * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x;
+  int me = xcd_block() * blockDim.x + lane;
   if (me >= u.N) return;
   float4 P = pos[me], V = vel[me];
   float ri = mat[me];
   float ci = ri * (mod4(P.w) < 0.5f ? u.cpIron : 1.0f);
-  float pushx = 0, pushy = 0, pushz = 0, nb = 0, lost = 0, flow = 0;
-  {
-    float px = P.x, py = P.y, pz = P.z, vx = V.x, vy = V.y, vz = V.z, Ti = V.w;
-    int cx = (int)floorf(px * u.invCell), cy = (int)floorf(py * u.invCell), cz = (int)floorf(pz * u.invCell);
-    float c0x = (float)cx * u.cell, c0y = (float)cy * u.cell, c0z = (float)cz * u.cell;
+  float px = P.x, py = P.y, pz = P.z, vx = V.x, vy = V.y, vz = V.z, Ti = V.w;
+  Contact acc = { 0, 0, 0, 0, 0, 0 };
+  int cx = (int)floorf(px * u.invCell), cy = (int)floorf(py * u.invCell), cz = (int)floorf(pz * u.invCell);
+  float c0x = (float)cx * u.cell, c0y = (float)cy * u.cell, c0z = (float)cz * u.cell;
+  // A: candidates in order
+  int n = 0;
+  bool over = false;
+  for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
+    float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
+    float qx = gclamp(px, lox, lox + u.cell) - px, qy = gclamp(py, loy, loy + u.cell) - py,
+          qz = gclamp(pz, loz, loz + u.cell) - pz;
+    if (dot3(qx, qy, qz, qx, qy, qz) > u.link2) continue;
+    int h = hash_cell(u.grid, cx + dx, cy + dy, cz + dz);
+    const uint4* s4 = reinterpret_cast<const uint4*>(seats + (size_t)h * SLOTS);
+    uint4 ea = s4[0], ob = s4[1];
+    uint32_t sv8[8] = { ea.x, ob.x, ea.y, ob.y, ea.z, ob.z, ea.w, ob.w };
+    bool open = true;
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+      uint32_t sv = sv8[k];
+      open = open && !(sv & 0x80000000u);
+      if (open && seat_passes(sv, me, lox, loy, loz, px, py, pz, seatpos, u)) {
+        if (n < CAND_CAP) cand[n][lane] = (int)(sv & 0x3FFFFu);
+        else over = true;
+        n++;
+      }
+    }
+  }
+  if (!over) {
+    // B: linked pairs counted, touching ones kept (in place, in order)
+    int nt = 0;
+    for (int t = 0; t < n; t++) {
+      int j = cand[t][lane];
+      float4 Pj = pos[j];
+      float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
+      float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
+      if (r2 < u.link2) {
+        acc.nb += 1.0f;
+        if (r2 < touch2r) cand[nt++][lane] = j;
+      }
+    }
+    // C: the contacts, in order
+    for (int t = 0; t < nt; t++) {
+      int j = cand[t][lane];
+      contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, pos[j], vel[j], mat[j], acc);
+    }
+  } else {
+    // the direct path: every cell and seat in order, as the reference walks them
     for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
       float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
       float qx = gclamp(px, lox, lox + u.cell) - px, qy = gclamp(py, loy, loy + u.cell) - py,
             qz = gclamp(pz, loz, loz + u.cell) - pz;
       if (dot3(qx, qy, qz, qx, qy, qz) > u.link2) continue;
       int h = hash_cell(u.grid, cx + dx, cy + dy, cz + dz);
-      const uint4* s4 = reinterpret_cast<const uint4*>(seats + (size_t)h * SLOTS);
-      uint4 ea = s4[0], ob = s4[1];
-      uint32_t sv8[8] = { ea.x, ob.x, ea.y, ob.y, ea.z, ob.z, ea.w, ob.w };
-      // stage 1: the seats worth a fetch (up to the first empty one), their positions requested together
-      uint32_t cand = 0;
-      bool open = true;
-#pragma unroll
       for (int k = 0; k < SLOTS; k++) {
-        uint32_t sv = sv8[k];
-        open = open && !(sv & 0x80000000u);
+        uint32_t sv = seats[(size_t)h * SLOTS + ((k & 1) ? 4 : 0) + (k >> 1)];
+        if (sv & 0x80000000u) break;
+        if (!seat_passes(sv, me, lox, loy, loz, px, py, pz, seatpos, u)) continue;
         int j = (int)(sv & 0x3FFFFu);
-        float dqx = lox + seatpos[(sv >> 18) & 15u] * u.cell - px, dqy = loy + seatpos[(sv >> 22) & 15u] * u.cell - py,
-              dqz = loz + seatpos[(sv >> 26) & 15u] * u.cell - pz;
-        bool c = open && j != me && !(dot3(dqx, dqy, dqz, dqx, dqy, dqz) > u.reach2);
-        cand |= (uint32_t)c << k;
-      }
-      if (!cand) continue;
-      float4 Pj[SLOTS];
-#pragma unroll
-      for (int k = 0; k < SLOTS; k++) Pj[k] = (cand >> k & 1u) ? pos[sv8[k] & 0x3FFFFu] : make_float4(0, 0, 0, 0);
-      // stage 2: which candidates are linked and which touch; the touching ones' velocities and masses
-      uint32_t link = 0, touch = 0;
-#pragma unroll
-      for (int k = 0; k < SLOTS; k++) {
-        float ddx = Pj[k].x - px, ddy = Pj[k].y - py, ddz = Pj[k].z - pz;
+        float4 Pj = pos[j];
+        float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
         float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
-        bool l = (cand >> k & 1u) && r2 < u.link2;
-        link |= (uint32_t)l << k;
-        touch |= (uint32_t)(l && u.touch - sqrtf(r2) > 0.0f) << k;
-      }
-      if (!link) continue;
-      float4 Vj[SLOTS];
-      float rjk[SLOTS];
-#pragma unroll
-      for (int k = 0; k < SLOTS; k++) {
-        bool t = touch >> k & 1u;
-        Vj[k] = t ? vel[sv8[k] & 0x3FFFFu] : make_float4(0, 0, 0, 0);
-        rjk[k] = t ? mat[sv8[k] & 0x3FFFFu] : 1.0f;
-      }
-      // stage 3: the sums, in seat order
-#pragma unroll
-      for (int k = 0; k < SLOTS; k++) {
-        if (!(link >> k & 1u)) continue;
-        float ddx = Pj[k].x - px, ddy = Pj[k].y - py, ddz = Pj[k].z - pz;
-        float r = sqrtf(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
-        float rr = gmax(r, 1e-6f);
-        float nx = ddx / rr, ny = ddy / rr, nz = ddz / rr;
-        float ov = u.touch - r;
-        if (touch >> k & 1u) {
-          float rj = rjk[k];
-          float cij = u.c * sqrtf(2.0f * ri * rj / (ri + rj));
-          float vn = dot3(Vj[k].x - vx, Vj[k].y - vy, Vj[k].z - vz, nx, ny, nz);
-          float fs = u.k * ov - cij * vn, fn = gmax(fs, 0.0f), on = fs < 0.0f ? 0.0f : 1.0f;
-          pushx -= nx * fn; pushy -= ny * fn; pushz -= nz * fn;
-          lost += cij * vn * vn * on + u.k * ov * vn * (1.0f - on);
-          float cj = rj * (mod4(Pj[k].w) < 0.5f ? u.cpIron : 1.0f);
-          flow += (Vj[k].w - Ti) * (2.0f * ci * cj / (ci + cj));
+        if (r2 < u.link2) {
+          if (r2 < touch2r) contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, Pj, vel[j], mat[j], acc);
+          acc.nb += 1.0f;
         }
-        nb += 1.0f;
       }
     }
   }
-  float px = P.x, py = P.y, pz = P.z, vx = V.x, vy = V.y, vz = V.z;
+  float pushx = acc.pushx, pushy = acc.pushy, pushz = acc.pushz, nb = acc.nb, lost = acc.lost, flow = acc.flow;
   float body = P.w;
   float gvx = 0, gvy = 0, gvz = 0;
   if (u.field) {
@@ -731,13 +790,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   oQ[me] = Q;
 }
 
+// the smallest r^2 whose correctly rounded square root is not below the
+// contact distance: r^2 < touch2r is then the same test as touch - sqrt(r^2) > 0
+float touch_r2(float touch) {
+  float x = touch * touch;
+  while (x > 0.0f && std::sqrt(std::nextafter(x, 0.0f)) >= touch) x = std::nextafter(x, 0.0f);
+  while (std::sqrt(x) < touch) x = std::nextafter(x, INFINITY);
+  return x;
+}
+
 void launch_sim(const SimU& u, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
                 const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s) {
-  int blocks = (u.N + 255) / 256;
+  int blocks = (u.N + SIM_BLOCK - 1) / SIM_BLOCK;
   blocks = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
-  hipLaunchKernelGGL(k_sim, dim3(blocks), dim3(256), 0, s, u, pos, vel, mat, q, seats, m0.force, m0.mono, m1.force,
-                     m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ);
+  hipLaunchKernelGGL(k_sim, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, touch_r2(u.touch), pos, vel, mat, q, seats,
+                     m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ);
 }
 
 // ----------------------------------------------------------- rigid (A14) --

@@ -1,0 +1,87 @@
+"""The JavaScript host (cosmic-collisions_amd/js/) against the reference's own
+host functions: body setup (build/ball/relax/maclaurin), the mesh pair table
+and the friends-of-friends analysis with its books and loose list.  CPU only:
+these run in node without the GPU."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_util as gu
+
+JS = os.path.join(gu.REPO, "cosmic-collisions_amd", "js")
+NODE = shutil.which("node")
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+
+def node(script, tmp_path):
+    path = tmp_path / "t.js"
+    path.write_text(script)
+    r = subprocess.run([NODE, str(path)], capture_output=True, text=True, timeout=300, cwd=JS)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_build_bit_exact(tmp_path):
+    fx = gu.fixture("c1_build")
+    opt = json.loads(str(fx["opt"]))
+    out = node("""
+const nb = require('%s/nbody.js'); const fs = require('fs');
+const r = nb.buildRun(%s);
+for (const [k, a] of [['pos', r.pos], ['vel', r.vel], ['mat', r.sim.matR], ['home', r.sim.homeArr]])
+  fs.writeFileSync('%s/' + k, Buffer.from(a.buffer));
+const s = {}; for (const k of ['N', 'n1', 'n2', 'm', 'a', 'cell', 'dt', 'k', 'c', 't', 'settleLeft', 'tK', 'damp', 'vc', 'Re1', 'Re2', 'M2']) s[k] = r.sim[k];
+s.cen0 = r.sim.cen0; s.cen1 = r.sim.cen1; s.rel = r.sim.rel;
+console.log(JSON.stringify(s));
+""" % (JS, json.dumps(opt), tmp_path), tmp_path)
+    mine = json.loads(out)
+    ref = json.loads(str(fx["sim"]))
+    for k in ("pos", "vel", "mat", "home"):
+        a = np.fromfile(tmp_path / k, np.float32).reshape(fx[k].shape)
+        assert np.array_equal(a, fx[k]), k
+    for k in ("N", "n1", "n2", "m", "a", "cell", "dt", "k", "c", "t", "settleLeft", "tK", "damp", "vc", "Re1", "Re2",
+              "M2", "cen0", "cen1", "rel"):
+        assert mine[k] == ref[k], k
+
+
+def test_pair_table_bit_exact(tmp_path):
+    node("""
+const nb = require('%s/nbody.js'); require('fs').writeFileSync('%s/tab', Buffer.from(nb.pairTable(64, nb.PP_CUT).buffer));
+""" % (JS, tmp_path), tmp_path)
+    tab = np.fromfile(tmp_path / "tab", np.float32).reshape(64, 2)
+    assert np.array_equal(tab, gu.fixture("c1_build")["pptab"])
+
+
+@pytest.mark.parametrize("name", ["impact", "twobox"])
+def test_analysis_matches_reference(tmp_path, name):
+    """The host analysis on the golden state (positions, velocities, masses and
+    the reference's own DIAG read-out) against the reference analysis."""
+    fx = gu.fixture("c1_" + name)
+    sim = json.loads(str(fx["sim"]))
+    for k, a in (("P", fx["pos"]), ("V", fx["vel"]), ("R", fx["mat"]), ("D", fx["diag"].reshape(-1, 4))):
+        np.ascontiguousarray(a, np.float32).tofile(tmp_path / k)
+    out = node("""
+const { analyse } = require('%s/fof.js'); const fs = require('fs');
+const rd = (k) => { const b = fs.readFileSync('%s/' + k); return new Float32Array(b.buffer, b.byteOffset, b.byteLength / 4); };
+const r = analyse(%d, %r, %r, rd('P'), rd('V'), rd('R'), rd('D'), %r, %r, 8192);
+fs.writeFileSync('%s/list', Buffer.from(r.pp.list.buffer));
+delete r.pp; console.log(JSON.stringify(r));
+""" % (JS, tmp_path, sim["N"], sim["m"], sim["a"], sim["k"], 2 * sim["a"], tmp_path), tmp_path)
+    mine = json.loads(out)
+    ref = json.loads(str(fx["report"]))
+    for k in ("largest", "largestImp", "largestIron", "day", "tilt", "axis", "bodyVel", "orbit", "escape", "com", "core",
+              "vel", "R", "Redge", "Tsurf", "hotCom", "hotR", "impMass", "impCom", "impRms", "M", "iron", "drift", "L",
+              "KE", "PE", "EL", "Q", "unseated", "full", "jD", "moon"):
+        assert mine[k] == ref[k], k
+    assert (mine["second"] is None) == (ref["second"] is None)
+    if ref["second"] is not None:
+        for k in ref["second"]:
+            assert mine["second"][k] == ref["second"][k], k
+    lst = np.fromfile(tmp_path / "list", np.int32)
+    # the reference's list comes back translated into the Morton order it sorted into
+    into = np.empty_like(lst, shape=len(fx["perm"]))
+    into[fx["perm"].astype(np.int64)] = np.arange(len(fx["perm"]))
+    assert np.array_equal(into[lst].astype(np.float32), fx["pplist_fof"])
