@@ -36,8 +36,13 @@ class SimArgs(C.Structure):
 EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_upload_state", "cc_download_state",
            "cc_set_boxes", "cc_set_mono", "cc_slots", "cc_read_seats", "cc_deposit", "cc_coarse", "cc_cell", "cc_mesh",
            "cc_read_mesh", "cc_write_mesh", "cc_set_pp_table", "cc_set_pp", "cc_pp", "cc_read_pp", "cc_sim", "cc_step",
-           "cc_step_n", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_diag", "cc_morton", "cc_books", "cc_sync",
-           "cc_last_step_ms"]
+           "cc_step_n", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_couple", "cc_diag", "cc_morton", "cc_books",
+           "cc_sync", "cc_last_step_ms"]
+
+
+class CoupleArgs(C.Structure):
+    _fields_ = [("centre", C.c_double * 3)] + [(n, C.c_double) for n in (
+        "vimp", "aimp", "c1", "invMu", "pexp", "stall", "z", "heat", "tK", "frozen", "ambient", "melt")]
 
 
 class CCError(RuntimeError):
@@ -68,6 +73,7 @@ def lib():
             "cc_step_n": ([P, C.c_int, C.POINTER(SimArgs), dp, C.c_int], C.c_int),
             "cc_get_step_no": ([P, ip], C.c_int), "cc_set_step_no": ([P, C.c_int], C.c_int),
             "cc_rigid": ([P, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int), "cc_diag": ([P, fp], C.c_int),
+            "cc_couple": ([P, C.POINTER(CoupleArgs)], C.c_int),
             "cc_morton": ([P, dp, ip, ip], C.c_int), "cc_books": ([P, dp], C.c_int), "cc_sync": ([P], C.c_int),
             "cc_last_step_ms": ([P, dp], C.c_int),
         }
@@ -252,6 +258,13 @@ class Context:
     def rigid(self, bulk0, bulk1, cen0, cen1, rot0, rot1, dtF):
         self._chk(lib().cc_rigid(self.h, _d3(bulk0), _d3(bulk1), _d3(cen0), _d3(cen1), _d3(rot0), _d3(rot1),
                                  float(dtF)), "cc_rigid")
+
+    def couple(self, centre, **kw):
+        a = CoupleArgs()
+        a.centre[:] = [float(x) for x in centre]
+        for k, v in kw.items():
+            setattr(a, k, float(v))
+        self._chk(lib().cc_couple(self.h, C.byref(a)), "cc_couple")
 
     def diag(self):
         out = np.empty((self.N, 4), np.float32)

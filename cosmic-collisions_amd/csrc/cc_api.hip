@@ -424,6 +424,13 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   int n = c->stepNo++;
   int r;
   if (n % 2 == 0 && (r = cc_slots(c))) return r;
+  if (c->p.field) {   // constant gravity: no mesh, and the field acts on every step
+    cc_sim_args b = *a;
+    b.settle = settle;
+    b.kick = kick;
+    b.gdt = c->p.dt;
+    return sim_pass(c, &b);
+  }
   bool mesh = n % c->p.meshEvery == 0;
   if (mesh && (r = mesh_all(c))) return r;
   if (mesh && (r = cc_pp(c))) return r;
@@ -482,6 +489,20 @@ int cc_rigid(cc_ctx* c, const double bulk0[3], const double bulk1[3], const doub
   int s = c->cur, d = c->cur ^ 1;
   launch_rigid(c->N, c->pos[s], c->vel[s], c->aux[s], c->q[s], b0, b1, c0, c1, s0, s1, r0, r1, (float)dtF, c->pos[d],
                c->vel[d], c->aux[d], c->q[d], c->s);
+  LAUNCHCHK(c);
+  c->cur = d;
+  return 0;
+}
+
+int cc_couple(cc_ctx* c, const cc_couple_args* a) {
+  if (!c || !a) return CC_EINVAL;
+  CoupleU u;
+  for (int k = 0; k < 3; k++) u.centre[k] = (float)a->centre[k];
+  u.vimp = (float)a->vimp; u.aimp = (float)a->aimp; u.c1 = (float)a->c1; u.invMu = (float)a->invMu;
+  u.pexp = (float)a->pexp; u.stall = (float)a->stall; u.z = (float)a->z; u.heat = (float)a->heat;
+  u.tK = (float)a->tK; u.frozen = (float)a->frozen; u.ambient = (float)a->ambient; u.melt = (float)a->melt;
+  int s = c->cur, d = c->cur ^ 1;
+  launch_couple(c->N, u, c->pos[s], c->vel[s], c->aux[s], c->mat, c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
   LAUNCHCHK(c);
   c->cur = d;
   return 0;

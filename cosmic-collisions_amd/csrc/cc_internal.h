@@ -59,6 +59,11 @@ struct SimU {           // the step pass's uniforms, already FP32
   float frozen;
 };
 
+struct CoupleU {        // COUPLE_FS's uniforms (chicxulub.html:260-276), FP32
+  float centre[3];
+  float vimp, aimp, c1, invMu, pexp, stall, z, heat, tK, frozen, ambient, melt;
+};
+
 struct MeshBuf {
   unsigned long long* acc;   // int64 two's complement, [FINE_TEXELS*4]
   float4* fine;
@@ -102,6 +107,8 @@ void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux
                   const float* bulk0, const float* bulk1, const float* cen0, const float* cen1,
                   const float* spin0, const float* spin1, const float* rot0, const float* rot1, float dt,
                   float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
+void launch_couple(int N, const CoupleU& c, const float4* pos, const float4* vel, const float4* aux, const float* mat,
+                   float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
 void launch_diag(const SimU& u, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
                  const MeshBuf& m1, const float* ppSlot, const float4* ppForce, float4* out, hipStream_t s);
 void launch_pp_slot(int N, const int* index, int count, float* slot, hipStream_t s);

@@ -847,6 +847,53 @@ void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux
   hipLaunchKernelGGL(k_rigid, dim3((N + 255) / 256), dim3(256), 0, s, N, r, pos, vel, aux, q, oPos, oVel, oAux, oQ);
 }
 
+// ----------------------------------------------- crater coupling (A20) --
+// The excavation flow at touchdown (COUPLE_FS, chicxulub.html:255-315): each
+// unfrozen grain gets the Housen-Holsapple speed for its range from the point
+// source, capped at the impact speed and stalled at the transient radius, in
+// the Z-model direction about the downward axis; the impactor becomes melt
+// and keeps half its own velocity; the heat is that of the speed.
+__global__ void __launch_bounds__(256) k_couple(int N, CoupleU c, const float4* __restrict__ pos,
+                                                const float4* __restrict__ vel, const float4* __restrict__ aux,
+                                                const float* __restrict__ mat, float4* oPos, float4* oVel,
+                                                float4* oAux, float* oQ) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float4 P = pos[i], V = vel[i];
+  float w = P.w, ri = mat[i];
+  oAux[i] = aux[i];
+  if (fabsf(w - c.frozen) < 0.5f) {
+    oPos[i] = P; oVel[i] = make_float4(0.0f, 0.0f, 0.0f, c.ambient); oQ[i] = 0.0f;
+    return;
+  }
+  float Rx = P.x - c.centre[0], Ry = P.y - c.centre[1], Rz = P.z - c.centre[2];
+  float r = gmax(sqrtf(dot3(Rx, Ry, Rz, Rx, Ry, Rz)), 1e-4f);
+  float v = c.c1 * c.vimp * powf(r / c.aimp, -c.invMu);
+  v = gmin(v, c.vimp);
+  v *= powf(gmax(0.0f, 1.0f - r / c.stall), c.pexp);
+  float nx = Rx / r, ny = Ry / r, nz = Rz / r;
+  float ct = gclamp(dot3(nx, ny, nz, 0.0f, -1.0f, 0.0f), -1.0f, 1.0f);
+  float st = sqrtf(gmax(1.0f - ct * ct, 1e-8f));
+  float ex = (ct * nx - 0.0f) / st, ey = (ct * ny - -1.0f) / st, ez = (ct * nz - 0.0f) / st;
+  float th = acosf(ct);
+  float f = (c.z - 2.0f) * tanf(0.5f * th);
+  float dx = nx + f * ex, dy = ny + f * ey, dz = nz + f * ez;
+  float inv = rsq(dot3(dx, dy, dz, dx, dy, dz));
+  dx *= inv; dy *= inv; dz *= inv;
+  bool imp = w > 3.5f;
+  float vx = dx * v, vy = dy * v, vz = dz * v;
+  if (imp) { vx = gmix(vx, V.x, 0.5f); vy = gmix(vy, V.y, 0.5f); vz = gmix(vz, V.z, 0.5f); }
+  float speed = sqrtf(dot3(vx, vy, vz, vx, vy, vz));
+  float T = c.ambient + c.heat * 0.5f * speed * speed;
+  oVel[i] = make_float4(vx, vy, vz, T);
+  oQ[i] = T * ri / gmax(c.tK, 1e-9f);
+  oPos[i] = make_float4(P.x, P.y, P.z, imp ? c.melt : (T > 1500.0f ? c.melt : w));
+}
+void launch_couple(int N, const CoupleU& c, const float4* pos, const float4* vel, const float4* aux, const float* mat,
+                   float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s) {
+  hipLaunchKernelGGL(k_couple, dim3((N + 255) / 256), dim3(256), 0, s, N, c, pos, vel, aux, mat, oPos, oVel, oAux, oQ);
+}
+
 // --------------------------------------------------------- read-out (A16) --
 __global__ void __launch_bounds__(256) k_diag(SimU u, const float4* __restrict__ pos, const float* __restrict__ q,
                                               const uint32_t* __restrict__ seats, const float4* __restrict__ force0,

@@ -333,6 +333,20 @@ napi_value Rigid(napi_env env, napi_callback_info info) {
   RET(cc_rigid(c, b0, b1, c0, c1, r0, r1, num(env, a.argv[7])), "cc_rigid");
 }
 
+napi_value Couple(napi_env env, napi_callback_info info) {
+  CTX0();
+  cc_couple_args u;
+  memset(&u, 0, sizeof u);
+  napi_value o = a.argv[1];
+  if (!nums(env, prop(env, o, "centre"), u.centre, 3)) { napi_throw_type_error(env, nullptr, "bad centre"); return nullptr; }
+  u.vimp = num(env, prop(env, o, "vimp")); u.aimp = num(env, prop(env, o, "aimp")); u.c1 = num(env, prop(env, o, "c1"));
+  u.invMu = num(env, prop(env, o, "invMu")); u.pexp = num(env, prop(env, o, "pexp"));
+  u.stall = num(env, prop(env, o, "stall")); u.z = num(env, prop(env, o, "z")); u.heat = num(env, prop(env, o, "heat"));
+  u.tK = num(env, prop(env, o, "tK")); u.frozen = num(env, prop(env, o, "frozen"));
+  u.ambient = num(env, prop(env, o, "ambient")); u.melt = num(env, prop(env, o, "melt"));
+  RET(cc_couple(c, &u), "cc_couple");
+}
+
 napi_value Diag(napi_env env, napi_callback_info info) {
   CTX0();
   TA(out, float, 1, napi_float32_array, 4);
@@ -366,7 +380,7 @@ napi_value Init(napi_env env, napi_value exports) {
     { "deposit", Deposit }, { "coarse", Coarse }, { "cell", Cell }, { "mesh", Mesh }, { "readMesh", ReadMesh },
     { "writeMesh", WriteMesh }, { "setPPTable", SetPPTable }, { "setPP", SetPP }, { "pp", PP }, { "readPP", ReadPP },
     { "sim", Sim }, { "step", Step }, { "stepN", StepN }, { "getStepNo", GetStepNo }, { "setStepNo", SetStepNo },
-    { "rigid", Rigid }, { "diag", Diag }, { "morton", Morton }, { "books", Books }, { "sync", Sync },
+    { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "books", Books }, { "sync", Sync },
   };
   for (auto& f : fns) {
     napi_value fn;

@@ -98,7 +98,8 @@ int cc_read_pp(cc_ctx* ctx, float* list, float* box, float* force);
 /* passSim (src/nbody.js:458-484): one step pass, state src -> dst, swap */
 int cc_sim(cc_ctx* ctx, const cc_sim_args* a);
 /* step (src/nbody.js:551-558): seats every 2nd step, mesh + pairwise every
- * meshEvery-th with gravity as one meshEvery*dt impulse.  box_L is the
+ * meshEvery-th with gravity as one meshEvery*dt impulse; with field gravity
+ * (the crater, chicxulub.html:779-782) no mesh and gravity every step.  box_L is the
  * boxes() uniform for the mesh step (NULL: keep the last); stepNo advances. */
 int cc_step(cc_ctx* ctx, int settle, int kick, const cc_sim_args* a);
 /* n steps without settle or kick; boxL[n_mesh][7] = {LA, LB, boxB} for each mesh step in order (NULL: keep) */
@@ -108,6 +109,13 @@ int cc_set_step_no(cc_ctx* ctx, int stepNo);
 /* rigidStep's GPU half (RIGID_FS, src/glsl-sim.js:537-570): rot column-major */
 int cc_rigid(cc_ctx* ctx, const double bulk0[3], const double bulk1[3], const double cen0[3], const double cen1[3],
              const double rot0[9], const double rot1[9], double dtF);
+/* The crater's coupling at touchdown (COUPLE_FS + couple(), chicxulub.html:255-315, :820-842):
+ * the excavation flow from the point source `centre`; MAT_MELT is `melt`. */
+typedef struct {
+  double centre[3];
+  double vimp, aimp, c1, invMu, pexp, stall, z, heat, tK, frozen, ambient, melt;
+} cc_couple_args;
+int cc_couple(cc_ctx* ctx, const cc_couple_args* a);
 /* DIAG_FS alone (src/glsl-sim.js:608-667) on the current seats/mesh/pp; out N*4 (NULL: keep on device) */
 int cc_diag(cc_ctx* ctx, float* out);
 /* Morton order (src/nbody.js:871-887) from the analysis' core centre, applied

@@ -521,6 +521,47 @@ void cco_perm(int N, const float* perm, const float* pos, const float* vel, cons
   }
 }
 
+void cco_couple(int N, const float* args, const float* pos, const float* vel, const float* aux, const float* mat,
+                float* oPos, float* oVel, float* oAux, float* oQ) {
+  const float *centre = args;
+  float vimp = args[3], aimp = args[4], c1 = args[5], invMu = args[6], pexp = args[7], stall = args[8], z = args[9];
+  float heat = args[10], tK = args[11], frozen = args[12], ambient = args[13], melt = args[14];
+  for (int i = 0; i < N; i++) {
+    const float *P = pos + (size_t)i * 4, *V = vel + (size_t)i * 4;
+    float w = P[3], ri = mat[i];
+    memcpy(oPos + (size_t)i * 4, P, 16);
+    memcpy(oAux + (size_t)i * 4, aux + (size_t)i * 4, 16);
+    if (fabsf(w - frozen) < 0.5f) {
+      oVel[i * 4] = oVel[i * 4 + 1] = oVel[i * 4 + 2] = 0.0f; oVel[i * 4 + 3] = ambient; oQ[i] = 0.0f;
+      continue;
+    }
+    float R[3] = { P[0] - centre[0], P[1] - centre[1], P[2] - centre[2] };
+    float r = fmaxf_(sqrtf(dot3(R, R)), 1e-4f);
+    float v = c1 * vimp * powf(r / aimp, -invMu);
+    v = fminf_(v, vimp);
+    v *= powf(fmaxf_(0.0f, 1.0f - r / stall), pexp);
+    float n[3] = { R[0] / r, R[1] / r, R[2] / r }, axis[3] = { 0.0f, -1.0f, 0.0f };
+    float ct = clampf(dot3(n, axis), -1.0f, 1.0f);
+    float st = sqrtf(fmaxf_(1.0f - ct * ct, 1e-8f));
+    float eth[3];
+    for (int a = 0; a < 3; a++) eth[a] = (ct * n[a] - axis[a]) / st;
+    float th = acosf(ct);
+    float f = (z - 2.0f) * tanf(0.5f * th);
+    float d[3];
+    for (int a = 0; a < 3; a++) d[a] = n[a] + f * eth[a];
+    float inv = rsq(dot3(d, d));
+    float vv[3];
+    for (int a = 0; a < 3; a++) vv[a] = d[a] * inv * v;
+    int imp = w > 3.5f;
+    if (imp) for (int a = 0; a < 3; a++) vv[a] = mixf(vv[a], V[a], 0.5f);
+    float speed = sqrtf(dot3(vv, vv));
+    float T = ambient + heat * 0.5f * speed * speed;
+    oVel[i * 4] = vv[0]; oVel[i * 4 + 1] = vv[1]; oVel[i * 4 + 2] = vv[2]; oVel[i * 4 + 3] = T;
+    oQ[i] = T * ri / fmaxf_(tK, 1e-9f);
+    oPos[i * 4 + 3] = imp ? melt : (T > 1500.0f ? melt : w);
+  }
+}
+
 void cco_diag(const cco_sim_u* u, const float* pos, const float* q, const uint32_t* seatA, const uint32_t* seatB,
               const float* force0, const float* mono0, const float* force1, const float* mono1, const float* ppSlot,
               const float* ppForce, float* out) {
@@ -696,5 +737,23 @@ void cco_mesh_pair_table(int bins, double cut, float* f, float* p) {
     }
     f[b] = (float)(sf / 80 * r * r);
     p[b] = (float)(sp / 80 * r);
+  }
+}
+
+/* The FP64 direct sum the mesh and pairwise gravity approximate: Newton
+ * softened at the particle radius (the pairwise pass's own softening,
+ * src/glsl-sim.js:735-751), every pair, acceleration and potential times G m.
+ * O(N^2): for the 16,384-particle states only. */
+void cco_direct_f64(int N, const float* pos, const float* mat, double gm, double a2, double* out) {
+  for (int i = 0; i < N; i++) {
+    double px = pos[i * 4], py = pos[i * 4 + 1], pz = pos[i * 4 + 2];
+    double ax = 0, ay = 0, az = 0, pot = 0;
+    for (int j = 0; j < N; j++) {
+      if (j == i) continue;
+      double dx = pos[j * 4] - px, dy = pos[j * 4 + 1] - py, dz = pos[j * 4 + 2] - pz;
+      double r2 = dx * dx + dy * dy + dz * dz + a2, inv = 1.0 / sqrt(r2), s = mat[j] * inv * inv * inv;
+      ax += dx * s; ay += dy * s; az += dz * s; pot -= mat[j] * inv;
+    }
+    out[i * 4] = ax * gm; out[i * 4 + 1] = ay * gm; out[i * 4 + 2] = az * gm; out[i * 4 + 3] = pot * gm;
   }
 }

@@ -83,6 +83,10 @@ void cco_rigid(int N, const float* pos, const float* vel, const float* aux, cons
 /* A15 PERM_FS, src/glsl-sim.js:578-602 */
 void cco_perm(int N, const float* perm, const float* pos, const float* vel, const float* aux, const float* q,
               float* oPos, float* oVel, float* oAux, float* oQ);
+/* A20 COUPLE_FS, chicxulub.html:255-315: args = centre[3], vimp, aimp, c1, invMu, pexp, stall, z, heat, tK,
+ * frozen, ambient, melt (15 floats) */
+void cco_couple(int N, const float* args, const float* pos, const float* vel, const float* aux, const float* mat,
+                float* oPos, float* oVel, float* oAux, float* oQ);
 /* A16 DIAG_FS, src/glsl-sim.js:608-667 */
 void cco_diag(const cco_sim_u* u, const float* pos, const float* q, const uint32_t* seatA, const uint32_t* seatB,
               const float* force0, const float* mono0, const float* force1, const float* mono1, const float* ppSlot,
@@ -91,6 +95,8 @@ void cco_diag(const cco_sim_u* u, const float* pos, const float* q, const uint32
 void cco_morton(const float* pos, int N, const double core[3], int32_t* keys, float* perm, int32_t* into);
 /* A17 books, src/nbody.js:831-850: out[0..13] = M, px, py, pz, Jx, Jy, Jz, KE, PE, Q, unseated, full, iron, drift */
 void cco_books(int N, double mref, const float* pos, const float* vel, const float* mat, const float* diag, double* out);
+/* FP64 direct sum of Newton gravity softened at a (a2 = a^2), x G m: out N*4 (acc, pot) */
+void cco_direct_f64(int N, const float* pos, const float* mat, double gm, double a2, double* out);
 /* src/math.js:8-15 Mulberry32 */
 uint32_t cco_mulberry_next(uint32_t* state, double* out);
 /* A10 meshPairTable, src/nbody.js:56-109 (bins x {force r^2, potential r}) */

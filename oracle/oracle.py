@@ -60,10 +60,12 @@ def lib():
         L.cco_rigid.argtypes = [C.c_int] + [f32p] * 4 + [f32p] * 8 + [C.c_float] + [f32p] * 4
         L.cco_perm.argtypes = [C.c_int] + [f32p] * 5 + [f32p] * 4
         L.cco_diag.argtypes = [C.POINTER(SimU), f32p, f32p, u32p, u32p] + [f32p] * 6 + [f32p]
+        L.cco_couple.argtypes = [C.c_int, f32p] + [f32p] * 4 + [f32p] * 4
         L.cco_morton.argtypes = [f32p, C.c_int, f64p, i32p, f32p, i32p]
         L.cco_books.argtypes = [C.c_int, C.c_double, f32p, f32p, f32p, f32p, f64p]
         L.cco_mulberry_next.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.cco_mulberry_next.restype = C.c_uint32
+        L.cco_direct_f64.argtypes = [C.c_int, f32p, f32p, C.c_double, C.c_double, f64p]
         L.cco_mesh_pair_table.argtypes = [C.c_int, C.c_double, f32p, f32p]
         _LIB = L
     return _LIB
@@ -191,6 +193,14 @@ def perm(perm_, pos, vel, aux, q):
     return outs
 
 
+def couple(args, pos, vel, aux, mat):
+    """args: centre[3], vimp, aimp, c1, invMu, pexp, stall, z, heat, tK, frozen, ambient, melt"""
+    N = pos.size // 4
+    outs = [np.empty((N, 4), np.float32) for _ in range(3)] + [np.empty(N, np.float32)]
+    lib().cco_couple(N, c32(args), c32(pos).ravel(), c32(vel).ravel(), c32(aux).ravel(), c32(mat).ravel(), *outs)
+    return outs
+
+
 def morton(pos, core):
     N = pos.size // 4
     keys = np.empty(N, np.int32)
@@ -207,6 +217,14 @@ def books(mref, pos, vel, mat, diag_):
     out = np.empty(14, np.float64)
     lib().cco_books(pos.size // 4, mref, c32(pos).ravel(), c32(vel).ravel(), c32(mat).ravel(), c32(diag_).ravel(), out)
     return dict(zip(BOOK_NAMES, out.tolist()))
+
+
+def direct_f64(pos, mat, gm, a2):
+    """FP64 direct-sum gravity softened at a: (N, 4) acc xyz, potential."""
+    N = pos.size // 4
+    out = np.empty((N, 4), np.float64)
+    lib().cco_direct_f64(N, c32(pos).ravel(), c32(mat).ravel(), gm, a2, out)
+    return out
 
 
 def mulberry(seed, n):

@@ -57,7 +57,7 @@ def trace_and_replay(work, ops, tag):
 def kind_of(fs):
     if "uInject" in fs:
         return "cell"
-    for key, k in (("flat in float vSeat", "slot"), ("in vec4 vW", "dep"), ("oT1", "far"), ("uKick", "sim"),
+    for key, k in (("uVimp", "couple"), ("flat in float vSeat", "slot"), ("in vec4 vW", "dep"), ("oT1", "far"), ("uKick", "sim"),
                    ("uRot0", "rigid"), ("uPerm", "perm"), ("seated", "diag"), ("oBox", "ppgather"),
                    ("uTab", "ppforce"), ("c * 4 + ivec3(x, y, z)", "coarse"),
                    ("for (int y = 0; y < 64; y++)", "mono")):
@@ -127,13 +127,53 @@ def make_state(base_pos, base_vel, mat, sim, kind, rng):
     return pos.astype(np.float32), vel.astype(np.float32), q
 
 
+CRATER_OPT = {"nIdx": 0, "imp": 14, "vimp": 20, "angle": 60, "sed": 3, "dens": 2500, "rest": 0.25}
+
+
+def make_crater(args):
+    """The crater page (C3 preset at 16,384 particles): its build, two settle
+    steps, the approach to touchdown, the coupling pass and two full steps,
+    each state dumped so that every pass compares from identical input."""
+    st = ["pos", "vel", "aux", "q"]
+    ops = [{"op": "crater_build", "opt": CRATER_OPT, "tag": "crater"},
+           {"op": "dump", "what": ["pos", "vel"], "tag": "b0"},
+           {"op": "crater_step", "settle": 1}, {"op": "dump", "what": st + ["seatA", "seatB"], "tag": "s1"},
+           {"op": "crater_step", "settle": 1}, {"op": "dump", "what": st, "tag": "s2"},
+           {"op": "crater_approach", "what": st, "tagBefore": "pre"},
+           {"op": "dump", "what": st, "tag": "post"},
+           {"op": "crater_step", "settle": 0}, {"op": "dump", "what": st + ["seatA", "seatB"], "tag": "f1"},
+           {"op": "crater_step", "settle": 0}, {"op": "dump", "what": st, "tag": "f2"}]
+    d = trace_and_replay(args.work, ops, "crater")
+    host = json.load(open(os.path.join(d, "host.json")))
+    fx = {"mat": np.fromfile(os.path.join(d, "crater.mat.f32"), np.float32), "opt": json.dumps(CRATER_OPT),
+          "host": json.dumps(host["ops"]), "draws": json.dumps(uniform_snapshots(d))}
+    for tag, names in (("b0", ["pos", "vel"]), ("s1", st), ("s2", st), ("pre", st), ("post", st), ("f1", st),
+                       ("f2", st)):
+        for nm in names:
+            fx["%s_%s" % (tag, nm)] = load(d, tag, nm)
+    for tag in ("s1", "f1"):
+        for nm in ("seatA", "seatB"):
+            u32 = load(d, tag, nm).view(np.uint32).reshape(-1, 4)
+            occ = np.flatnonzero((u32 != 0xBF800000).any(1))
+            fx["%s_%s_idx" % (tag, nm)] = occ.astype(np.int32)
+            fx["%s_%s_val" % (tag, nm)] = u32[occ]
+    np.savez_compressed(os.path.join(args.out, "c3_crater.npz"), **fx)
+    print("crater fixture: N=%d, touchdown %s" % (len(fx["mat"]), [o for o in host["ops"] if o["op"] == "crater_touchdown"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "fixtures"))
     ap.add_argument("--work", default=os.path.join(REPO, "_work", "golden"))
+    ap.add_argument("--cases", default="c1,crater", help="comma list of: c1 (build, impact, twobox), crater")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     os.makedirs(args.work, exist_ok=True)
+    cases = args.cases.split(",")
+    if "crater" in cases:
+        make_crater(args)
+    if "c1" not in cases:
+        return 0
 
     # 1. the reference's body setup for C1, and its pair table
     d0 = trace_and_replay(args.work, [{"op": "build", "opt": C1_OPT, "tag": "c1"},

@@ -133,7 +133,76 @@ CC.gl.init({ gl: gl, floatBlend: true });
 CC.nbody.init({ gl: gl, floatBlend: true });
 const NB = CC.nbody;
 
+// ---- the crater page (chicxulub.html), on the same recording GL ----
+// The crater view is page code, not an engine module: its constants, shaders
+// and functions live inside the page's script. This takes the pieces the
+// crater's physics needs out of the page's text -- the constants and
+// COUPLE_FS (from `var L_UNIT` to the globe section), and latticeFill,
+// buildCrater, setStaticUniforms, passSlots, passSim, step, settleSome,
+// approachStep and couple by name -- and evaluates them in one function
+// scope beside stubs for the page's view, DOM and camera.
+const HTML = process.env.REF_HTML || path.join(REF, '..', 'chicxulub.html');
+let crater = null;
+function craterPage() {
+  const text = fs.readFileSync(HTML, 'utf8');
+  const between = (a, b) => {
+    const i = text.indexOf(a), j = text.indexOf(b, i);
+    if (i < 0 || j < 0) throw new Error('crater page section not found: ' + a);
+    return text.slice(i, j);
+  };
+  const fn = (name) => {
+    const start = text.indexOf('function ' + name + '(');
+    if (start < 0) throw new Error('crater page function not found: ' + name);
+    let depth = 0;
+    for (let i = text.indexOf('{', start); i < text.length; i++) {
+      if (text[i] === '{') depth++;
+      else if (text[i] === '}') { depth--; if (depth === 0) return text.slice(start, i + 1); }
+    }
+    throw new Error('unbalanced ' + name);
+  };
+  const frames = /var APPROACH_FRAMES = [^;]+;/.exec(text)[0];
+  const src = [
+    '(function (gl, hooks) {',
+    "'use strict';",
+    'var $ = function () { return { classList: { remove: function () {} }, hidden: false }; };',
+    'var GLH = CC.gl, program = GLH.program, floatTex = GLH.floatTex;',
+    'var simTarget = GLH.simTarget, freeTarget = GLH.freeTarget, makeGrid = GLH.makeGrid, freeGrid = GLH.freeGrid;',
+    between('var L_UNIT', '// ---------- the first day'),
+    'var simProg = program(QUAD_VS, SIM_FS), slotProg = program(SLOT_VS, SLOT_FS);',
+    'var rigidProg = program(QUAD_VS, RIGID_FS), coupleProg = program(QUAD_VS, COUPLE_FS);',
+    'var VIEW = { setPalette: function () {}, setMaxPoint: function () {}, setRadPow: function () {}, setLook: function () {}, clampDist: function () {} };',
+    'var cam = {}, run = {}, MATH = CC.math, BODIES = CC.bodies, mulberry = MATH.mulberry, ball = BODIES.ball;',
+    "var grid = null, sim = null, report = null, framedAt = 0, gen = 0, stepAcc = 0, lastReport = '';",
+    fn('latticeFill'), fn('buildCrater'), fn('setStaticUniforms'), fn('passSlots'), fn('passSim'), fn('step'),
+    fn('settleSome'), frames, fn('approachStep'), fn('couple'),
+    'var coupleOf = couple;',
+    'couple = function () { if (hooks.beforeCouple) hooks.beforeCouple(); coupleOf(); };',
+    'return { S: S, build: buildCrater, settleSome: settleSome, step: step, approachStep: approachStep,',
+    '         get sim() { return sim; }, get grid() { return grid; },',
+    '         consts: { HH_C1: HH_C1, HH_MU: HH_MU, HH_P: HH_P, Z_MODEL: Z_MODEL, E_KG: E_KG, CP_ROCK: CP_ROCK,',
+    '                   G_EARTH: G_EARTH, T_UNIT: T_UNIT, L_UNIT: L_UNIT, MAT_MELT: MAT_MELT, MAT_FROZEN: MAT_FROZEN,',
+    '                   APPROACH_FRAMES: APPROACH_FRAMES } };',
+    '})'
+  ].join('\n');
+  const hooks = {};
+  const page = vm.runInContext(src, ctx, { filename: 'chicxulub.html#crater' })(gl, hooks);
+  page.hooks = hooks;
+  return page;
+}
+
 function texOf(name) {
+  if (crater) {
+    const s = crater.sim, g = crater.grid;
+    switch (name) {
+      case 'pos': return { tex: s.src.pos, w: s.W, h: s.H };
+      case 'vel': return { tex: s.src.vel, w: s.W, h: s.H };
+      case 'aux': return { tex: s.src.aux, w: s.W, h: s.H };
+      case 'q': return { tex: s.src.q, w: s.W, h: s.H };
+      case 'seatA': return { tex: g.slot[0], w: g.AW, h: g.AH };
+      case 'seatB': return { tex: g.slot[1], w: g.AW, h: g.AH };
+      default: throw new Error('unknown crater texture ' + name);
+    }
+  }
   const sim = NB.sim, pm = NB.pm, g = NB.grid, pp = NB.pp;
   const m = /^(fine|coarse|mono|farA|farT0|farT1|force)([01])$/.exec(name);
   if (m) {
@@ -254,6 +323,44 @@ for (const op of job.ops) {
     case 'dump': {
       trace.push({ op: '__dump', args: [], tag: op.tag,
                    dump: op.what.map((nm) => { const t = texOf(nm); return { name: nm, tex: t.tex.__h, w: t.w, h: t.h }; }) });
+      break;
+    }
+    // the crater page: build (its S fields from op.opt), settle, approach, step
+    case 'crater_build': {
+      crater = craterPage();
+      Object.assign(crater.S, op.opt);
+      crater.build();
+      const s = crater.sim;
+      const rec = {};
+      for (const k of Object.keys(s)) {
+        const v = s[k];
+        if (typeof v === 'number' || typeof v === 'string' || Array.isArray(v)) rec[k] = JSON.parse(JSON.stringify(v));
+      }
+      rec.G = crater.grid.G;
+      host.ops.push({ op: 'crater_build', sim: rec, consts: crater.consts });
+      fs.writeFileSync(path.join(outDir, (op.tag || 'crater') + '.mat.f32'), Buffer.from(s.matR.buffer));
+      break;
+    }
+    case 'crater_settle': crater.settleSome(op.n); break;
+    case 'crater_step': crater.step(!!op.settle); break;
+    case 'crater_set_sim': {
+      for (const k of Object.keys(op.fields)) crater.sim[k] = op.fields[k];
+      break;
+    }
+    case 'crater_approach': {
+      // frames until the coupling pass; the state just before that pass is dumped
+      const what = op.what;
+      crater.hooks.beforeCouple = () => {
+        trace.push({ op: '__dump', args: [], tag: op.tagBefore,
+                     dump: what.map((nm) => { const t = texOf(nm); return { name: nm, tex: t.tex.__h, w: t.w, h: t.h }; }) });
+        const s = crater.sim;
+        host.ops.push({ op: 'crater_couple', tag: op.tagBefore,
+                        sim: { cen2: s.cen2.slice(), dir: s.dir, dImp: s.dImp, vImp: s.vImp, aImp: s.aImp, Dtc: s.Dtc, tK: s.tK } });
+      };
+      let frames = 0;
+      while (crater.sim.phase !== 'full' && frames < 10 * crater.consts.APPROACH_FRAMES) { crater.approachStep(); frames++; }
+      if (crater.sim.phase !== 'full') throw new Error('no touchdown');
+      host.ops.push({ op: 'crater_touchdown', frames: frames, centre: crater.sim.centre });
       break;
     }
     default: throw new Error('unknown job op ' + op.op);

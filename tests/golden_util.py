@@ -76,7 +76,25 @@ def sim_u(u, mode="mesh"):
     else:
         s.field = 1
         s.gField[:] = v3(u, "uGField")
+    if "uFrozen" in u:
+        s.frozenOn = 1
+        s.frozen = float(f(u, "uFrozen"))
     return s
+
+
+CRATER_STEPS = (("b0", "s1", 1), ("s1", "s2", 1), ("post", "f1", 0), ("f1", "f2", 0))
+
+
+def crater_state(fx, tag):
+    """(pos, vel, q) of a crater dump; the build's q is zero."""
+    pos = fx[tag + "_pos"].reshape(-1, 4)
+    vel = fx[tag + "_vel"].reshape(-1, 4)
+    q = fx[tag + "_q"].reshape(-1, 4)[:, 0] if tag + "_q" in fx else np.zeros(pos.shape[0], np.float32)
+    return pos, vel, q
+
+
+def crater_host(fx, op):
+    return [o for o in json.loads(str(fx["host"])) if o["op"] == op]
 
 
 def box_from(u):

@@ -85,3 +85,30 @@ delete r.pp; console.log(JSON.stringify(r));
     into = np.empty_like(lst, shape=len(fx["perm"]))
     into[fx["perm"].astype(np.int64)] = np.arange(len(fx["perm"]))
     assert np.array_equal(into[lst].astype(np.float32), fx["pplist_fof"])
+
+
+def test_crater_build_bit_exact(tmp_path):
+    """A20 the crater page's build (latticeFill, the impactor ball, materials,
+    encounter and contact constants) against the reference page's own."""
+    fx = gu.fixture("c3_crater")
+    opt = json.loads(str(fx["opt"]))
+    ref = gu.crater_host(fx, "crater_build")[0]["sim"]
+    out = node("""
+const cr = require('%s/crater.js'); const fs = require('fs');
+const r = cr.buildCraterRun(Object.assign({}, %s));
+for (const [k, a] of [['pos', r.pos], ['vel', r.vel], ['mat', r.sim.matR]])
+  fs.writeFileSync('%s/' + k, Buffer.from(a.buffer));
+const s = {}; for (const k of Object.keys(r.sim)) { const v = r.sim[k]; if (typeof v === 'number' || typeof v === 'string' || Array.isArray(v)) s[k] = v; }
+console.log(JSON.stringify(s));
+""" % (JS, json.dumps(opt), tmp_path), tmp_path)
+    mine = json.loads(out)
+    N = ref["N"]
+    assert np.array_equal(np.fromfile(tmp_path / "pos", np.float32), fx["b0_pos"].ravel())
+    assert np.array_equal(np.fromfile(tmp_path / "vel", np.float32), fx["b0_vel"].ravel())
+    assert np.array_equal(np.fromfile(tmp_path / "mat", np.float32), fx["mat"])
+    for k, v in ref.items():
+        if k in ("gen", "G"):
+            continue
+        assert k in mine, k
+        assert mine[k] == v, k
+    assert N == len(fx["mat"])
