@@ -16,6 +16,7 @@ import math
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -27,82 +28,6 @@ sys.path.insert(0, PKG)
 FL = 6.0
 FH = 2 * FL / 64
 HBM_PEAK_GBS = 8000.0
-
-
-def twins_scalars(N=262144, q=1.0, dens=0.75, spin0=0.18, spin1=0.20, vfac=1.05, rest=0.3):
-    """The sim scalars build() derives for the twins config (src/nbody.js:114-206)."""
-    V1, V2 = 1.0, q / dens
-    vp = (V1 + V2) / N
-    s = ((16 * math.pi * vp / 3) ** (1.0 / 3.0))
-    a = 0.96 * s / (2 * math.sqrt(2))
-    n1 = round(N * V1 / (V1 + V2))
-    M = 2.0                      # mass ratio 1: both bodies one Earth mass in the reference's units
-    mref = 1.0 / (n1 * (0.325 / (2.5 - 0.325 * 1.5) * 2.5 + (1 - 0.325 / (2.5 - 0.325 * 1.5))))
-    Rsum = 1.0 + ((V2) ** (1.0 / 3.0))
-    vesc = math.sqrt(2 * M / Rsum)
-    vc = vfac * vesc
-    vref = max(1.5, vc)
-    k = 4 * mref * vref * vref / (a * a)
-    dt = 0.35 / math.sqrt(2 * k / (mref * min(1, dens)))
-    lnE = math.log(rest)
-    zeta = -lnE / math.sqrt(math.pi ** 2 + lnE ** 2)
-    c = 2 * zeta * math.sqrt(k * mref / 2)
-    return dict(N=N, W=512, G=128, s=s, a=a, n1=n1, m=mref, cell=2.6 * a, dt=dt, k=k, c=c, vc=vc,
-                R1=1.0, R2=((V2) ** (1.0 / 3.0)), spin0=(0, spin0, 0), spin1=(0, spin1 * math.sqrt(dens), 0))
-
-
-def lattice_ball(n, s, rng):
-    """n points of a jittered FCC lattice of cube edge s, nearest to the centre."""
-    r = (3 * n * s ** 3 / (16 * math.pi)) ** (1 / 3) * 1.1
-    m = int(math.ceil(r / s)) + 1
-    g = np.arange(-m, m + 1) * s
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-    base = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
-    offs = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) * s
-    pts = (base[:, None, :] + offs[None]).reshape(-1, 3)
-    d = np.linalg.norm(pts, axis=1)
-    pts = pts[np.argsort(d, kind="stable")[:n]]
-    return pts + rng.uniform(-0.03, 0.03, pts.shape) * s
-
-
-def synthetic_twins(sc, seed=0xC0FFEE):
-    """Two lattice bodies of the twins' sizes, touching at 30 degrees, closing at
-    the contact speed -- the configuration a step sees just after the kick."""
-    rng = np.random.default_rng(seed)
-    N, n1 = sc["N"], sc["n1"]
-    b1 = lattice_ball(n1, sc["s"], rng)
-    b2 = lattice_ball(N - n1, sc["s"], rng)
-    ang = math.radians(30)
-    dist = sc["R1"] + sc["R2"] - 0.05
-    off = np.array([-dist * math.cos(ang), dist * math.sin(ang), 0.0])
-    c0 = -off * 0.5
-    c1 = off * 0.5
-    pos = np.zeros((N, 4), np.float32)
-    pos[:n1, :3] = b1 + c0
-    pos[n1:, :3] = b2 + c1
-    r1 = np.linalg.norm(b1, axis=1)
-    r2 = np.linalg.norm(b2, axis=1)
-    pos[:n1, 3] = np.where(r1 < 0.66 * sc["R1"], 0, np.where(r1 > 0.97 * sc["R1"], 2, 1))
-    pos[n1:, 3] = 4 + np.where(r2 < 0.66 * sc["R2"], 0, np.where(r2 > 0.97 * sc["R2"], 2, 1))
-    mat = np.where(pos[:, 3] % 4 < 0.5, 2.5, 1.0).astype(np.float32)
-    mat[n1:] *= 0.75
-    vel = np.zeros((N, 4), np.float32)
-    vrel = np.array([sc["vc"], 0, 0]) * 0.5
-    vel[:n1, :3] = vrel
-    vel[n1:, :3] = -vrel
-    w0, w1 = np.array(sc["spin0"]), np.array(sc["spin1"])
-    vel[:n1, :3] += np.cross(w0, pos[:n1, :3] - c0)
-    vel[n1:, :3] += np.cross(w1, pos[n1:, :3] - c1)
-    return pos, vel, mat, np.zeros(N, np.float32), c0, c1
-
-
-def morton_order(ctx, core):
-    ctx.morton(core)
-
-
-def box_uniform(cen):
-    L = [FL - float(x) for x in cen]
-    return L + [FL, FL, FL] + [0.0]
 
 
 def cpu_baseline(sc, pos, vel, mat, q, budget_s=15.0):
@@ -138,80 +63,58 @@ def cpu_baseline(sc, pos, vel, mat, q, budget_s=15.0):
                       "262,144-particle state, one core, %.1f s" % (n, el)}
 
 
-TWINS_OPT = {"nIdx": 4, "q": 1.0, "dens": 0.75, "core": 0.325, "spin0": 0.18, "spin1": 0.20, "angle": 30, "incl": 0,
-             "vfac": 1.05, "rest": 0.3}
+def batch_leg(out_dir, count, steps, warmup, reuse):
+    B = runs.BatchRunner(runs.prepare_batch(out_dir, count, reuse=reuse))
+    B.steps(warmup)
+    t0 = time.perf_counter()
+    dev_ms = B.steps(steps)
+    wall = time.perf_counter() - t0
+    sizes = [r.s["N"] for r in B.rs]
+    out = {"value": B.N * steps / wall, "unit": "particle-steps/s", "instances": len(sizes), "particles": B.N,
+           "n_min": min(sizes), "n_max": max(sizes), "steps": steps, "warmup": warmup,
+           "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
+           "workload": "batched what-if sweep: %d giant impacts, 16k-262k particles (log-uniform, multiples of 512), "
+                       "mass ratio 0.02-1, angle 0-75 deg, 1.0-2.6 v_esc, core 0.2-0.4, seeded per instance; "
+                       "one graph launch per 8-step cycle for the whole batch" % len(sizes)}
+    B.close()
+    return out
 
 
-def prepare_twins(out_dir, extra_steps=0, reuse=False):
-    """The twins run through the JavaScript engine's own phases (build, settle,
-    approach, kick, a look with the Morton sort) on this GPU, written to out_dir
-    (or read from there when reuse is set and a run for these options is there)."""
-    have = os.path.exists(os.path.join(out_dir, "run.json")) and \
-        json.load(open(os.path.join(out_dir, "run.json"))).get("opt") == TWINS_OPT
-    if not (reuse and have):
-        cmd = ["node", os.path.join(PKG, "js", "prep_state.js"), json.dumps(TWINS_OPT), out_dir, str(extra_steps)]
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-        sys.stderr.write(r.stdout[-3000:])
-        if r.returncode != 0:
-            raise RuntimeError("state preparation failed:\n" + r.stderr[-4000:])
-    run = json.load(open(os.path.join(out_dir, "run.json")))
+def learned_leg(state_root, steps, warmup, reuse, cycles, iters):
+    """The learned far field: trained on the GPU from the exact far field of
+    the theia-like, twins and shatter runs, then judged in the shatter run
+    (every 16 steps, fallback above a 1e-3 relative force error) beside the
+    same run with the exact far field."""
+    import ff_train
+    ff, rep = ff_train.build(["THEIA", "TWINS", "SHATTER"], state_root, cycles, iters, reuse=reuse,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    run, st = runs.prepare(runs.SHATTER, os.path.join(state_root, "cc_shatter_state"), reuse=True)
     N = run["sim"]["N"]
-    st = {k: np.fromfile(os.path.join(out_dir, "state.%s.f32" % k), np.float32) for k in ("pos", "vel", "q", "mat")}
-    st["pos"] = st["pos"].reshape(N, 4)
-    st["vel"] = st["vel"].reshape(N, 4)
-    st["pptab"] = np.fromfile(os.path.join(out_dir, "pptab.f32"), np.float32).reshape(64, 2)
-    st["pplist"] = np.fromfile(os.path.join(out_dir, "pplist.i32"), np.int32)
-    return run, st
-
-
-class Runner:
-    """The engine's full-phase step loop over the C ABI: boxes() for every
-    mesh step, the softening an hour and a half after first touch
-    (src/nbody.js:538-558), sim.t advanced per step as the caller does."""
-
-    def __init__(self, ccamd, run, st):
-        s = run["sim"]
-        self.ccamd, self.s = ccamd, dict(s)
-        self.p = ccamd.params(N=s["N"], W=s["W"], G=s["G"], m=s["m"], a=s["a"], cell=s["cell"], dt=s["dt"], k=s["k"],
-                              c=s["c"], creep=s["creep"], damp=s["damp"], tK=s["tK"], kappa=s["kappa"],
-                              spin0=s["spin0"], spin1=s["spin1"])
-        self.ctx = ccamd.Context(self.p)
-        self.ctx.upload(st["pos"], st["vel"], st["mat"], st["q"])
-        self.ctx.set_pp_table(st["pptab"])
-        self.ctx.set_pp(st["pplist"])
-        self.args = ccamd.sim_args(bulk0=s["bulk0"], bulk1=s["bulk1"], cen0=s["cen0"], cen1=s["cen1"])
-        self.t, self.step_no = s["t"], s["stepNo"]
-
-    def soften_if_due(self):
-        s = self.s
-        if not s["soft"] and self.t - s["impactT"] > 1.5:
-            s["soft"] = True
-            s["k"] *= 0.5
-            s["c"] *= math.sqrt(0.5)
-            s["dt"] *= math.sqrt(2.0)
-            s["damp"] = math.exp(-s["dt"] * 12)
-            self.p.k, self.p.c, self.p.dt, self.p.damp = s["k"], s["c"], s["dt"], s["damp"]
-            self.ctx.set_params(self.p)
-
-    def steps(self, n):
-        """n steps; returns the device milliseconds (HIP events around the launches)."""
-        ms, done = 0.0, 0
-        while done < n:
-            self.soften_if_due()
-            s, t, chunk, L = self.s, self.t, 0, []
-            # run until the softening would fire (it is checked at each step's start)
-            while done + chunk < n and (chunk == 0 or s["soft"] or t - s["impactT"] <= 1.5):
-                if (self.step_no + chunk) % 8 == 0:
-                    for b in s["boxes"]:
-                        dtb = t - b["t"]
-                        L += [FL - b["cen"][k] - b["vel"][k] * dtb for k in range(3)]
-                    L.append(1.0 if s["boxes"][1]["on"] else 0.0)
-                t += s["dt"]
-                chunk += 1
-            self.ctx.step_no = self.step_no
-            ms += self.ctx.step_n(chunk, self.args, np.array(L, np.float64).reshape(-1, 7) if L else None)
-            self.t, self.step_no, done = t, self.step_no + chunk, done + chunk
-        return ms
+    out = {"workload": "shatter: 262,144 particles, 0.5 M_earth impactor head-on at 2.6 v_esc; the far field from "
+                       "a 4-level 32-channel 3D conv operator (BF16 on the matrix cores) trained here on the GPU, "
+                       "exact far field every 16 steps as judge, fallback above 1e-3 relative force error",
+           "particles": N, "steps": steps, "training": rep}
+    for mode in ("exact", "learned"):
+        R = runs.Runner(run, st)
+        if mode == "learned":
+            R.ctx.set_far_field(ff, 16, 1e-3)
+        R.steps(warmup)
+        R.ctx.sync()
+        t0 = time.perf_counter()
+        R.steps(steps)
+        R.ctx.sync()
+        wall = time.perf_counter() - t0
+        out[mode] = {"value": N * steps / wall, "ms_per_step": wall * 1e3 / steps,
+                     "mesh_ms": R.ctx.time_pass("mesh_between", 20)}
+        if mode == "learned":
+            st_ = R.ctx.ff_stats()
+            out[mode].update({"last_force_err": float(st_[0, 0]), "judged": int(st_[0, 1]),
+                              "fallbacks": int(st_[0, 2]), "learned_mesh_steps": int(st_[0, 3])})
+        R.close()
+    ff.close()
+    out["value"] = out["learned"]["value"]
+    out["unit"] = "particle-steps/s"
+    return out
 
 
 def main():
@@ -220,17 +123,24 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--state-dir", default=os.path.join(REPO, "gpurun_out", "twins_state"))
+    ap.add_argument("--state-dir", default=os.path.join(tempfile.gettempdir(), "cc_twins_state"))
     ap.add_argument("--reuse-state", action="store_true", help="use a prepared state in --state-dir if there is one")
+    ap.add_argument("--batch", type=int, default=64, help="instances in the batched-sweep leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=240)
+    ap.add_argument("--batch-dir", default=os.path.join(tempfile.gettempdir(), "cc_batch_state"))
+    ap.add_argument("--learned", type=int, default=1, help="the learned far field leg (0: skip it)")
+    ap.add_argument("--learned-steps", type=int, default=1000)
+    ap.add_argument("--ff-cycles", type=int, default=48)
+    ap.add_argument("--ff-iters", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)
         return 2
-    import ccamd
+    import runs
 
-    run, st = prepare_twins(args.state_dir, reuse=args.reuse_state)
+    run, st = runs.prepare(runs.TWINS, args.state_dir, reuse=args.reuse_state)
     import torch
-    R = Runner(ccamd, run, st)
+    R = runs.Runner(run, st)
     N = run["sim"]["N"]
     R.steps(args.warmup)
     torch.cuda.synchronize()
@@ -241,14 +151,8 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = N * args.steps / wall
-    # the step pass alone (an odd step: no seats, no mesh), timed with HIP events on the library's stream
-    ev = []
-    saved = R.step_no
-    for _ in range(50):
-        R.ctx.step_no = 1
-        ev.append(R.ctx.step_n(1, R.args))
-    R.ctx.step_no = saved
-    sim_ms = float(np.median(ev))
+    # the step pass alone, 200 back to back between HIP events on the library's stream
+    sim_ms = R.ctx.time_pass("sim", 200)
     alg_bytes = 92.0 * N     # pos+vel+mat+q read, pos+vel+aux+q written: each particle's own data once
     achieved = alg_bytes / (sim_ms * 1e-3) / 1e9
     out = {
@@ -269,7 +173,12 @@ def main():
     if not args.no_cpu:
         P, V, _, Q, M = R.ctx.download()
         out["cpu_baseline"] = cpu_baseline(twins_like(run), P, V, M, Q)
-    R.ctx.close()
+    R.close()
+    if args.batch > 0:
+        out["batched"] = batch_leg(args.batch_dir, args.batch, args.batch_steps, 24, args.reuse_state)
+    if args.learned > 0:
+        out["learned"] = learned_leg(tempfile.gettempdir(), args.learned_steps, 200, args.reuse_state, args.ff_cycles,
+                                     args.ff_iters)
     print(json.dumps(out))
     return 0
 

@@ -36,8 +36,12 @@ class SimArgs(C.Structure):
 EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_upload_state", "cc_download_state",
            "cc_set_boxes", "cc_set_mono", "cc_slots", "cc_read_seats", "cc_deposit", "cc_coarse", "cc_cell", "cc_mesh",
            "cc_read_mesh", "cc_write_mesh", "cc_set_pp_table", "cc_set_pp", "cc_pp", "cc_read_pp", "cc_sim", "cc_step",
-           "cc_step_n", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_couple", "cc_diag", "cc_morton", "cc_books",
-           "cc_sync", "cc_last_step_ms"]
+           "cc_step_n", "cc_set_graphs", "cc_time_pass", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_couple",
+           "cc_diag", "cc_morton", "cc_books", "cc_sync", "cc_last_step_ms", "cc_batch_create", "cc_batch_destroy", "cc_batch_last_error",
+           "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error",
+           "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
+           "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
+           "cc_set_far_field", "cc_ff_stats"]
 
 
 class CoupleArgs(C.Structure):
@@ -71,11 +75,26 @@ def lib():
             "cc_read_pp": ([P, fp, fp, fp], C.c_int), "cc_sim": ([P, C.POINTER(SimArgs)], C.c_int),
             "cc_step": ([P, C.c_int, C.c_int, C.POINTER(SimArgs)], C.c_int),
             "cc_step_n": ([P, C.c_int, C.POINTER(SimArgs), dp, C.c_int], C.c_int),
+            "cc_set_graphs": ([P, C.c_int], C.c_int), "cc_time_pass": ([P, C.c_int, C.c_int, dp], C.c_int),
             "cc_get_step_no": ([P, ip], C.c_int), "cc_set_step_no": ([P, C.c_int], C.c_int),
             "cc_rigid": ([P, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int), "cc_diag": ([P, fp], C.c_int),
             "cc_couple": ([P, C.POINTER(CoupleArgs)], C.c_int),
             "cc_morton": ([P, dp, ip, ip], C.c_int), "cc_books": ([P, dp], C.c_int), "cc_sync": ([P], C.c_int),
             "cc_last_step_ms": ([P, dp], C.c_int),
+            "cc_batch_create": ([C.POINTER(P), C.c_int, C.POINTER(P)], C.c_int), "cc_batch_destroy": ([P], C.c_int),
+            "cc_batch_last_error": ([P], C.c_char_p),
+            "cc_batch_step_n": ([P, C.c_int, C.POINTER(SimArgs), dp, ip], C.c_int),
+            "cc_batch_last_ms": ([P, dp], C.c_int),
+            "cc_ff_create": ([C.c_int, C.POINTER(P)], C.c_int), "cc_ff_destroy": ([P], C.c_int),
+            "cc_ff_last_error": ([P], C.c_char_p), "cc_ff_weight_count": ([C.POINTER(C.c_int64)], C.c_int),
+            "cc_ff_get_weights": ([P, fp, C.c_int64], C.c_int), "cc_ff_set_weights": ([P, fp, C.c_int64], C.c_int),
+            "cc_ff_record": ([P, P, C.c_int], C.c_int), "cc_ff_push_sample": ([P, fp, fp, fp], C.c_int),
+            "cc_ff_read_sample": ([P, C.c_int, fp, fp, fp], C.c_int), "cc_ff_samples": ([P, ip], C.c_int),
+            "cc_ff_clear": ([P], C.c_int),
+            "cc_ff_train": ([P, C.c_int, C.c_int, C.c_double, C.c_uint32, dp], C.c_int),
+            "cc_ff_eval": ([P, C.c_int, C.c_int, fp, dp, dp], C.c_int),
+            "cc_ff_gradient": ([P, C.c_int, C.c_int, fp, dp], C.c_int),
+            "cc_set_far_field": ([P, P, C.c_int, C.c_double], C.c_int), "cc_ff_stats": ([P, dp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -245,6 +264,27 @@ class Context:
         lib().cc_last_step_ms(self.h, C.byref(ms))
         return ms.value
 
+    def time_pass(self, which, reps):
+        """Device ms per pass: which = 'sim' | 'slots' | 'mesh' | 'pp' | 'mesh_between' | 'mesh_judge' (cc_time_pass)."""
+        ms = C.c_double()
+        k = {"sim": 0, "slots": 1, "mesh": 2, "pp": 3, "mesh_between": 4, "mesh_judge": 5}[which]
+        self._chk(lib().cc_time_pass(self.h, k, int(reps), C.byref(ms)), "cc_time_pass")
+        return ms.value
+
+    def set_far_field(self, ff, judge_every=16, tol=1e-3):
+        """Attach a FarField network (None: detach) to this instance's step."""
+        self._chk(lib().cc_set_far_field(self.h, ff.h if ff is not None else None, int(judge_every), float(tol)),
+                  "cc_set_far_field")
+
+    def ff_stats(self):
+        """Per box: last relative force error, judge passes, fallbacks, mesh steps on the network."""
+        out = np.zeros(8, np.float64)
+        self._chk(lib().cc_ff_stats(self.h, _ptr(out, C.c_double)), "cc_ff_stats")
+        return out.reshape(2, 4)
+
+    def set_graphs(self, on):
+        self._chk(lib().cc_set_graphs(self.h, int(bool(on))), "cc_set_graphs")
+
     @property
     def step_no(self):
         v = C.c_int32()
@@ -286,3 +326,127 @@ class Context:
 
     def sync(self):
         self._chk(lib().cc_sync(self.h), "cc_sync")
+
+
+class Batch:
+    """Instances stepped together, one graph launch per mesh cycle for the
+    whole batch (cc_batch_*).  The Contexts must outlive the Batch."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        arr = (C.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = C.c_void_p()
+        r = lib().cc_batch_create(arr, len(self.ctxs), C.byref(h))
+        if r:
+            raise CCError("cc_batch_create failed (%d): %s" % (r, lib().cc_batch_last_error(None).decode()))
+        self.h = h
+
+    def step_n(self, n, args=None, boxL=None):
+        """args: one SimArgs for all or a list; boxL: a list (per instance) of (k, 7) arrays or None."""
+        m = len(self.ctxs)
+        if args is None:
+            args = sim_args()
+        if not isinstance(args, (list, tuple)):
+            args = [args] * m
+        arr = (SimArgs * m)(*args)
+        if boxL is None:
+            r = lib().cc_batch_step_n(self.h, n, arr, None, None)
+        else:
+            parts = [np.ascontiguousarray(b, np.float64).reshape(-1, 7) for b in boxL]
+            flat = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros((0, 7)), np.float64)
+            counts = np.array([len(p) for p in parts], np.int32)
+            r = lib().cc_batch_step_n(self.h, n, arr, _ptr(flat, C.c_double), _ptr(counts, C.c_int32))
+        if r:
+            raise CCError("cc_batch_step_n failed (%d): %s" % (r, lib().cc_batch_last_error(self.h).decode()))
+        ms = C.c_double()
+        lib().cc_batch_last_ms(self.h, C.byref(ms))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().cc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FarField:
+    """The learned far field network (cc_ff_*)."""
+
+    def __init__(self, capacity=0):
+        h = C.c_void_p()
+        r = lib().cc_ff_create(int(capacity), C.byref(h))
+        if r:
+            raise CCError("cc_ff_create failed (%d): %s" % (r, lib().cc_ff_last_error(None).decode()))
+        self.h = h
+        n = C.c_int64()
+        lib().cc_ff_weight_count(C.byref(n))
+        self.nw = n.value
+
+    def _chk(self, r, what):
+        if r:
+            raise CCError("%s failed (%d): %s" % (what, r, lib().cc_ff_last_error(self.h).decode()))
+
+    def weights(self):
+        w = np.empty(self.nw, np.float32)
+        self._chk(lib().cc_ff_get_weights(self.h, _ptr(w), self.nw), "cc_ff_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._chk(lib().cc_ff_set_weights(self.h, _ptr(w), w.size), "cc_ff_set_weights")
+
+    def record(self, ctx, box=0):
+        self._chk(lib().cc_ff_record(self.h, ctx.h, int(box)), "cc_ff_record")
+
+    def push(self, x, target, mask):
+        self._chk(lib().cc_ff_push_sample(self.h, _ptr(_f(x)), _ptr(_f(target)), _ptr(_f(mask))), "cc_ff_push_sample")
+
+    def sample(self, k):
+        x, t, m = np.empty((4096, 16), np.float32), np.empty((4096, 16), np.float32), np.empty(4096, np.float32)
+        self._chk(lib().cc_ff_read_sample(self.h, int(k), _ptr(x), _ptr(t), _ptr(m)), "cc_ff_read_sample")
+        return x, t, m
+
+    @property
+    def count(self):
+        n = C.c_int()
+        self._chk(lib().cc_ff_samples(self.h, C.byref(n)), "cc_ff_samples")
+        return n.value
+
+    def clear(self):
+        self._chk(lib().cc_ff_clear(self.h), "cc_ff_clear")
+
+    def train(self, iters, batch=16, lr=1e-3, seed=1):
+        losses = np.empty(int(iters), np.float64)
+        self._chk(lib().cc_ff_train(self.h, int(iters), int(batch), float(lr), int(seed), _ptr(losses, C.c_double)),
+                  "cc_ff_train")
+        return losses
+
+    def eval(self, first=0, count=None, outputs=False):
+        count = self.count - first if count is None else count
+        out = np.empty((count, 4096, 16), np.float32) if outputs else None
+        ra, rl = C.c_double(), C.c_double()
+        self._chk(lib().cc_ff_eval(self.h, int(first), int(count), _ptr(out), C.byref(ra), C.byref(rl)), "cc_ff_eval")
+        return (ra.value, rl.value, out) if outputs else (ra.value, rl.value)
+
+    def gradient(self, first, count):
+        g = np.empty(self.nw, np.float32)
+        loss = C.c_double()
+        self._chk(lib().cc_ff_gradient(self.h, int(first), int(count), _ptr(g), C.byref(loss)), "cc_ff_gradient")
+        return g, loss.value
+
+    def close(self):
+        if self.h:
+            lib().cc_ff_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -11,6 +11,16 @@
 
 using namespace cc;
 
+// One captured step cycle (meshEvery steps from a mesh step) and the host
+// bookkeeping it leaves: the key is everything its launches bake in.
+struct CycleGraph {
+  std::vector<double> key;
+  hipGraphExec_t exec = nullptr;
+  int curAfter = 0, gridCurAfter = 0;
+};
+
+constexpr int SCHED_CAP = 4096;   // step cycles per schedule upload
+
 struct cc_ctx {
   cc_params p;
   hipStream_t s = nullptr;
@@ -37,7 +47,25 @@ struct cc_ctx {
   float4* diag = nullptr;
   int *mkeys = nullptr, *mslot = nullptr, *mhist = nullptr, *mblock = nullptr, *mperm = nullptr, *minto = nullptr;
   double *bpart = nullptr, *bout = nullptr;
-  Box box;
+  Box box;                 // host copy of the boxes last set (decides whether box B's mesh is built)
+  Box* boxDev = nullptr;   // the boxes the kernels read
+  Box* boxSched = nullptr; // a step_n call's boxes, one per step cycle, taken in turn by k_box_next
+  int* boxIdx = nullptr;
+  Box* boxHost = nullptr;  // pinned staging of the schedule
+  CycleGraph graphs[4];    // captured step cycles, by what their launches bake in
+  int nextGraph = 0;
+  int useGraphs = 1;
+  // the learned far field, when attached: network, workspace, judge cadence and bound, and per box the gate
+  // (1: the network stands in), stats {last error, judged, fallbacks, learned mesh steps}, error sums
+  FFNet* ff = nullptr;
+  FFWork* ffWork = nullptr;
+  int ffJudge = 16;
+  float ffTol = 1e-3f;
+  int* ffGate = nullptr;
+  float* ffStats = nullptr;
+  double* ffSums = nullptr;
+  float4* ffFar = nullptr;     // 3 x 4096: the network's farA, farT0, farT1
+  float4* ffForce = nullptr;   // the force mesh from the network's far field (judge steps)
   int stepNo = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastMs = 0.0;
@@ -120,7 +148,6 @@ static SimU sim_uniforms(const cc_ctx* c, const cc_sim_args* a) {
   u.cpIron = (float)p.cpIron;
   for (int k = 0; k < 3; k++) { u.spin0[k] = (float)p.spin0[k]; u.spin1[k] = (float)p.spin1[k]; }
   u.grid = c->grid;
-  u.box = c->box;
   u.gm = (float)p.m;
   u.eps2 = (float)(FH * FH);
   u.field = p.field;
@@ -184,6 +211,9 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
     ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4);
     ALLOC(c->bpart, 256 * 13 * 8); ALLOC(c->bout, 13 * 8);
+    ALLOC(c->boxDev, sizeof(Box)); ALLOC(c->boxIdx, 4);
+    ALLOC(c->boxSched, (size_t)SCHED_CAP * sizeof(Box));
+    HIPCHK(c, hipHostMalloc((void**)&c->boxHost, (size_t)SCHED_CAP * sizeof(Box), hipHostMallocDefault));
     // the empty state: seats empty, counters and accumulators zero, meshes zero
     launch_fill_u32(c->gb.seats, c->cells * SLOTS, EMPTY_SEAT, c->s);
     HIPCHK(c, hipMemsetAsync(c->gb.cnt, 0, c->cells * 4, c->s));
@@ -218,6 +248,8 @@ int cc_create(const cc_params* p, cc_ctx** out) {
   if (rr) { g_err = c->err; cc_destroy(c); return rr; }
   memset(&c->box, 0, sizeof c->box);
   c->box.invH = (float)(1.0 / FH);
+  launch_put_box(c->box, c->boxDev, c->s);
+  if (hipStreamSynchronize(c->s) != hipSuccess) { g_err = "box upload failed"; cc_destroy(c); return CC_EDEVICE; }
   *out = c;
   return 0;
 }
@@ -225,6 +257,9 @@ int cc_create(const cc_params* p, cc_ctx** out) {
 int cc_destroy(cc_ctx* c) {
   if (!c) return 0;
   if (c->s) hipStreamSynchronize(c->s);
+  for (CycleGraph& g : c->graphs) if (g.exec) hipGraphExecDestroy(g.exec);
+  if (c->ffWork) ffwork_destroy(c->ffWork);
+  if (c->boxHost) hipHostFree(c->boxHost);
   for (void* p : c->allocs) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -270,6 +305,8 @@ int cc_set_boxes(cc_ctx* c, const double LA[3], const double LB[3], int boxB) {
   for (int k = 0; k < 3; k++) { c->box.LA[k] = (float)LA[k]; c->box.LB[k] = (float)LB[k]; }
   c->box.boxB = boxB ? 1.0f : 0.0f;
   c->box.invH = (float)(1.0 / FH);
+  launch_put_box(c->box, c->boxDev, c->s);
+  LAUNCHCHK(c);
   return 0;
 }
 
@@ -338,7 +375,7 @@ int cc_read_seats(cc_ctx* c, uint32_t* seatA, uint32_t* seatB) {
 
 int cc_deposit(cc_ctx* c, int box) {
   if (!c || box < 0 || box > 1) return CC_EINVAL;
-  launch_deposit(c->pos[c->cur], c->mat, c->N, c->box, box, (float)(FH * FH), c->mesh[box], c->s);
+  launch_deposit(c->pos[c->cur], c->mat, c->N, c->boxDev, box, (float)(FH * FH), c->mesh[box], c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -354,10 +391,35 @@ int cc_cell(cc_ctx* c, int box) {
   LAUNCHCHK(c);
   return 0;
 }
-static int mesh_all(cc_ctx* c) {
+// Deposit, coarse and per-cell passes of each box that is up.  With a
+// learned far field attached, stepIdx (the step of this mesh pass) decides:
+// on a judge step the exact far field makes the force mesh and the network's
+// is measured against it, which sets the gate; otherwise the gate picks the
+// exact far field or the network's.  stepIdx < 0: exact only.
+static int mesh_all(cc_ctx* c, int stepIdx = -1) {
   int r;
-  if ((r = cc_deposit(c, 0)) || (r = cc_coarse(c, 0)) || (r = cc_cell(c, 0))) return r;
-  if (c->box.boxB > 0.5f && ((r = cc_deposit(c, 1)) || (r = cc_coarse(c, 1)) || (r = cc_cell(c, 1)))) return r;
+  for (int b = 0; b < 2; b++) {
+    if (b == 1 && !(c->box.boxB > 0.5f)) break;
+    if ((r = cc_deposit(c, b)) || (r = cc_coarse(c, b))) return r;
+    MeshBuf& m = c->mesh[b];
+    float eps2 = (float)(0.5 * FH * FH), gm = (float)c->p.m;
+    if (!c->ff || stepIdx < 0) {
+      launch_cell(m, eps2, gm, c->s);
+    } else if (stepIdx % c->ffJudge == 0) {
+      float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
+      launch_far(m.coarse, m.farA, m.farT0, m.farT1, nullptr, c->s);
+      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->s);
+      ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, A, T0, T1, nullptr, c->s);
+      launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->s);
+      launch_judge(c->ffForce, m.force, c->ffSums + 2 * b, c->ffTol, c->ffStats + 4 * b, c->ffGate + b, c->s);
+    } else {
+      launch_far(m.coarse, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
+      ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
+      launch_count_learned(c->ffGate + b, c->ffStats + 4 * b, c->s);
+      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->s);
+    }
+    LAUNCHCHK(c);
+  }
   return 0;
 }
 int cc_mesh(cc_ctx* c) {
@@ -375,7 +437,7 @@ int cc_set_pp_table(cc_ctx* c, const float* tab) {
 int cc_pp(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   if (!c->ppCount) return 0;
-  launch_pp(c->pos[c->cur], c->mat, c->box, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+  launch_pp(c->pos[c->cur], c->mat, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
             (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppPartial, c->ppForce, c->s);
   LAUNCHCHK(c);
   return 0;
@@ -408,8 +470,8 @@ int cc_read_pp(cc_ctx* c, float* list, float* box, float* force) {
 static int sim_pass(cc_ctx* c, const cc_sim_args* a) {
   SimU u = sim_uniforms(c, a);
   int s = c->cur, d = c->cur ^ 1;
-  launch_sim(u, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot, c->ppForce,
-             c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
+  launch_sim(u, c->boxDev, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot,
+             c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
   LAUNCHCHK(c);
   c->cur = d;
   return 0;
@@ -432,7 +494,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
     return sim_pass(c, &b);
   }
   bool mesh = n % c->p.meshEvery == 0;
-  if (mesh && (r = mesh_all(c))) return r;
+  if (mesh && (r = mesh_all(c, n))) return r;
   if (mesh && (r = cc_pp(c))) return r;
   cc_sim_args b = *a;
   b.settle = settle;
@@ -446,23 +508,184 @@ int cc_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   return one_step(c, settle, kick, a);
 }
 
-int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nBoxL) {
-  if (!c || !a || n < 0) return CC_EINVAL;
-  HIPCHK(c, hipEventRecord(c->ev0, c->s));
-  int used = 0;
-  for (int i = 0; i < n; i++) {
-    if (c->stepNo % c->p.meshEvery == 0 && boxL && used < nBoxL) {
-      const double* L = boxL + 7 * used++;
-      cc_set_boxes(c, L, L + 3, L[6] > 0.5);
-    }
-    int r = one_step(c, a->settle, a->kick, a);
-    if (r) return r;
+// What a captured cycle's launches bake in: the uniforms (params and the
+// call's settle/kick/bulk/centre values), the loose-list count (its grid), box
+// B (whether its mesh is built), the parities of the state and seat-list
+// double buffers and the mass array (swapped by the Morton permutation).
+static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
+  const cc_params& p = c->p;
+  std::vector<double> k = { (double)p.N, (double)p.G, p.m, p.a, p.cell, p.dt, p.k, p.c, p.creep, p.damp, p.tK,
+                            p.kappa, p.cpIron, p.seatSlack, (double)p.field, (double)p.frozenOn, p.frozen,
+                            (double)p.meshEvery, (double)a->settle, (double)a->kick };
+  for (int i = 0; i < 3; i++) {
+    k.push_back(p.spin0[i]); k.push_back(p.spin1[i]); k.push_back(p.gField[i]);
+    k.push_back(a->bulk0[i]); k.push_back(a->bulk1[i]); k.push_back(a->cen0[i]); k.push_back(a->cen1[i]);
   }
+  k.push_back((double)c->ppCount);
+  k.push_back((double)(c->box.boxB > 0.5f));
+  k.push_back((double)c->cur);
+  k.push_back((double)c->gb.cur);
+  k.push_back((double)(uintptr_t)c->mat);
+  k.push_back((double)(uintptr_t)c->ff);
+  if (c->ff) {
+    k.push_back((double)c->ffJudge);
+    k.push_back((double)c->ffTol);
+    k.push_back((double)(c->stepNo % c->ffJudge));
+  }
+  return k;
+}
+
+// The cycle graph for the ctx's present state, captured on first use: the
+// box of the cycle (k_box_next), then meshEvery steps from a mesh step.  The
+// capture runs the host side of the steps without executing them, so the
+// host bookkeeping is put back and the graph's net effect recorded.
+static int cycle_graph(cc_ctx* c, const cc_sim_args* a, CycleGraph** out) {
+  std::vector<double> key = cycle_key(c, a);
+  for (CycleGraph& g : c->graphs)
+    if (g.exec && g.key == key) { *out = &g; return 0; }
+  CycleGraph& g = c->graphs[c->nextGraph];
+  c->nextGraph = (c->nextGraph + 1) % 4;
+  if (g.exec) { hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+  int stepNo = c->stepNo, cur = c->cur, gcur = c->gb.cur;
+  HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+  launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
+  int r = 0;
+  for (int i = 0; i < c->p.meshEvery && !r; i++) r = one_step(c, a->settle, a->kick, a);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(c->s, &graph);
+  g.curAfter = c->cur;
+  g.gridCurAfter = c->gb.cur;
+  c->stepNo = stepNo; c->cur = cur; c->gb.cur = gcur;
+  if (r) { if (graph) hipGraphDestroy(graph); return r; }
+  if (e != hipSuccess) FAIL(c, CC_EDEVICE, "step cycle capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) { g.exec = nullptr; FAIL(c, CC_EDEVICE, "step cycle instantiate: %s", hipGetErrorString(e)); }
+  g.key = key;
+  *out = &g;
+  return 0;
+}
+
+// n steps: eager steps up to the next mesh step, whole cycles as replays of
+// the captured cycle graph (one launch per meshEvery steps, each cycle's box
+// from the schedule uploaded once for the call), eager steps for the rest.
+int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nBoxL) {
+  if (!c || !a || n < 0 || nBoxL < 0) return CC_EINVAL;
+  HIPCHK(c, hipEventRecord(c->ev0, c->s));
+  const int M = c->p.meshEvery;
+  int used = 0, i = 0, r;
+  auto next_box = [&](Box& b) {
+    b = c->box;
+    if (boxL && used < nBoxL) {
+      const double* L = boxL + 7 * used++;
+      for (int k = 0; k < 3; k++) { b.LA[k] = (float)L[k]; b.LB[k] = (float)L[3 + k]; }
+      b.boxB = L[6] > 0.5 ? 1.0f : 0.0f;
+      b.invH = (float)(1.0 / FH);
+    }
+  };
+  auto eager = [&]() -> int {
+    if (c->stepNo % M == 0 && boxL && used < nBoxL) {
+      const double* L = boxL + 7 * used++;
+      int e = cc_set_boxes(c, L, L + 3, L[6] > 0.5);
+      if (e) return e;
+    }
+    return one_step(c, a->settle, a->kick, a);
+  };
+  while (i < n && c->stepNo % M != 0) { if ((r = eager())) return r; i++; }
+  int cycles = c->useGraphs ? (n - i) / M : 0;
+  while (cycles > 0) {   // in chunks of the schedule's capacity; the staging is reused once the last upload is done
+    int chunk = cycles < SCHED_CAP ? cycles : SCHED_CAP;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (int k = 0; k < chunk; k++) next_box(c->boxHost[k]);
+    HIPCHK(c, hipMemcpyAsync(c->boxSched, c->boxHost, (size_t)chunk * sizeof(Box), hipMemcpyHostToDevice, c->s));
+    launch_fill_u32(reinterpret_cast<uint32_t*>(c->boxIdx), 1, 0u, c->s);
+    LAUNCHCHK(c);
+    for (int k = 0; k < chunk; k++) {
+      c->box = c->boxHost[k];
+      CycleGraph* g = nullptr;
+      if ((r = cycle_graph(c, a, &g))) return r;
+      HIPCHK(c, hipGraphLaunch(g->exec, c->s));
+      c->cur = g->curAfter;
+      c->gb.cur = g->gridCurAfter;
+      c->stepNo += M;
+    }
+    i += chunk * M;
+    cycles -= chunk;
+  }
+  while (i < n) { if ((r = eager())) return r; i++; }
   HIPCHK(c, hipEventRecord(c->ev1, c->s));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->lastMs = ms;
+  return 0;
+}
+
+int cc_time_pass(cc_ctx* c, int pass, int reps, double* ms) {
+  if (!c || !ms || reps <= 0 || pass < 0 || pass > 5) return CC_EINVAL;
+  cc_sim_args a;
+  memset(&a, 0, sizeof a);
+  HIPCHK(c, hipEventRecord(c->ev0, c->s));
+  for (int i = 0; i < reps; i++) {
+    int r = pass == 0 ? sim_pass(c, &a) : pass == 1 ? cc_slots(c) : pass == 2 ? mesh_all(c) : pass == 3 ? cc_pp(c)
+          : mesh_all(c, pass == 4 ? c->p.meshEvery : 0);
+    if (r) return r;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->s));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float t = 0;
+  HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t / reps;
+  return 0;
+}
+
+int cc_set_far_field(cc_ctx* c, cc_ff* f, int judgeEvery, double tol) {
+  if (!c || (f && (judgeEvery <= 0 || judgeEvery % c->p.meshEvery != 0 || !(tol > 0))))
+    FAIL(c, CC_EINVAL, "judge cadence must be a positive multiple of meshEvery and the bound positive");
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  if (f && !c->ffWork) {
+    std::string err;
+    c->ffWork = ffwork_create(err);
+    if (!c->ffWork) FAIL(c, CC_EDEVICE, "learned far field workspace: %s", err.c_str());
+    ALLOC(c->ffGate, 2 * 4); ALLOC(c->ffStats, 8 * 4); ALLOC(c->ffSums, 4 * 8);
+    ALLOC(c->ffFar, 3 * 4096 * 16); ALLOC(c->ffForce, (size_t)FINE_TEXELS * 16);
+  }
+  if (f) {
+    HIPCHK(c, hipMemsetAsync(c->ffGate, 0, 8, c->s));
+    HIPCHK(c, hipMemsetAsync(c->ffStats, 0, 32, c->s));
+    HIPCHK(c, hipMemsetAsync(c->ffSums, 0, 32, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+  }
+  c->ff = f ? ff_net(f) : nullptr;
+  c->ffJudge = judgeEvery;
+  c->ffTol = (float)tol;
+  return 0;
+}
+
+int cc_ff_stats(cc_ctx* c, double* out) {
+  if (!c || !out) return CC_EINVAL;
+  float st[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+  if (c->ffStats) {
+    HIPCHK(c, hipMemcpyAsync(st, c->ffStats, 32, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+  }
+  for (int k = 0; k < 8; k++) out[k] = st[k];
+  return 0;
+}
+
+int cc_ff_record(cc_ff* f, cc_ctx* c, int box) {
+  if (!f || !c || box < 0 || box > 1) return CC_EINVAL;
+  MeshBuf& m = c->mesh[box];
+  if (ff_record(ff_net(f), m.coarse, m.mono, c->boxDev, box, m.farA, m.farT0, m.farT1, c->s) < 0)
+    FAIL(c, CC_ESTATE, "the learned far field's sample record is full");
+  LAUNCHCHK(c);
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_set_graphs(cc_ctx* c, int on) {
+  if (!c) return CC_EINVAL;
+  c->useGraphs = on ? 1 : 0;
   return 0;
 }
 
@@ -511,8 +734,8 @@ int cc_couple(cc_ctx* c, const cc_couple_args* a) {
 int cc_diag(cc_ctx* c, float* out) {
   if (!c) return CC_EINVAL;
   SimU u = sim_uniforms(c, nullptr);
-  launch_diag(u, c->pos[c->cur], c->q[c->cur], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot, c->ppForce, c->diag,
-              c->s);
+  launch_diag(u, c->boxDev, c->pos[c->cur], c->q[c->cur], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot,
+              c->ppForce, c->diag, c->s);
   LAUNCHCHK(c);
   if (out) {
     HIPCHK(c, hipMemcpyAsync(out, c->diag, (size_t)c->N * 16, hipMemcpyDeviceToHost, c->s));
@@ -557,6 +780,219 @@ int cc_sync(cc_ctx* c) {
 int cc_last_step_ms(cc_ctx* c, double* ms) {
   if (!c || !ms) return CC_EINVAL;
   *ms = c->lastMs;
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch --
+// Independent instances stepped together: each instance's step cycle is one
+// branch of a single captured graph (forked from and joined to the batch's
+// stream), so a cycle of meshEvery steps of every instance is one launch.
+struct cc_batch {
+  std::vector<cc_ctx*> ctx;
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> join;
+  std::vector<double> key;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> curAfter, gridCurAfter;
+  double lastMs = 0.0;
+  std::string err;
+};
+
+#define BFAIL(b, code, ...)                       \
+  do {                                            \
+    char _b[512];                                 \
+    snprintf(_b, sizeof _b, __VA_ARGS__);         \
+    (b)->err = _b;                                \
+    return code;                                  \
+  } while (0)
+#define BCHK(b, call)                                                                         \
+  do {                                                                                        \
+    hipError_t _e = (call);                                                                   \
+    if (_e != hipSuccess) BFAIL(b, CC_EDEVICE, "%s: %s", #call, hipGetErrorString(_e));       \
+  } while (0)
+
+static int batch_graph(cc_batch* b, const cc_sim_args* args) {
+  std::vector<double> key;
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    std::vector<double> k = cycle_key(b->ctx[i], &args[i]);
+    key.insert(key.end(), k.begin(), k.end());
+  }
+  if (b->exec && key == b->key) return 0;
+  if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  size_t n = b->ctx.size();
+  std::vector<int> stepNo(n), cur(n), gcur(n);
+  for (size_t i = 0; i < n; i++) { stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; }
+  BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipEventRecord(b->fork, b->s);
+  int r = 0;
+  for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
+    cc_ctx* c = b->ctx[i];
+    e = hipStreamWaitEvent(c->s, b->fork, 0);
+    if (e != hipSuccess) break;
+    launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
+    for (int k = 0; k < c->p.meshEvery && !r; k++) r = one_step(c, args[i].settle, args[i].kick, &args[i]);
+    if (r) { b->err = c->err; break; }
+    e = hipEventRecord(b->join[i], c->s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->s, b->join[i], 0);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t ee = hipStreamEndCapture(b->s, &graph);
+  b->curAfter.assign(n, 0);
+  b->gridCurAfter.assign(n, 0);
+  for (size_t i = 0; i < n; i++) {
+    cc_ctx* c = b->ctx[i];
+    b->curAfter[i] = c->cur; b->gridCurAfter[i] = c->gb.cur;
+    c->stepNo = stepNo[i]; c->cur = cur[i]; c->gb.cur = gcur[i];
+  }
+  if (r || e != hipSuccess || ee != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    if (r) return r;
+    BFAIL(b, CC_EDEVICE, "batch capture: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+  }
+  e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) { b->exec = nullptr; BFAIL(b, CC_EDEVICE, "batch instantiate: %s", hipGetErrorString(e)); }
+  b->key = key;
+  return 0;
+}
+
+extern "C" {
+
+int cc_batch_create(cc_ctx* const* ctxs, int n, cc_batch** out) {
+  if (!out) return CC_EINVAL;
+  *out = nullptr;
+  if (!ctxs || n <= 0) { g_err = "batch needs at least one instance"; return CC_EINVAL; }
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i]) { g_err = "batch instance is NULL"; return CC_EINVAL; }
+    if (ctxs[i]->p.meshEvery != ctxs[0]->p.meshEvery) { g_err = "batch instances differ in meshEvery"; return CC_EINVAL; }
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) { g_err = "an instance appears twice in the batch"; return CC_EINVAL; }
+  }
+  cc_batch* b = new cc_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  b->join.assign(n, nullptr);
+  hipError_t e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  for (int i = 0; i < n && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+  if (e != hipSuccess) { g_err = std::string("batch create: ") + hipGetErrorString(e); cc_batch_destroy(b); return CC_EDEVICE; }
+  *out = b;
+  return 0;
+}
+
+int cc_batch_destroy(cc_batch* b) {
+  if (!b) return 0;
+  if (b->s) hipStreamSynchronize(b->s);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (hipEvent_t e : b->join) if (e) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->s) hipStreamDestroy(b->s);
+  delete b;
+  return 0;
+}
+
+const char* cc_batch_last_error(cc_batch* b) { return b ? b->err.c_str() : g_err.c_str(); }
+
+int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* boxL, const int* nBoxL) {
+  if (!b || !args || n < 0) return CC_EINVAL;
+  size_t m = b->ctx.size();
+  const int M = b->ctx[0]->p.meshEvery;
+  int phase = b->ctx[0]->stepNo % M;
+  for (size_t i = 1; i < m; i++)
+    if (b->ctx[i]->stepNo % M != phase) BFAIL(b, CC_ESTATE, "instance %zu is at another step of the mesh cycle", i);
+  std::vector<const double*> L(m, nullptr);
+  std::vector<int> nL(m, 0), used(m, 0);
+  if (boxL && nBoxL) {
+    size_t off = 0;
+    for (size_t i = 0; i < m; i++) {
+      if (nBoxL[i] < 0) BFAIL(b, CC_EINVAL, "negative box count");
+      L[i] = boxL + 7 * off; nL[i] = nBoxL[i]; off += (size_t)nBoxL[i];
+    }
+  }
+  BCHK(b, hipEventRecord(b->ev0, b->s));
+  for (size_t i = 0; i < m; i++) {   // earlier work of each instance before anything of this call
+    BCHK(b, hipEventRecord(b->join[i], b->ctx[i]->s));
+    BCHK(b, hipStreamWaitEvent(b->s, b->join[i], 0));
+  }
+  auto eager = [&](size_t i) -> int {
+    cc_ctx* c = b->ctx[i];
+    if (c->stepNo % M == 0 && L[i] && used[i] < nL[i]) {
+      const double* q = L[i] + 7 * used[i]++;
+      int e = cc_set_boxes(c, q, q + 3, q[6] > 0.5);
+      if (e) { b->err = c->err; return e; }
+    }
+    int e = one_step(c, args[i].settle, args[i].kick, &args[i]);
+    if (e) b->err = c->err;
+    return e;
+  };
+  // eager on the batch stream (ordered after every instance's earlier work) up to the cycle start
+  int i0 = 0, r;
+  while (i0 < n && b->ctx[0]->stepNo % M != 0) {
+    for (size_t i = 0; i < m; i++) {
+      hipStream_t own = b->ctx[i]->s;
+      b->ctx[i]->s = b->s;
+      r = eager(i);
+      b->ctx[i]->s = own;
+      if (r) return r;
+    }
+    i0++;
+  }
+  int cycles = (n - i0) / M;
+  while (cycles > 0) {
+    int chunk = cycles < SCHED_CAP ? cycles : SCHED_CAP;
+    BCHK(b, hipStreamSynchronize(b->s));
+    for (size_t i = 0; i < m; i++) {
+      cc_ctx* c = b->ctx[i];
+      for (int k = 0; k < chunk; k++) {
+        Box& x = c->boxHost[k];
+        x = c->box;
+        if (L[i] && used[i] < nL[i]) {
+          const double* q = L[i] + 7 * used[i]++;
+          for (int t = 0; t < 3; t++) { x.LA[t] = (float)q[t]; x.LB[t] = (float)q[3 + t]; }
+          x.boxB = q[6] > 0.5 ? 1.0f : 0.0f;
+          x.invH = (float)(1.0 / FH);
+        }
+      }
+      BCHK(b, hipMemcpyAsync(c->boxSched, c->boxHost, (size_t)chunk * sizeof(Box), hipMemcpyHostToDevice, b->s));
+      launch_fill_u32(reinterpret_cast<uint32_t*>(c->boxIdx), 1, 0u, b->s);
+    }
+    for (int k = 0; k < chunk; k++) {
+      for (size_t i = 0; i < m; i++) b->ctx[i]->box = b->ctx[i]->boxHost[k];
+      if ((r = batch_graph(b, args))) return r;
+      BCHK(b, hipGraphLaunch(b->exec, b->s));
+      for (size_t i = 0; i < m; i++) {
+        cc_ctx* c = b->ctx[i];
+        c->cur = b->curAfter[i]; c->gb.cur = b->gridCurAfter[i]; c->stepNo += M;
+      }
+    }
+    i0 += chunk * M;
+    cycles -= chunk;
+  }
+  for (; i0 < n; i0++)
+    for (size_t i = 0; i < m; i++) {
+      hipStream_t own = b->ctx[i]->s;
+      b->ctx[i]->s = b->s;
+      r = eager(i);
+      b->ctx[i]->s = own;
+      if (r) return r;
+    }
+  BCHK(b, hipEventRecord(b->ev1, b->s));
+  BCHK(b, hipEventSynchronize(b->ev1));
+  float ms = 0;
+  BCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->lastMs = ms;
+  return 0;
+}
+
+int cc_batch_last_ms(cc_batch* b, double* ms) {
+  if (!b || !ms) return CC_EINVAL;
+  *ms = b->lastMs;
   return 0;
 }
 

@@ -1,0 +1,996 @@
+// The learned far field (no reference counterpart): a 3D convolutional
+// operator on the 16^3 coarse mesh that stands in for the block-to-block far
+// field (FAR_FS, src/glsl-sim.js:263-295): coarse masses and centres of mass in,
+// each block's far-field acceleration, potential and tidal tensor out.
+//
+// Network (weights FP32 masters, BF16 on the matrix cores, FP32 accumulation):
+//   4 levels 16^3 / 8^3 / 4^3 / 2^3, 32 channels, 3x3x3 convolutions, leaky
+//   ReLU (slope 1/4), 2^3 average pooling down, nearest upsampling plus the
+//   level's skip up, a linear 32 -> 16 output convolution (10 used).
+//     c0  16->32  c1 32->32            (16^3)  e0
+//     c2, c3                            (8^3)  e1 = from pool(e0)
+//     c4, c5                            (4^3)  e2 = from pool(e1)
+//     c6, c7                            (2^3)  bottom = from pool(e2)
+//     c8  on up(bottom) + e2            (4^3)
+//     c9  on up(c8) + e1                (8^3)
+//     c10 on up(c9) + e0               (16^3)
+//     c11 32->16 linear                (16^3)  out
+// Activations are channels-last BF16, [instance][z][y][x][channel].
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16bf16_1k: a wavefront
+// owns 16 output voxels x all output channels and walks K = 27 taps x input
+// channels in steps of 16.  The data gradient is the same kernel on the
+// flipped, transposed weights; the weight gradient reduces over voxels, one
+// wavefront per (tap, 256-voxel chunk), into FP32 with atomics.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cc_hip.h"
+#include "cc_internal.h"
+
+namespace cc {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float leaky(float x) { return x > 0.0f ? x : 0.25f * x; }
+
+// ------------------------------------------------------------ convolution --
+// in  [M voxels][CIN] bf16 (M = B*S^3), wt [27][CIN/16][COUT][16] bf16,
+// bias [COUT] f32 or null; act 1: leaky ReLU.  Writes out (bf16, [M][COUT])
+// and/or outf (f32), and pre (f32 pre-activation) when asked.
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv(const uint16_t* __restrict__ in, int B, int S,
+                                              const uint16_t* __restrict__ wt, const float* __restrict__ bias,
+                                              int act, uint16_t* out, float* outf, float* pre,
+                                              const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  int lane = threadIdx.x & 63;
+  int S3 = S * S * S, M = B * S3;
+  int m0 = wave * 16;
+  if (m0 >= M) return;
+  int row = lane & 15, kq = (lane >> 4) * 4;
+  int v = m0 + row;
+  bool vin = v < M;
+  int b = v / S3, r = v - b * S3;
+  int z = r / (S * S), y = (r / S) % S, x = r % S;
+  f32x4 acc[COUT / 16];
+  for (int n = 0; n < COUT / 16; n++) acc[n] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+  const bf16x4 zero = bf16x4{ 0, 0, 0, 0 };
+  for (int t = 0; t < 27; t++) {
+    int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
+    bool ok = vin && zz >= 0 && zz < S && yy >= 0 && yy < S && xx >= 0 && xx < S;
+    size_t src = ok ? ((size_t)b * S3 + ((size_t)zz * S + yy) * S + xx) * CIN : 0;
+    for (int kc = 0; kc < CIN / 16; kc++) {
+      bf16x4 a = ok ? *reinterpret_cast<const bf16x4*>(in + src + kc * 16 + kq) : zero;
+      const uint16_t* wp = wt + ((size_t)(t * (CIN / 16) + kc) * COUT + row) * 16 + kq;
+      for (int n = 0; n < COUT / 16; n++) {
+        bf16x4 w = *reinterpret_cast<const bf16x4*>(wp + (size_t)n * 16 * 16);
+        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc[n], 0, 0, 0);
+      }
+    }
+  }
+  for (int n = 0; n < COUT / 16; n++) {
+    int co = n * 16 + row;
+    float bb = bias ? bias[co] : 0.0f;
+    for (int j = 0; j < 4; j++) {
+      int vo = m0 + kq + j;
+      if (vo >= M) continue;
+      float p = acc[n][j] + bb;
+      size_t o = (size_t)vo * COUT + co;
+      if (pre) pre[o] = p;
+      float yv = act ? leaky(p) : p;
+      if (out) out[o] = f2bf(yv);
+      if (outf) outf[o] = yv;
+    }
+  }
+}
+
+// dW[t][ci][co] += sum over voxels v of in[v + t][ci] * g[v][co] (g bf16, the
+// gradient at the convolution's output); one wavefront per (tap, chunk of 256).
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv_wgrad(const uint16_t* __restrict__ in, int B, int S,
+                                                    const uint16_t* __restrict__ g, float* dW) {
+  int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  int lane = threadIdx.x & 63;
+  int S3 = S * S * S, M = B * S3;
+  int chunks = (M + 255) / 256;
+  int t = wave / chunks, ch = wave - t * chunks;
+  if (t >= 27) return;
+  int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+  int row = lane & 15, kq = (lane >> 4) * 4;
+  f32x4 acc[CIN / 16][COUT / 16];
+  for (int i = 0; i < CIN / 16; i++)
+    for (int n = 0; n < COUT / 16; n++) acc[i][n] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+  for (int k0 = ch * 256; k0 < ch * 256 + 256 && k0 < M; k0 += 16) {
+    // this lane's four voxels of the K step
+    bf16x4 av[CIN / 16], gv[COUT / 16];
+    for (int i = 0; i < CIN / 16; i++) av[i] = bf16x4{ 0, 0, 0, 0 };
+    for (int n = 0; n < COUT / 16; n++) gv[n] = bf16x4{ 0, 0, 0, 0 };
+    for (int j = 0; j < 4; j++) {
+      int v = k0 + kq + j;
+      if (v >= M) continue;
+      int b = v / S3, r = v - b * S3;
+      int z = r / (S * S) + dz, y = (r / S) % S + dy, x = r % S + dx;
+      for (int n = 0; n < COUT / 16; n++) gv[n][j] = (short)g[(size_t)v * COUT + n * 16 + row];
+      if (z < 0 || z >= S || y < 0 || y >= S || x < 0 || x >= S) continue;
+      size_t src = ((size_t)b * S3 + ((size_t)z * S + y) * S + x) * CIN;
+      for (int i = 0; i < CIN / 16; i++) av[i][j] = (short)in[src + i * 16 + row];
+    }
+    for (int i = 0; i < CIN / 16; i++)
+      for (int n = 0; n < COUT / 16; n++)
+        acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], gv[n], acc[i][n], 0, 0, 0);
+  }
+  // acc[i][n] holds D[ci = i*16 + kq + j][co = n*16 + row]
+  for (int i = 0; i < CIN / 16; i++)
+    for (int n = 0; n < COUT / 16; n++)
+      for (int j = 0; j < 4; j++)
+        atomicAdd(&dW[((size_t)t * CIN + i * 16 + kq + j) * COUT + n * 16 + row], acc[i][n][j]);
+}
+
+// FP32 masters [27][cin][cout] -> the forward layout [27][cin/16][cout][16]
+// and the data-gradient layout (taps flipped, in and out swapped)
+// [27][cout/16][cin][16], both BF16.
+__global__ void k_pack(const float* __restrict__ w, int cin, int cout, uint16_t* fwd, uint16_t* bwd) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 27 * cin * cout) return;
+  int co = e % cout, ci = (e / cout) % cin, t = e / (cin * cout);
+  uint16_t h = f2bf(w[e]);
+  fwd[((size_t)(t * (cin / 16) + ci / 16) * cout + co) * 16 + (ci & 15)] = h;
+  bwd[((size_t)((26 - t) * (cout / 16) + co / 16) * cin + ci) * 16 + (co & 15)] = h;
+}
+
+// ------------------------------------------------------------ elementwise --
+// 2^3 average pooling: [B][S]^3[C] -> [B][S/2]^3[C]
+__global__ void k_pool(const uint16_t* __restrict__ in, int B, int S, int C, uint16_t* out,
+                       const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  int h = S / 2, e = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = B * h * h * h * C;
+  if (e >= n) return;
+  int c = e % C, v = e / C, b = v / (h * h * h), r = v % (h * h * h);
+  int z = r / (h * h), y = (r / h) % h, x = r % h;
+  float s = 0.0f;
+  for (int k = 0; k < 8; k++) {
+    int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+    s += bf2f(in[(((size_t)b * S * S * S) + ((size_t)zz * S + yy) * S + xx) * C + c]);
+  }
+  out[e] = f2bf(s * 0.125f);
+}
+// gradient of the pooling: each fine voxel gets an eighth of its parent's
+__global__ void k_pool_bwd(const float* __restrict__ gout, int B, int S, int C, float* gin, int accumulate) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = B * S * S * S * C;
+  if (e >= n) return;
+  int h = S / 2, c = e % C, v = e / C, b = v / (S * S * S), r = v % (S * S * S);
+  int z = r / (S * S), y = (r / S) % S, x = r % S;
+  float gv = 0.125f * gout[(((size_t)b * h * h * h) + ((size_t)(z / 2) * h + y / 2) * h + x / 2) * C + c];
+  gin[e] = accumulate ? gin[e] + gv : gv;
+}
+// nearest upsampling of [B][S/2]^3[C] to [B][S]^3[C] plus the skip
+__global__ void k_up_add(const uint16_t* __restrict__ coarse, const uint16_t* __restrict__ skip, int B, int S, int C,
+                         uint16_t* out, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = B * S * S * S * C;
+  if (e >= n) return;
+  int h = S / 2, c = e % C, v = e / C, b = v / (S * S * S), r = v % (S * S * S);
+  int z = r / (S * S), y = (r / S) % S, x = r % S;
+  float u = bf2f(coarse[(((size_t)b * h * h * h) + ((size_t)(z / 2) * h + y / 2) * h + x / 2) * C + c]);
+  out[e] = f2bf(u + bf2f(skip[e]));
+}
+// gradient of upsampling: the coarse voxel sums its eight children
+__global__ void k_up_bwd(const float* __restrict__ gout, int B, int S, int C, float* gcoarse) {
+  int h = S / 2, e = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = B * h * h * h * C;
+  if (e >= n) return;
+  int c = e % C, v = e / C, b = v / (h * h * h), r = v % (h * h * h);
+  int z = r / (h * h), y = (r / h) % h, x = r % h;
+  float s = 0.0f;
+  for (int k = 0; k < 8; k++) {
+    int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+    s += gout[(((size_t)b * S * S * S) + ((size_t)zz * S + yy) * S + xx) * C + c];
+  }
+  gcoarse[e] = s;
+}
+// gradient through the leaky ReLU, to BF16 for the matrix cores
+__global__ void k_act_bwd(const float* __restrict__ gout, const float* __restrict__ pre, int n, int act, uint16_t* g) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float d = gout[e];
+  if (act && pre[e] <= 0.0f) d *= 0.25f;
+  g[e] = f2bf(d);
+}
+// bias gradient: sum over voxels of g (bf16) per channel
+__global__ void k_bias_grad(const uint16_t* __restrict__ g, int M, int C, float* db) {
+  int c = blockIdx.x, lane = threadIdx.x;
+  float s = 0.0f;
+  for (int v = lane; v < M; v += blockDim.x) s += bf2f(g[(size_t)v * C + c]);
+  __shared__ float red[256];
+  red[lane] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (lane < w) red[lane] += red[lane + w];
+    __syncthreads();
+  }
+  if (lane == 0) db[c] += red[0];
+}
+__global__ void k_add_f32(float* a, const float* __restrict__ b, int n) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] += b[e];
+}
+__global__ void k_zero_f32(float* a, size_t n) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = 0.0f;
+}
+
+// ------------------------------------------------------ the mesh interface --
+// Network input from a coarse mesh (64x64 atlas of 4x4 slices of 16^2; xyz
+// mass-weighted world positions): mass in units of the box's total mass
+// times 64, and the block's centre of mass less its centre in block lengths
+// (4 FH); channels 4..15 zero.
+__device__ __forceinline__ int coarse_texel(int x, int y, int z) { return ((z >> 2) * 16 + y) * 64 + (z & 3) * 16 + x; }
+constexpr float BLOCK_LEN = (float)(4.0 * FH);
+__global__ void k_ff_in(const float4* __restrict__ coarse, const float4* __restrict__ mono,
+                        const Box* __restrict__ bxp, int which, int inst, uint16_t* in, float* scale,
+                        const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 4096) return;
+  int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
+  const Box bx = *bxp;
+  const float* L = which ? bx.LB : bx.LA;
+  float4 c = coarse[coarse_texel(x, y, z)];
+  float mt = fmaxf(mono[0].w, 1e-20f);
+  uint16_t* o = in + ((size_t)inst * 4096 + v) * 16;
+  float f[4] = { 0, 0, 0, 0 };
+  if (c.w > 0.0f) {
+    f[0] = c.w / mt * 64.0f;
+    f[1] = (c.x / c.w - ((4 * x + 2) * (float)FH - L[0])) / BLOCK_LEN;
+    f[2] = (c.y / c.w - ((4 * y + 2) * (float)FH - L[1])) / BLOCK_LEN;
+    f[3] = (c.z / c.w - ((4 * z + 2) * (float)FH - L[2])) / BLOCK_LEN;
+  }
+  for (int k = 0; k < 16; k++) o[k] = f2bf(k < 4 ? f[k] : 0.0f);
+  if (v == 0) scale[inst] = mt;
+}
+// Output scales: acceleration, potential and tidal tensor in units of the
+// total mass over the block length squared, to the first and cubed.
+constexpr float FF_A = 1.0f / (BLOCK_LEN * BLOCK_LEN), FF_P = 1.0f / BLOCK_LEN,
+                FF_T = 1.0f / (BLOCK_LEN * BLOCK_LEN * BLOCK_LEN);
+__global__ void k_ff_out(const float* __restrict__ y, const float4* __restrict__ coarse, const float* __restrict__ scale,
+                         int inst, float4* farA, float4* farT0, float4* farT1, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 4096) return;
+  int x = v & 15, yv = (v >> 4) & 15, z = v >> 8;
+  int t = coarse_texel(x, yv, z);
+  if (coarse[t].w <= 0.0f) {
+    farA[t] = make_float4(0, 0, 0, 0); farT0[t] = make_float4(0, 0, 0, 0); farT1[t] = make_float4(0, 0, 0, 0);
+    return;
+  }
+  const float* o = y + ((size_t)inst * 4096 + v) * 16;
+  float m = scale[inst];
+  farA[t] = make_float4(o[0] * m * FF_A, o[1] * m * FF_A, o[2] * m * FF_A, o[3] * m * FF_P);
+  farT0[t] = make_float4(o[4] * m * FF_T, o[5] * m * FF_T, o[6] * m * FF_T, o[7] * m * FF_T);
+  farT1[t] = make_float4(o[8] * m * FF_T, o[9] * m * FF_T, 0.0f, 0.0f);
+}
+// Training target from the exact far field, in the output's units; the mask
+// marks occupied blocks.
+__global__ void k_ff_target(const float4* __restrict__ coarse, const float4* __restrict__ farA,
+                            const float4* __restrict__ farT0, const float4* __restrict__ farT1,
+                            const float4* __restrict__ mono, int inst, float* tgt, float* mask) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 4096) return;
+  int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
+  int t = coarse_texel(x, y, z);
+  float* o = tgt + ((size_t)inst * 4096 + v) * 16;
+  float m = fmaxf(mono[0].w, 1e-20f);
+  bool occ = coarse[t].w > 0.0f;
+  float4 A = farA[t], T0 = farT0[t], T1 = farT1[t];
+  float vals[10] = { A.x / (m * FF_A), A.y / (m * FF_A), A.z / (m * FF_A), A.w / (m * FF_P), T0.x / (m * FF_T),
+                     T0.y / (m * FF_T), T0.z / (m * FF_T), T0.w / (m * FF_T), T1.x / (m * FF_T), T1.y / (m * FF_T) };
+  for (int k = 0; k < 16; k++) o[k] = occ && k < 10 ? vals[k] : 0.0f;
+  mask[(size_t)inst * 4096 + v] = occ ? 1.0f : 0.0f;
+}
+// Masked mean squared error over the 10 used channels and its gradient;
+// loss accumulated in double.
+__global__ void k_ff_loss(const float* __restrict__ y, const float* __restrict__ tgt, const float* __restrict__ mask,
+                          int M, float invCount, float* gy, double* loss) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (e < M * 16) {
+    int v = e >> 4, c = e & 15;
+    float d = (c < 10 && mask[v] > 0.0f) ? y[e] - tgt[e] : 0.0f;
+    gy[e] = 2.0f * d * invCount;
+    l = d * d * invCount;
+  }
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss, (double)l);
+}
+// Adam on the FP32 masters
+__global__ void k_adam(float* w, const float* __restrict__ g, float* m, float* v, int n, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float gg = g[e];
+  float mm = b1 * m[e] + (1.0f - b1) * gg, vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+  m[e] = mm; v[e] = vv;
+  w[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+// The judge: |F_learned - F_exact| / |F_exact| over the force mesh's
+// accelerations, accumulated in double.
+__global__ void k_judge_sum(const float4* __restrict__ a, const float4* __restrict__ b, int n, double* sums) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  float d2 = 0.0f, r2 = 0.0f;
+  if (e < n) {
+    float4 x = a[e], y = b[e];
+    float dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
+    d2 = dx * dx + dy * dy + dz * dz;
+    r2 = y.x * y.x + y.y * y.y + y.z * y.z;
+  }
+  for (int o = 32; o > 0; o >>= 1) { d2 += __shfl_xor(d2, o); r2 += __shfl_xor(r2, o); }
+  if ((threadIdx.x & 63) == 0) { atomicAdd(&sums[0], (double)d2); atomicAdd(&sums[1], (double)r2); }
+}
+// stats: [0] last error, [1] judged, [2] fallbacks, [3] learned mesh steps;
+// flag: 1 -> the next mesh steps use the network
+__global__ void k_judge_decide(double* sums, float tol, float* stats, int* flag) {
+  if (threadIdx.x != 0) return;
+  double err = sums[1] > 0.0 ? sqrt(sums[0] / sums[1]) : 0.0;
+  stats[0] = (float)err;
+  stats[1] += 1.0f;
+  int ok = err <= tol;
+  if (!ok) stats[2] += 1.0f;
+  *flag = ok;
+  sums[0] = 0.0; sums[1] = 0.0;
+}
+__global__ void k_count_learned(const int* flag, float* stats) {
+  if (threadIdx.x == 0 && *flag) stats[3] += 1.0f;
+}
+__global__ void k_zero_d(double* p, int n) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = 0.0;
+}
+
+
+// relative error sums of a forward pass: [0] |dA|^2 [1] |A|^2 over the
+// acceleration channels, [2] |dY|^2 [3] |Y|^2 over all ten
+__global__ void k_eval_sum(const float* __restrict__ y, const float* __restrict__ tgt, const float* __restrict__ mask,
+                           int M, double* sums) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (e < M * 16) {
+    int v = e >> 4, c = e & 15;
+    if (c < 10 && mask[v] > 0.0f) {
+      float d = y[e] - tgt[e], t = tgt[e];
+      s2 = d * d; s3 = t * t;
+      if (c < 3) { s0 = s2; s1 = s3; }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_xor(s0, o); s1 += __shfl_xor(s1, o); s2 += __shfl_xor(s2, o); s3 += __shfl_xor(s3, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&sums[0], (double)s0); atomicAdd(&sums[1], (double)s1);
+    atomicAdd(&sums[2], (double)s2); atomicAdd(&sums[3], (double)s3);
+  }
+}
+
+// ------------------------------------------------------------------- host --
+struct Layer {
+  int cin, cout, act;
+  size_t w, b;          // offsets of the layer's weights and bias in the master array
+  uint16_t *fwd, *bwd;  // packed BF16 weights
+};
+// the network's twelve convolutions (see the top of this file)
+static const int LAYER_IO[12][3] = { { 16, 32, 1 }, { 32, 32, 1 }, { 32, 32, 1 }, { 32, 32, 1 }, { 32, 32, 1 },
+                                     { 32, 32, 1 }, { 32, 32, 1 }, { 32, 32, 1 }, { 32, 32, 1 }, { 32, 32, 1 },
+                                     { 32, 32, 1 }, { 32, 16, 0 } };
+static const int LAYER_S[12] = { 16, 16, 8, 8, 4, 4, 2, 2, 4, 8, 16, 16 };
+
+size_t ff_weight_count() {
+  size_t n = 0;
+  for (auto& l : LAYER_IO) n += (size_t)27 * l[0] * l[1] + l[1];
+  return n;
+}
+
+// Forward activations for B instances (outputs, and pre-activations when
+// training) and the gradient buffers.
+struct Acts {
+  int B = 0;
+  uint16_t *x0, *a0, *e0, *p1, *a2, *e1, *p2, *a4, *e2, *p3, *a6, *bot, *u2, *d2, *u1, *d1, *u0, *d0;
+  float* y;
+  float* pre[12];
+  float* scale;
+  // gradients (training only)
+  float *gy, *gA16, *gB16, *gE16, *gA8, *gB8, *gE8, *gA4, *gB4, *gE4, *gA2, *gB2;
+  uint16_t* gbf;
+  std::vector<void*> allocs;
+};
+
+struct FFNet {
+  Layer L[12];
+  size_t nw = 0;
+  float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  int adamT = 0;
+  int cap = 0, count = 0;
+  uint16_t* dataIn = nullptr;   // [cap][4096][16]
+  float* dataTgt = nullptr;     // [cap][4096][16]
+  float* dataMask = nullptr;    // [cap][4096]
+  float* dataScale = nullptr;   // [cap] total mass of each sample's box
+  Acts train;                   // batch activations for training
+  int* idx = nullptr;
+  int idxCap = 0;
+  double* loss = nullptr;
+  int lossCap = 0;
+  hipStream_t s = nullptr;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+struct FFWork {
+  Acts a;
+};
+
+static bool dev_alloc(std::vector<void*>& list, void** p, size_t bytes, std::string& err) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) { err = std::string("hipMalloc: ") + hipGetErrorString(e); return false; }
+  list.push_back(*p);
+  return true;
+}
+#define DA(list, ptr, bytes) if (!dev_alloc(list, (void**)&(ptr), (bytes), err)) return false
+
+static bool acts_alloc(Acts& a, int B, bool training, std::string& err) {
+  a.B = B;
+  size_t v16 = (size_t)B * 4096, v8 = (size_t)B * 512, v4 = (size_t)B * 64, v2 = (size_t)B * 8;
+  DA(a.allocs, a.x0, v16 * 16 * 2);
+  DA(a.allocs, a.a0, v16 * 64); DA(a.allocs, a.e0, v16 * 64); DA(a.allocs, a.p1, v8 * 64);
+  DA(a.allocs, a.a2, v8 * 64); DA(a.allocs, a.e1, v8 * 64); DA(a.allocs, a.p2, v4 * 64);
+  DA(a.allocs, a.a4, v4 * 64); DA(a.allocs, a.e2, v4 * 64); DA(a.allocs, a.p3, v2 * 64);
+  DA(a.allocs, a.a6, v2 * 64); DA(a.allocs, a.bot, v2 * 64); DA(a.allocs, a.u2, v4 * 64);
+  DA(a.allocs, a.d2, v4 * 64); DA(a.allocs, a.u1, v8 * 64); DA(a.allocs, a.d1, v8 * 64);
+  DA(a.allocs, a.u0, v16 * 64); DA(a.allocs, a.d0, v16 * 64);
+  DA(a.allocs, a.y, v16 * 16 * 4);
+  DA(a.allocs, a.scale, (size_t)B * 4);
+  for (int l = 0; l < 12; l++) a.pre[l] = nullptr;
+  if (!training) return true;
+  for (int l = 0; l < 12; l++) {
+    size_t vox = (size_t)B * LAYER_S[l] * LAYER_S[l] * LAYER_S[l];
+    DA(a.allocs, a.pre[l], vox * LAYER_IO[l][1] * 4);
+  }
+  DA(a.allocs, a.gy, v16 * 16 * 4);
+  DA(a.allocs, a.gA16, v16 * 32 * 4); DA(a.allocs, a.gB16, v16 * 32 * 4); DA(a.allocs, a.gE16, v16 * 32 * 4);
+  DA(a.allocs, a.gA8, v8 * 32 * 4); DA(a.allocs, a.gB8, v8 * 32 * 4); DA(a.allocs, a.gE8, v8 * 32 * 4);
+  DA(a.allocs, a.gA4, v4 * 32 * 4); DA(a.allocs, a.gB4, v4 * 32 * 4); DA(a.allocs, a.gE4, v4 * 32 * 4);
+  DA(a.allocs, a.gA2, v2 * 32 * 4); DA(a.allocs, a.gB2, v2 * 32 * 4);
+  DA(a.allocs, a.gbf, v16 * 32 * 2);
+  return true;
+}
+static void acts_free(Acts& a) {
+  for (void* p : a.allocs) hipFree(p);
+  a.allocs.clear();
+}
+
+static inline unsigned grid_of(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+static void conv(const Layer& L, const float* bias, const uint16_t* in, int B, int S, uint16_t* out, float* outf,
+                 float* pre, const int* gate, hipStream_t s) {
+  size_t M = (size_t)B * S * S * S;
+  unsigned blocks = grid_of((M + 15) / 16 * 64);
+  if (L.cin == 16)
+    hipLaunchKernelGGL((k_conv<16, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, L.act, out, outf, pre,
+                       gate);
+  else if (L.cout == 32)
+    hipLaunchKernelGGL((k_conv<32, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, L.act, out, outf, pre,
+                       gate);
+  else
+    hipLaunchKernelGGL((k_conv<32, 16>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, L.act, out, outf, pre,
+                       gate);
+}
+// the data gradient of a layer: g (bf16, at its output) through the flipped,
+// transposed weights to f32 at its input
+static void conv_dgrad(const Layer& L, const uint16_t* g, int B, int S, float* gin, hipStream_t s) {
+  size_t M = (size_t)B * S * S * S;
+  unsigned blocks = grid_of((M + 15) / 16 * 64);
+  if (L.cout == 16)   // the output layer: 16 gradient channels in, 32 out
+    hipLaunchKernelGGL((k_conv<16, 32>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 0, nullptr, gin,
+                       nullptr, nullptr);
+  else if (L.cin == 32)
+    hipLaunchKernelGGL((k_conv<32, 32>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 0, nullptr, gin,
+                       nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((k_conv<32, 16>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 0, nullptr, gin,
+                       nullptr, nullptr);
+}
+static void conv_wgrad(const Layer& L, const uint16_t* in, const uint16_t* g, int B, int S, float* dW, float* db,
+                       hipStream_t s) {
+  size_t M = (size_t)B * S * S * S;
+  unsigned waves = (unsigned)(27 * ((M + 255) / 256));
+  unsigned blocks = grid_of((size_t)waves * 64);
+  if (L.cin == 16)
+    hipLaunchKernelGGL((k_conv_wgrad<16, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, g, dW);
+  else if (L.cout == 32)
+    hipLaunchKernelGGL((k_conv_wgrad<32, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, g, dW);
+  else
+    hipLaunchKernelGGL((k_conv_wgrad<32, 16>), dim3(blocks), dim3(256), 0, s, in, B, S, g, dW);
+  hipLaunchKernelGGL(k_bias_grad, dim3(L.cout), dim3(256), 0, s, g, (int)M, L.cout, db);
+}
+
+static void forward(const FFNet* n, Acts& a, int B, bool training, const int* gate, hipStream_t s) {
+  const float* w = n->w;
+  auto pre = [&](int l) { return training ? a.pre[l] : nullptr; };
+  auto C = [&](int l, const uint16_t* in, uint16_t* out) {
+    conv(n->L[l], w + n->L[l].b, in, B, LAYER_S[l], out, nullptr, pre(l), gate, s);
+  };
+  auto pool = [&](const uint16_t* in, int S, uint16_t* out) {
+    hipLaunchKernelGGL(k_pool, dim3(grid_of((size_t)B * (S / 2) * (S / 2) * (S / 2) * 32)), dim3(256), 0, s, in, B, S,
+                       32, out, gate);
+  };
+  auto up = [&](const uint16_t* coarse, const uint16_t* skip, int S, uint16_t* out) {
+    hipLaunchKernelGGL(k_up_add, dim3(grid_of((size_t)B * S * S * S * 32)), dim3(256), 0, s, coarse, skip, B, S, 32,
+                       out, gate);
+  };
+  C(0, a.x0, a.a0); C(1, a.a0, a.e0);
+  pool(a.e0, 16, a.p1); C(2, a.p1, a.a2); C(3, a.a2, a.e1);
+  pool(a.e1, 8, a.p2); C(4, a.p2, a.a4); C(5, a.a4, a.e2);
+  pool(a.e2, 4, a.p3); C(6, a.p3, a.a6); C(7, a.a6, a.bot);
+  up(a.bot, a.e2, 4, a.u2); C(8, a.u2, a.d2);
+  up(a.d2, a.e1, 8, a.u1); C(9, a.u1, a.d1);
+  up(a.d1, a.e0, 16, a.u0); C(10, a.u0, a.d0);
+  conv(n->L[11], w + n->L[11].b, a.d0, B, 16, nullptr, a.y, pre(11), gate, s);
+}
+
+// backward from a.gy (f32, at the output) to the weight gradients n->g
+static void backward(FFNet* n, Acts& a, int B, hipStream_t s) {
+  float* g = n->g;
+  auto actb = [&](const float* gout, int l) {   // gradient at layer l's output through its activation, to bf16
+    size_t cnt = (size_t)B * LAYER_S[l] * LAYER_S[l] * LAYER_S[l] * n->L[l].cout;
+    hipLaunchKernelGGL(k_act_bwd, dim3(grid_of(cnt)), dim3(256), 0, s, gout, a.pre[l], (int)cnt, n->L[l].act, a.gbf);
+  };
+  auto W = [&](int l, const uint16_t* in) {
+    conv_wgrad(n->L[l], in, a.gbf, B, LAYER_S[l], g + n->L[l].w, g + n->L[l].b, s);
+  };
+  auto D = [&](int l, float* gin) { conv_dgrad(n->L[l], a.gbf, B, LAYER_S[l], gin, s); };
+  auto upb = [&](const float* gout, int S, float* gc) {
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid_of((size_t)B * (S / 2) * (S / 2) * (S / 2) * 32)), dim3(256), 0, s, gout, B,
+                       S, 32, gc);
+  };
+  auto poolb = [&](const float* gout, int S, float* gin) {   // accumulate into gin
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid_of((size_t)B * S * S * S * 32)), dim3(256), 0, s, gout, B, S, 32, gin, 1);
+  };
+  auto copy = [&](float* dst, const float* src, size_t cnt) {
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_of(cnt)), dim3(256), 0, s, dst, cnt);
+    hipLaunchKernelGGL(k_add_f32, dim3(grid_of(cnt)), dim3(256), 0, s, dst, src, (int)cnt);
+  };
+  size_t v16 = (size_t)B * 4096 * 32, v8 = (size_t)B * 512 * 32, v4 = (size_t)B * 64 * 32;
+  actb(a.gy, 11); W(11, a.d0); D(11, a.gA16);                 // gA16: d0
+  actb(a.gA16, 10); W(10, a.u0); D(10, a.gB16);               // gB16: u0 = up(d1) + e0
+  copy(a.gE16, a.gB16, v16);                                  // gE16: e0 (skip)
+  upb(a.gB16, 16, a.gA8);                                     // gA8: d1
+  actb(a.gA8, 9); W(9, a.u1); D(9, a.gB8);                    // gB8: u1 = up(d2) + e1
+  copy(a.gE8, a.gB8, v8);
+  upb(a.gB8, 8, a.gA4);                                       // gA4: d2
+  actb(a.gA4, 8); W(8, a.u2); D(8, a.gB4);                    // gB4: u2 = up(bot) + e2
+  copy(a.gE4, a.gB4, v4);
+  upb(a.gB4, 4, a.gA2);                                       // gA2: bot
+  actb(a.gA2, 7); W(7, a.a6); D(7, a.gB2);                    // gB2: a6
+  actb(a.gB2, 6); W(6, a.p3); D(6, a.gA2);                    // gA2: p3
+  poolb(a.gA2, 4, a.gE4);                                     // e2 += pool grad
+  actb(a.gE4, 5); W(5, a.a4); D(5, a.gB4);                    // gB4: a4
+  actb(a.gB4, 4); W(4, a.p2); D(4, a.gA4);                    // gA4: p2
+  poolb(a.gA4, 8, a.gE8);                                     // e1
+  actb(a.gE8, 3); W(3, a.a2); D(3, a.gB8);                    // a2
+  actb(a.gB8, 2); W(2, a.p1); D(2, a.gA8);                    // p1
+  poolb(a.gA8, 16, a.gE16);                                   // e0
+  actb(a.gE16, 1); W(1, a.a0); D(1, a.gB16);                  // a0
+  actb(a.gB16, 0); W(0, a.x0);
+}
+
+static void repack(FFNet* n, hipStream_t s) {
+  for (int l = 0; l < 12; l++) {
+    const Layer& L = n->L[l];
+    size_t cnt = (size_t)27 * L.cin * L.cout;
+    hipLaunchKernelGGL(k_pack, dim3(grid_of(cnt)), dim3(256), 0, s, n->w + L.w, L.cin, L.cout, L.fwd, L.bwd);
+  }
+}
+
+static bool ff_init(FFNet* n, int cap, std::string& err) {
+  size_t off = 0;
+  for (int l = 0; l < 12; l++) {
+    Layer& L = n->L[l];
+    L.cin = LAYER_IO[l][0]; L.cout = LAYER_IO[l][1]; L.act = LAYER_IO[l][2];
+    L.w = off; off += (size_t)27 * L.cin * L.cout;
+    L.b = off; off += L.cout;
+    DA(n->allocs, L.fwd, (size_t)27 * L.cin * L.cout * 2);
+    DA(n->allocs, L.bwd, (size_t)27 * L.cin * L.cout * 2);
+  }
+  n->nw = off;
+  DA(n->allocs, n->w, off * 4); DA(n->allocs, n->g, off * 4); DA(n->allocs, n->m, off * 4); DA(n->allocs, n->v, off * 4);
+  n->cap = cap;
+  if (cap > 0) {
+    DA(n->allocs, n->dataIn, (size_t)cap * 4096 * 16 * 2);
+    DA(n->allocs, n->dataTgt, (size_t)cap * 4096 * 16 * 4);
+    DA(n->allocs, n->dataMask, (size_t)cap * 4096 * 4);
+    DA(n->allocs, n->dataScale, (size_t)cap * 4);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking);
+  if (e != hipSuccess) { err = hipGetErrorString(e); return false; }
+  // He-style initialisation, seeded: uniform with variance 2 / fan-in (1 / fan-in for the linear output)
+  std::vector<float> w(off, 0.0f);
+  uint32_t st = 0x5EEDF00Du;
+  auto rnd = [&]() { st ^= st << 13; st ^= st >> 17; st ^= st << 5; return (st >> 8) * (1.0f / 16777216.0f); };
+  for (int l = 0; l < 12; l++) {
+    const Layer& L = n->L[l];
+    float fan = 27.0f * (l == 0 ? 4.0f : (float)L.cin);
+    float a = sqrtf(3.0f * (L.act ? 2.0f : 1.0f) / fan);
+    for (int t = 0; t < 27; t++)
+      for (int ci = 0; ci < L.cin; ci++)
+        for (int co = 0; co < L.cout; co++)
+          w[L.w + ((size_t)t * L.cin + ci) * L.cout + co] = (l == 0 && ci >= 4) ? 0.0f : a * (2.0f * rnd() - 1.0f);
+  }
+  e = hipMemcpyAsync(n->w, w.data(), off * 4, hipMemcpyHostToDevice, n->s);
+  if (e == hipSuccess) e = hipMemsetAsync(n->m, 0, off * 4, n->s);
+  if (e == hipSuccess) e = hipMemsetAsync(n->v, 0, off * 4, n->s);
+  repack(n, n->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+  if (e != hipSuccess) { err = hipGetErrorString(e); return false; }
+  return true;
+}
+
+FFNet* ff_create(int cap, std::string& err) {
+  FFNet* n = new FFNet();
+  if (!ff_init(n, cap, err)) { ff_destroy(n); return nullptr; }
+  return n;
+}
+void ff_destroy(FFNet* n) {
+  if (!n) return;
+  if (n->s) hipStreamSynchronize(n->s);
+  acts_free(n->train);
+  for (void* p : n->allocs) hipFree(p);
+  if (n->s) hipStreamDestroy(n->s);
+  delete n;
+}
+FFWork* ffwork_create(std::string& err) {
+  FFWork* w = new FFWork();
+  if (!acts_alloc(w->a, 1, false, err)) { ffwork_destroy(w); return nullptr; }
+  return w;
+}
+void ffwork_destroy(FFWork* w) {
+  if (!w) return;
+  acts_free(w->a);
+  delete w;
+}
+
+// inference for one box of one instance into farA/T0/T1; every kernel does
+// nothing unless *gate (when given) is nonzero
+void ff_infer(const FFNet* n, FFWork* wk, const float4* coarse, const float4* mono, const Box* bx, int which,
+              float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
+  Acts& a = wk->a;
+  hipLaunchKernelGGL(k_ff_in, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, 0, a.x0, a.scale, gate);
+  forward(n, a, 1, false, gate, s);
+  hipLaunchKernelGGL(k_ff_out, dim3(16), dim3(256), 0, s, a.y, coarse, a.scale, 0, farA, farT0, farT1, gate);
+}
+
+// one training sample from a box's coarse mesh and its exact far field
+int ff_record(FFNet* n, const float4* coarse, const float4* mono, const Box* bx, int which, const float4* farA,
+              const float4* farT0, const float4* farT1, hipStream_t s) {
+  if (n->count >= n->cap) return -1;
+  int k = n->count++;
+  hipLaunchKernelGGL(k_ff_in, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, k, n->dataIn, n->dataScale, nullptr);
+  hipLaunchKernelGGL(k_ff_target, dim3(16), dim3(256), 0, s, coarse, farA, farT0, farT1, mono, k, n->dataTgt,
+                     n->dataMask);
+  return k;
+}
+
+__global__ void k_gather(const uint16_t* __restrict__ dataIn, const float* __restrict__ dataTgt,
+                         const float* __restrict__ dataMask, const int* __restrict__ idx, int B, uint16_t* x0,
+                         float* tgt, float* mask) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)B * 4096 * 16) return;
+  int b = (int)(e / (4096 * 16));
+  size_t off = e - (size_t)b * 4096 * 16, src = (size_t)idx[b] * 4096 * 16 + off;
+  x0[e] = dataIn[src];
+  tgt[e] = dataTgt[src];
+  if (off % 16 == 0) mask[(size_t)b * 4096 + off / 16] = dataMask[(size_t)idx[b] * 4096 + off / 16];
+}
+
+// iters steps of Adam on batches of B samples drawn (seeded) from the record;
+// losses[iters] the mean squared error of each step's batch
+int ff_train(FFNet* n, int iters, int B, float lr, uint32_t seed, double* losses, std::string& err) {
+  if (n->count <= 0) { err = "no training samples recorded"; return -1; }
+  if (n->train.B != B) {
+    acts_free(n->train);
+    if (!acts_alloc(n->train, B, true, err)) return -2;
+  }
+  Acts& a = n->train;
+  float *tgt = nullptr, *mask = nullptr;
+  std::vector<void*> tmp;
+  if (!dev_alloc(tmp, (void**)&tgt, (size_t)B * 4096 * 16 * 4, err) ||
+      !dev_alloc(tmp, (void**)&mask, (size_t)B * 4096 * 4, err)) {
+    for (void* p : tmp) hipFree(p);
+    return -2;
+  }
+  if (n->idxCap < iters * B) {
+    if (n->idx) hipFree(n->idx);
+    if (hipMalloc((void**)&n->idx, (size_t)iters * B * 4) != hipSuccess) { err = "hipMalloc idx"; return -2; }
+    n->idxCap = iters * B;
+  }
+  if (n->lossCap < iters) {
+    if (n->loss) hipFree(n->loss);
+    if (hipMalloc((void**)&n->loss, (size_t)iters * 8) != hipSuccess) { err = "hipMalloc loss"; return -2; }
+    n->lossCap = iters;
+  }
+  std::vector<int> idx((size_t)iters * B);
+  uint32_t st = seed ? seed : 1u;
+  for (auto& i : idx) {
+    st ^= st << 13; st ^= st >> 17; st ^= st << 5;
+    i = (int)(st % (uint32_t)n->count);
+  }
+  hipStream_t s = n->s;
+  hipMemcpyAsync(n->idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(k_zero_d, dim3(grid_of(iters)), dim3(256), 0, s, n->loss, iters);
+  const float b1 = 0.9f, b2 = 0.999f;
+  for (int it = 0; it < iters; it++) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_of((size_t)B * 4096 * 16)), dim3(256), 0, s, n->dataIn, n->dataTgt,
+                       n->dataMask, n->idx + (size_t)it * B, B, a.x0, tgt, mask);
+    forward(n, a, B, true, nullptr, s);
+    // the loss over the used channels of occupied blocks, per sample mean
+    hipLaunchKernelGGL(k_ff_loss, dim3(grid_of((size_t)B * 4096 * 16)), dim3(256), 0, s, a.y, tgt, mask, B * 4096,
+                       1.0f / (B * 4096.0f), a.gy, n->loss + it);
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_of(n->nw)), dim3(256), 0, s, n->g, n->nw);
+    backward(n, a, B, s);
+    n->adamT++;
+    float c1 = 1.0f - powf(b1, (float)n->adamT), c2 = 1.0f - powf(b2, (float)n->adamT);
+    hipLaunchKernelGGL(k_adam, dim3(grid_of(n->nw)), dim3(256), 0, s, n->w, n->g, n->m, n->v, (int)n->nw, lr, b1, b2,
+                       c1, c2, 1e-8f);
+    repack(n, s);
+  }
+  hipError_t e = hipMemcpyAsync(losses, n->loss, (size_t)iters * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (void* p : tmp) hipFree(p);
+  if (e != hipSuccess) { err = hipGetErrorString(e); return -3; }
+  return 0;
+}
+
+
+void launch_judge(const float4* force, const float4* ref, double* sums, float tol, float* stats, int* gate,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_judge_sum, dim3(grid_of(FINE_TEXELS)), dim3(256), 0, s, force, ref, FINE_TEXELS, sums);
+  hipLaunchKernelGGL(k_judge_decide, dim3(1), dim3(64), 0, s, sums, tol, stats, gate);
+}
+void launch_count_learned(const int* gate, float* stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_learned, dim3(1), dim3(64), 0, s, gate, stats);
+}
+
+// forward over recorded samples [first, first+count) in batches of up to 16;
+// optional outputs (count*4096*16 f32) and the error sums of k_eval_sum
+static int ff_forward_samples(FFNet* n, int first, int count, float* out, double sums[4], std::string& err) {
+  if (first < 0 || count <= 0 || first + count > n->count) { err = "sample range outside the record"; return -1; }
+  const int B = count < 16 ? count : 16;
+  if (n->train.B < B) {
+    acts_free(n->train);
+    n->train.B = 0;
+    if (!acts_alloc(n->train, B, true, err)) return -2;
+  }
+  Acts& a = n->train;
+  hipStream_t s = n->s;
+  double* ds = nullptr;
+  int* idx = nullptr;
+  float *tgt = nullptr, *mask = nullptr;
+  std::vector<void*> tmp;
+  if (!dev_alloc(tmp, (void**)&ds, 32, err) || !dev_alloc(tmp, (void**)&idx, (size_t)count * 4, err) ||
+      !dev_alloc(tmp, (void**)&tgt, (size_t)B * 4096 * 16 * 4, err) || !dev_alloc(tmp, (void**)&mask, (size_t)B * 4096 * 4, err)) {
+    for (void* p : tmp) hipFree(p);
+    return -2;
+  }
+  std::vector<int> ix(count);
+  for (int k = 0; k < count; k++) ix[k] = first + k;
+  hipMemcpyAsync(idx, ix.data(), (size_t)count * 4, hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, ds, 4);
+  for (int k = 0; k < count; k += B) {
+    int b = count - k < B ? count - k : B;
+    hipLaunchKernelGGL(k_gather, dim3(grid_of((size_t)b * 4096 * 16)), dim3(256), 0, s, n->dataIn, n->dataTgt,
+                       n->dataMask, idx + k, b, a.x0, tgt, mask);
+    forward(n, a, b, false, nullptr, s);
+    hipLaunchKernelGGL(k_eval_sum, dim3(grid_of((size_t)b * 4096 * 16)), dim3(256), 0, s, a.y, tgt, mask, b * 4096, ds);
+    if (out) hipMemcpyAsync(out + (size_t)k * 4096 * 16, a.y, (size_t)b * 4096 * 16 * 4, hipMemcpyDeviceToHost, s);
+  }
+  hipError_t e = hipMemcpyAsync(sums, ds, 32, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (void* p : tmp) hipFree(p);
+  if (e != hipSuccess) { err = hipGetErrorString(e); return -3; }
+  return 0;
+}
+
+}  // namespace cc
+
+using namespace cc;
+
+struct cc_ff {
+  FFNet* n = nullptr;
+  std::string err;
+};
+static std::string g_fferr;
+
+#define FFFAIL(f, code, msg)                \
+  do {                                      \
+    if (f) (f)->err = (msg); else g_fferr = (msg); \
+    return code;                            \
+  } while (0)
+
+FFNet* ff_net(cc_ff* f) { return f ? f->n : nullptr; }
+
+extern "C" {
+
+const char* cc_ff_last_error(cc_ff* f) { return f ? f->err.c_str() : g_fferr.c_str(); }
+
+int cc_ff_create(int capacity, cc_ff** out) {
+  if (!out || capacity < 0) return CC_EINVAL;
+  *out = nullptr;
+  std::string err;
+  FFNet* n = ff_create(capacity, err);
+  if (!n) FFFAIL((cc_ff*)nullptr, CC_EDEVICE, "cc_ff_create: " + err);
+  cc_ff* f = new cc_ff();
+  f->n = n;
+  *out = f;
+  return 0;
+}
+
+int cc_ff_destroy(cc_ff* f) {
+  if (!f) return 0;
+  ff_destroy(f->n);
+  delete f;
+  return 0;
+}
+
+int cc_ff_weight_count(int64_t* n) {
+  if (!n) return CC_EINVAL;
+  *n = (int64_t)ff_weight_count();
+  return 0;
+}
+
+int cc_ff_get_weights(cc_ff* f, float* w, int64_t n) {
+  if (!f || !w || n != (int64_t)f->n->nw) FFFAIL(f, CC_EINVAL, "weight count mismatch");
+  hipError_t e = hipMemcpyAsync(w, f->n->w, (size_t)n * 4, hipMemcpyDeviceToHost, f->n->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->n->s);
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
+  return 0;
+}
+
+int cc_ff_set_weights(cc_ff* f, const float* w, int64_t n) {
+  if (!f || !w || n != (int64_t)f->n->nw) FFFAIL(f, CC_EINVAL, "weight count mismatch");
+  hipError_t e = hipMemcpyAsync(f->n->w, w, (size_t)n * 4, hipMemcpyHostToDevice, f->n->s);
+  if (e == hipSuccess) e = hipMemsetAsync(f->n->m, 0, (size_t)n * 4, f->n->s);
+  if (e == hipSuccess) e = hipMemsetAsync(f->n->v, 0, (size_t)n * 4, f->n->s);
+  repack(f->n, f->n->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->n->s);
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
+  f->n->adamT = 0;
+  return 0;
+}
+
+int cc_ff_samples(cc_ff* f, int* count) {
+  if (!f || !count) return CC_EINVAL;
+  *count = f->n->count;
+  return 0;
+}
+
+int cc_ff_clear(cc_ff* f) {
+  if (!f) return CC_EINVAL;
+  f->n->count = 0;
+  return 0;
+}
+
+int cc_ff_push_sample(cc_ff* f, const float* in, const float* target, const float* mask) {
+  if (!f || !in || !target || !mask) return CC_EINVAL;
+  FFNet* n = f->n;
+  if (n->count >= n->cap) FFFAIL(f, CC_EINVAL, "the sample record is full");
+  std::vector<uint16_t> h((size_t)4096 * 16);
+  for (size_t k = 0; k < h.size(); k++) {   // round to nearest even, as the device does
+    uint32_t u;
+    std::memcpy(&u, &in[k], 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    h[k] = (uint16_t)(u >> 16);
+  }
+  int k = n->count;
+  hipError_t e = hipMemcpyAsync(n->dataIn + (size_t)k * 4096 * 16, h.data(), h.size() * 2, hipMemcpyHostToDevice, n->s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(n->dataTgt + (size_t)k * 4096 * 16, target, (size_t)4096 * 16 * 4, hipMemcpyHostToDevice, n->s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(n->dataMask + (size_t)k * 4096, mask, (size_t)4096 * 4, hipMemcpyHostToDevice, n->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
+  n->count++;
+  return 0;
+}
+
+int cc_ff_read_sample(cc_ff* f, int k, float* in, float* target, float* mask) {
+  if (!f || k < 0 || k >= f->n->count) return CC_EINVAL;
+  FFNet* n = f->n;
+  std::vector<uint16_t> h((size_t)4096 * 16);
+  hipError_t e = hipMemcpyAsync(h.data(), n->dataIn + (size_t)k * 4096 * 16, h.size() * 2, hipMemcpyDeviceToHost, n->s);
+  if (e == hipSuccess && target)
+    e = hipMemcpyAsync(target, n->dataTgt + (size_t)k * 4096 * 16, (size_t)4096 * 16 * 4, hipMemcpyDeviceToHost, n->s);
+  if (e == hipSuccess && mask)
+    e = hipMemcpyAsync(mask, n->dataMask + (size_t)k * 4096, (size_t)4096 * 4, hipMemcpyDeviceToHost, n->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
+  if (in)
+    for (size_t q = 0; q < h.size(); q++) {
+      uint32_t u = (uint32_t)h[q] << 16;
+      std::memcpy(&in[q], &u, 4);
+    }
+  return 0;
+}
+
+int cc_ff_train(cc_ff* f, int iters, int batch, double lr, uint32_t seed, double* losses) {
+  if (!f || iters <= 0 || batch <= 0 || batch > 64 || !losses) FFFAIL(f, CC_EINVAL, "bad training arguments");
+  std::string err;
+  int r = ff_train(f->n, iters, batch, (float)lr, seed, losses, err);
+  if (r) FFFAIL(f, r == -1 ? CC_ESTATE : CC_EDEVICE, "cc_ff_train: " + err);
+  return 0;
+}
+
+int cc_ff_eval(cc_ff* f, int first, int count, float* out, double* relAcc, double* relAll) {
+  if (!f) return CC_EINVAL;
+  std::string err;
+  double sums[4] = { 0, 0, 0, 0 };
+  int r = ff_forward_samples(f->n, first, count, out, sums, err);
+  if (r) FFFAIL(f, r == -1 ? CC_EINVAL : CC_EDEVICE, "cc_ff_eval: " + err);
+  if (relAcc) *relAcc = sums[1] > 0 ? sqrt(sums[0] / sums[1]) : 0.0;
+  if (relAll) *relAll = sums[3] > 0 ? sqrt(sums[2] / sums[3]) : 0.0;
+  return 0;
+}
+
+// the loss and the full weight gradient on samples [first, first+count) at
+// the present weights (no update)
+int cc_ff_gradient(cc_ff* f, int first, int count, float* grad, double* loss) {
+  if (!f || !grad || !loss || first < 0 || count <= 0 || count > 64 || first + count > f->n->count)
+    FFFAIL(f, CC_EINVAL, "bad gradient arguments");
+  FFNet* n = f->n;
+  std::string err;
+  if (n->train.B != count) {
+    acts_free(n->train);
+    n->train.B = 0;
+    if (!acts_alloc(n->train, count, true, err)) FFFAIL(f, CC_EDEVICE, err);
+  }
+  Acts& a = n->train;
+  hipStream_t s = n->s;
+  std::vector<void*> tmp;
+  int* idx = nullptr;
+  float *tgt = nullptr, *mask = nullptr;
+  double* l = nullptr;
+  if (!dev_alloc(tmp, (void**)&idx, (size_t)count * 4, err) || !dev_alloc(tmp, (void**)&tgt, (size_t)count * 4096 * 64, err) ||
+      !dev_alloc(tmp, (void**)&mask, (size_t)count * 4096 * 4, err) || !dev_alloc(tmp, (void**)&l, 8, err)) {
+    for (void* p : tmp) hipFree(p);
+    FFFAIL(f, CC_EDEVICE, err);
+  }
+  std::vector<int> ix(count);
+  for (int k = 0; k < count; k++) ix[k] = first + k;
+  hipMemcpyAsync(idx, ix.data(), (size_t)count * 4, hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, l, 1);
+  hipLaunchKernelGGL(k_gather, dim3(grid_of((size_t)count * 4096 * 16)), dim3(256), 0, s, n->dataIn, n->dataTgt,
+                     n->dataMask, idx, count, a.x0, tgt, mask);
+  forward(n, a, count, true, nullptr, s);
+  hipLaunchKernelGGL(k_ff_loss, dim3(grid_of((size_t)count * 4096 * 16)), dim3(256), 0, s, a.y, tgt, mask,
+                     count * 4096, 1.0f / (count * 4096.0f), a.gy, l);
+  hipLaunchKernelGGL(k_zero_f32, dim3(grid_of(n->nw)), dim3(256), 0, s, n->g, n->nw);
+  backward(n, a, count, s);
+  hipError_t e = hipMemcpyAsync(grad, n->g, n->nw * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(loss, l, 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (void* p : tmp) hipFree(p);
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

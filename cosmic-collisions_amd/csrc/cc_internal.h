@@ -19,6 +19,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
+struct cc_ff;
+
 namespace cc {
 
 constexpr int SLOTS = 8;             // seats per contact cell
@@ -51,7 +55,6 @@ struct SimU {           // the step pass's uniforms, already FP32
   float settle, kick;
   float bulk0[3], bulk1[3], cen0[3], cen1[3], spin0[3], spin1[3];
   Grid grid;
-  Box box;
   float gm, eps2;       // G m, monopole softening FH^2
   int field;            // 1: constant gravity gField
   float gField[3];
@@ -93,14 +96,20 @@ struct GridBuf {
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
-void launch_deposit(const float4* pos, const float* mat, int N, const Box& b, int box, float eps2,
+void launch_put_box(const Box& b, Box* dst, hipStream_t s);
+void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s);
+void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2,
                     MeshBuf& m, hipStream_t s);
 void launch_coarse(MeshBuf& m, hipStream_t s);
 void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s);
-void launch_pp(const float4* pos, const float* mat, const Box& b, const int* index, int count, float gm,
+// the exact far field (skipped while *gate is nonzero) and the per-cell pass from given far-field buffers
+void launch_far(const float4* coarse, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
+void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
+                      float eps2, float gm, hipStream_t s);
+void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm,
                float cut, float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force,
                hipStream_t s);
-void launch_sim(const SimU& u, const float4* pos, const float4* vel, const float* mat, const float* q,
+void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
                 const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
 void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux, const float* q,
@@ -109,7 +118,7 @@ void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux
                   float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
 void launch_couple(int N, const CoupleU& c, const float4* pos, const float4* vel, const float4* aux, const float* mat,
                    float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
-void launch_diag(const SimU& u, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
+void launch_diag(const SimU& u, const Box* bx, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
                  const MeshBuf& m1, const float* ppSlot, const float4* ppForce, float4* out, hipStream_t s);
 void launch_pp_slot(int N, const int* index, int count, float* slot, hipStream_t s);
 // Morton order: keys from FP64 arithmetic, a counting sort over 2^21 keys made
@@ -122,4 +131,25 @@ void launch_permute(int N, const int* perm, const float4* pos, const float4* vel
 void launch_books(int N, double mref, const float4* pos, const float4* vel, const float* mat, const float4* diag,
                   double* partial, double* out, hipStream_t s);
 
+// ---- the learned far field (cc_ff.hip) ----
+struct FFNet;
+struct FFWork;
+size_t ff_weight_count();
+FFNet* ff_create(int cap, std::string& err);
+void ff_destroy(FFNet* n);
+FFWork* ffwork_create(std::string& err);
+void ffwork_destroy(FFWork* w);
+// the network's far field for box `which` of one instance (kernels skipped unless *gate is nonzero, when given)
+void ff_infer(const FFNet* n, FFWork* wk, const float4* coarse, const float4* mono, const Box* bx, int which,
+              float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
+// a training sample from a box's coarse mesh and exact far field; -1 when the record is full
+int ff_record(FFNet* n, const float4* coarse, const float4* mono, const Box* bx, int which, const float4* farA,
+              const float4* farT0, const float4* farT1, hipStream_t s);
+// the judge: relative force error of `force` against `ref` -> stats and the gate for the coming mesh steps
+void launch_judge(const float4* force, const float4* ref, double* sums, float tol, float* stats, int* gate,
+                  hipStream_t s);
+void launch_count_learned(const int* gate, float* stats, hipStream_t s);
+
 }  // namespace cc
+
+cc::FFNet* ff_net(cc_ff* f);

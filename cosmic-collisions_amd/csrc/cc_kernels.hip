@@ -217,8 +217,8 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
 // the sums are exact and independent of the order the adds land in.
 constexpr int TILE = 8;
 __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos, const float* __restrict__ mat, int N,
-                                                 Box b, int which, const float4* __restrict__ mono, float eps2,
-                                                 unsigned long long* acc) {
+                                                 const Box* __restrict__ bxp, int which,
+                                                 const float4* __restrict__ mono, float eps2, unsigned long long* acc) {
   __shared__ unsigned long long tile[TILE * TILE * TILE * 4];
   __shared__ int lo[3], hi[3];
   int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -226,6 +226,7 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
   float4 p = live ? pos[i] : make_float4(0, 0, 0, 0);
   float mr = live ? mat[i] : 0.0f;
   float gx = 0, gy = 0, gz = 0;
+  const Box b = *bxp;
   int mine = live ? box_of(b, p.x, p.y, p.z, gx, gy, gz) : -2;
   bool dep = mine == which;
   int cx = 0, cy = 0, cz = 0;
@@ -300,12 +301,26 @@ __global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc,
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
-void launch_deposit(const float4* pos, const float* mat, int N, const Box& b, int box, float eps2, MeshBuf& m,
+void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2, MeshBuf& m,
                     hipStream_t s) {
   int blocks = (N + 255) / 256;
   hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, b, box, m.mono, eps2, m.acc);
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, m.occN);
   hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.occ, m.occN);
+}
+
+// The mesh boxes live in device memory so that a captured step cycle reads
+// the box of the cycle it is replayed for: k_put_box writes one (eager
+// steps), k_box_next takes the next of a call's schedule (graph replays).
+__global__ void k_put_box(Box b, Box* dst) {
+  if (threadIdx.x == 0) *dst = b;
+}
+__global__ void k_box_next(const Box* __restrict__ sched, int* idx, Box* dst) {
+  if (threadIdx.x == 0) { int k = *idx; *dst = sched[k]; *idx = k + 1; }
+}
+void launch_put_box(const Box& b, Box* dst, hipStream_t s) { hipLaunchKernelGGL(k_put_box, dim3(1), dim3(64), 0, s, b, dst); }
+void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_next, dim3(1), dim3(64), 0, s, sched, idx, dst);
 }
 
 // ------------------------------------------------------ coarse + mono (A5) --
@@ -342,7 +357,8 @@ __global__ void __launch_bounds__(256) k_mono(const float4* __restrict__ coarse,
 // -------------------------------------------------------- far field (A6) --
 // One wavefront per coarse block; its lanes share the 4096 source blocks.
 __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ coarse, float4* farA, float4* farT0,
-                                             float4* farT1) {
+                                             float4* farT1, const int* __restrict__ gate) {
+  if (gate && *gate != 0) return;   // the learned far field is standing in
   int lane = threadIdx.x & 63;
   int me = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (me >= 4096) return;
@@ -440,22 +456,31 @@ void launch_coarse(MeshBuf& m, hipStream_t s) {
   hipLaunchKernelGGL(k_coarse, dim3(16), dim3(256), 0, s, m.fine, m.coarse);
   hipLaunchKernelGGL(k_mono, dim3(1), dim3(256), 0, s, m.coarse, m.mono);
 }
+void launch_far(const float4* coarse, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, coarse, farA, farT0, farT1, gate);
+}
+void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
+                      float eps2, float gm, hipStream_t s) {
+  launch_fill_f32(reinterpret_cast<float*>(force), (size_t)FINE_TEXELS * 4, 0.0f, s);
+  hipLaunchKernelGGL(k_cell, dim3(2048), dim3(256), 0, s, m.fine, m.coarse, farA, farT0, farT1, m.occ, m.occN, eps2,
+                     gm, force);
+}
 void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s) {
-  hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, m.coarse, m.farA, m.farT0, m.farT1);
-  launch_fill_f32(reinterpret_cast<float*>(m.force), (size_t)FINE_TEXELS * 4, 0.0f, s);
-  hipLaunchKernelGGL(k_cell, dim3(2048), dim3(256), 0, s, m.fine, m.coarse, m.farA, m.farT0, m.farT1, m.occ, m.occN,
-                     eps2, gm, m.force);
+  launch_far(m.coarse, m.farA, m.farT0, m.farT1, nullptr, s);
+  launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, s);
 }
 
 // ------------------------------------------------------ pairwise P3M (A11) --
-__global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ pos, const float* __restrict__ mat, Box b,
-                                                   const int* __restrict__ index, float4* list, float* box) {
+__global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ pos, const float* __restrict__ mat,
+                                                   const Box* __restrict__ bxp, const int* __restrict__ index,
+                                                   float4* list, float* box) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= PP_CAP) return;
   int i = index[s];
   if (i < 0) { list[s] = make_float4(0, 0, 0, 0); box[s] = -1.0f; return; }
   float4 p = pos[i];
   float gx, gy, gz;
+  const Box b = *bxp;
   box[s] = (float)box_of(b, p.x, p.y, p.z, gx, gy, gz);
   list[s] = make_float4(p.x, p.y, p.z, mat[i]);
 }
@@ -537,7 +562,7 @@ __global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ parti
   force[slice * PP_CAP + me] = s;
 }
 
-void launch_pp(const float4* pos, const float* mat, const Box& b, const int* index, int count, float gm, float cut,
+void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm, float cut,
                float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force, hipStream_t s) {
   hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, b, index, list, box);
   int tb = (count + 255) / 256;   // seats past count only need the zero that k_pp_sum writes
@@ -629,7 +654,8 @@ __device__ __forceinline__ bool seat_passes(uint32_t sv, int me, float lox, floa
   return j != me && !(dot3(dqx, dqy, dqz, dqx, dqy, dqz) > u.reach2);
 }
 
-__global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, float touch2r, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict__ bxp, float touch2r,
+                                                   const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float* __restrict__ mat,
                                                    const float* __restrict__ q, const uint32_t* __restrict__ seats,
                                                    const float4* __restrict__ force0, const float4* __restrict__ mono0,
@@ -723,7 +749,8 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, float touch2r, const 
     gvx = u.gField[0]; gvy = u.gField[1]; gvz = u.gField[2];
   } else {
     float gx, gy, gz;
-    int box = box_of(u.box, px, py, pz, gx, gy, gz);
+    const Box B = *bxp;
+    int box = box_of(B, px, py, pz, gx, gy, gz);
     float4 m0 = mono0[0], m1 = mono1[0];
     if (box >= 0) {
       const float4* F = box == 1 ? force1 : force0;
@@ -739,14 +766,14 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, float touch2r, const 
       }
       float4 sc = F[SCRATCH];
       gvx -= sc.x * u.gm; gvy -= sc.y * u.gm; gvz -= sc.z * u.gm;
-      if (u.box.boxB > 0.5f) {
+      if (B.boxB > 0.5f) {
         float ox, oy, oz;
         if (box == 0) cross_at(m1, m0, u.gm, u.eps2, ox, oy, oz); else cross_at(m0, m1, u.gm, u.eps2, ox, oy, oz);
         gvx += ox; gvy += oy; gvz += oz;
       }
     } else {
       mass_at(px, py, pz, m0, u.gm, u.eps2, gvx, gvy, gvz);
-      if (u.box.boxB > 0.5f) {
+      if (B.boxB > 0.5f) {
         float ox, oy, oz;
         mass_at(px, py, pz, m1, u.gm, u.eps2, ox, oy, oz);
         gvx += ox; gvy += oy; gvz += oz;
@@ -799,12 +826,12 @@ float touch_r2(float touch) {
   return x;
 }
 
-void launch_sim(const SimU& u, const float4* pos, const float4* vel, const float* mat, const float* q,
+void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
                 const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s) {
   int blocks = (u.N + SIM_BLOCK - 1) / SIM_BLOCK;
   blocks = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
-  hipLaunchKernelGGL(k_sim, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, touch_r2(u.touch), pos, vel, mat, q, seats,
+  hipLaunchKernelGGL(k_sim, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, touch_r2(u.touch), pos, vel, mat, q, seats,
                      m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ);
 }
 
@@ -895,7 +922,8 @@ void launch_couple(int N, const CoupleU& c, const float4* pos, const float4* vel
 }
 
 // --------------------------------------------------------- read-out (A16) --
-__global__ void __launch_bounds__(256) k_diag(SimU u, const float4* __restrict__ pos, const float* __restrict__ q,
+__global__ void __launch_bounds__(256) k_diag(SimU u, const Box* __restrict__ bxp, const float4* __restrict__ pos,
+                                              const float* __restrict__ q,
                                               const uint32_t* __restrict__ seats, const float4* __restrict__ force0,
                                               const float4* __restrict__ mono0, const float4* __restrict__ force1,
                                               const float4* __restrict__ mono1, const float* __restrict__ ppSlot,
@@ -914,7 +942,8 @@ __global__ void __launch_bounds__(256) k_diag(SimU u, const float4* __restrict__
     if ((int)(sv8[k] & 0x3FFFFu) == me) seated = 1.0f;
   }
   float phi = 0.0f, off = 0.0f, gx, gy, gz;
-  int box = box_of(u.box, P.x, P.y, P.z, gx, gy, gz);
+  const Box B = *bxp;
+  int box = box_of(B, P.x, P.y, P.z, gx, gy, gz);
   float4 m0 = mono0[0], m1 = mono1[0];
   if (box >= 0) {
     const float4* F = box == 1 ? force1 : force0;
@@ -926,10 +955,10 @@ __global__ void __launch_bounds__(256) k_diag(SimU u, const float4* __restrict__
       float w = gmix(1.0f - tx, tx, (float)o0) * gmix(1.0f - ty, ty, (float)o1) * gmix(1.0f - tz, tz, (float)o2);
       phi += F[fine_at(min(ix + o0, 63), min(iy + o1, 63), min(iz + o2, 63))].w * w;
     }
-    if (u.box.boxB > 0.5f) phi += box == 0 ? pot_at(P.x, P.y, P.z, m1, u.gm, u.eps2) : pot_at(P.x, P.y, P.z, m0, u.gm, u.eps2);
+    if (B.boxB > 0.5f) phi += box == 0 ? pot_at(P.x, P.y, P.z, m1, u.gm, u.eps2) : pot_at(P.x, P.y, P.z, m0, u.gm, u.eps2);
   } else {
     phi = pot_at(P.x, P.y, P.z, m0, u.gm, u.eps2);
-    if (u.box.boxB > 0.5f) phi += pot_at(P.x, P.y, P.z, m1, u.gm, u.eps2);
+    if (B.boxB > 0.5f) phi += pot_at(P.x, P.y, P.z, m1, u.gm, u.eps2);
     off = 2.0f;
   }
   float seat = ppSlot[me];
@@ -939,9 +968,9 @@ __global__ void __launch_bounds__(256) k_diag(SimU u, const float4* __restrict__
   }
   out[me] = make_float4(phi, seated + off, taken, q[me]);
 }
-void launch_diag(const SimU& u, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
+void launch_diag(const SimU& u, const Box* bx, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
                  const MeshBuf& m1, const float* ppSlot, const float4* ppForce, float4* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag, dim3((u.N + 255) / 256), dim3(256), 0, s, u, pos, q, seats, m0.force, m0.mono,
+  hipLaunchKernelGGL(k_diag, dim3((u.N + 255) / 256), dim3(256), 0, s, u, bx, pos, q, seats, m0.force, m0.mono,
                      m1.force, m1.mono, ppSlot, ppForce, out);
 }
 

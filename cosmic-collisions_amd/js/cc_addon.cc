@@ -35,13 +35,19 @@ struct Args {
   }
 };
 
+// A context handle is a JS external holding this box: destroy() frees the
+// context at once and empties the box, the finalizer frees what is left.
+struct Handle {
+  cc_ctx* c;
+};
+
 cc_ctx* ctx_of(napi_env env, napi_value v) {
   void* p = nullptr;
-  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
-    napi_throw_type_error(env, nullptr, "expected a cc context handle");
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !static_cast<Handle*>(p)->c) {
+    napi_throw_type_error(env, nullptr, "expected a live cc context handle");
     return nullptr;
   }
-  return static_cast<cc_ctx*>(p);
+  return static_cast<Handle*>(p)->c;
 }
 
 bool is_nullish(napi_env env, napi_value v) {
@@ -135,7 +141,11 @@ napi_value undefined(napi_env env) {
   return u;
 }
 
-void finalize_ctx(napi_env, void* data, void*) { cc_destroy(static_cast<cc_ctx*>(data)); }
+void finalize_ctx(napi_env, void* data, void*) {
+  Handle* h = static_cast<Handle*>(data);
+  if (h->c) cc_destroy(h->c);
+  delete h;
+}
 
 // ---------------------------------------------------------------- exports --
 napi_value Create(napi_env env, napi_callback_info info) {
@@ -146,7 +156,7 @@ napi_value Create(napi_env env, napi_callback_info info) {
   int r = cc_create(&p, &c);
   if (r) return throw_cc(env, nullptr, "cc_create", r);
   napi_value out;
-  NAPI_OK(env, napi_create_external(env, c, finalize_ctx, nullptr, &out));
+  NAPI_OK(env, napi_create_external(env, new Handle{ c }, finalize_ctx, nullptr, &out));
   return out;
 }
 
@@ -322,6 +332,27 @@ napi_value SetStepNo(napi_env env, napi_callback_info info) {
   RET(cc_set_step_no(c, n), "cc_set_step_no");
 }
 
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  void* p = nullptr;
+  if (napi_get_value_external(env, a.argv[0], &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a cc context handle");
+    return nullptr;
+  }
+  Handle* h = static_cast<Handle*>(p);
+  if (h->c) { cc_destroy(h->c); h->c = nullptr; }
+  return undefined(env);
+}
+
+napi_value SetGraphs(napi_env env, napi_callback_info info) {
+  CTX0();
+  bool on = true;
+  napi_value v = a.argv[1];
+  napi_coerce_to_bool(env, v, &v);
+  napi_get_value_bool(env, v, &on);
+  RET(cc_set_graphs(c, on ? 1 : 0), "cc_set_graphs");
+}
+
 napi_value Rigid(napi_env env, napi_callback_info info) {
   CTX0();
   double b0[3], b1[3], c0[3], c1[3], r0[9], r1[9];
@@ -375,11 +406,12 @@ napi_value Books(napi_env env, napi_callback_info info) {
 
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char* name; napi_callback cb; } fns[] = {
-    { "create", Create }, { "setParams", SetParams }, { "uploadState", Upload }, { "downloadState", Download },
+    { "create", Create }, { "destroy", Destroy }, { "setParams", SetParams }, { "uploadState", Upload }, { "downloadState", Download },
     { "setBoxes", SetBoxes }, { "setMono", SetMono }, { "slots", Slots }, { "readSeats", ReadSeats },
     { "deposit", Deposit }, { "coarse", Coarse }, { "cell", Cell }, { "mesh", Mesh }, { "readMesh", ReadMesh },
     { "writeMesh", WriteMesh }, { "setPPTable", SetPPTable }, { "setPP", SetPP }, { "pp", PP }, { "readPP", ReadPP },
     { "sim", Sim }, { "step", Step }, { "stepN", StepN }, { "getStepNo", GetStepNo }, { "setStepNo", SetStepNo },
+    { "setGraphs", SetGraphs },
     { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "books", Books }, { "sync", Sync },
   };
   for (auto& f : fns) {

@@ -141,6 +141,7 @@ function createCraterEngine(opts) {
   function build(S) {
     const run = buildCraterRun(Object.assign({}, DEFAULTS, S || {}));
     sim = run.sim;
+    if (ctx) addon.destroy(ctx);   // a rebuild frees the last run's device objects (src/nbody.js:208-245)
     ctx = addon.create(params());
     addon.uploadState(ctx, run.pos, run.vel, sim.matR, new Float32Array(sim.N), sim.N);
     sim.initial = { pos: run.pos, vel: run.vel };
@@ -186,7 +187,8 @@ function createCraterEngine(opts) {
     addon.downloadState(ctx, st.pos, st.vel, st.aux, st.q, st.mat, N);
     return st;
   }
-  return { build, step, settleSome, approachStep, couple, readState, get sim() { return sim; }, get ctx() { return ctx; },
+  function destroy() { if (ctx) { addon.destroy(ctx); ctx = null; } }
+  return { build, step, settleSome, approachStep, couple, readState, destroy, get sim() { return sim; }, get ctx() { return ctx; },
            PRESETS, DEFAULTS };
 }
 

@@ -83,9 +83,13 @@ function pairTable(bins, cut) {
 // encounter walked back to the start line, the contact spring, step and
 // dashpot (src/nbody.js:114-250).  Host float64 arithmetic only; the result
 // is the initial Float32 state and the sim object.
+// Batched sweeps (no reference counterpart) add two options: opt.N, a
+// particle count outside N_CHOICES (a multiple of 512, at most 262144: 512
+// wide), and opt.seed, the bodies' own seed (the reference fixes 1234567).
 function buildRun(opt) {
-  const N = N_CHOICES[opt.nIdx];
-  const W = N >= 131072 ? 512 : N >= 32768 ? 256 : 128, H = N / W;
+  const N = opt.N > 0 ? opt.N : N_CHOICES[opt.nIdx];
+  if (opt.N > 0 && (N % 512 !== 0 || N > 262144 || N < 1024)) throw new Error('N must be a multiple of 512 in [1024, 262144]');
+  const W = opt.N > 0 ? 512 : N >= 131072 ? 512 : N >= 32768 ? 256 : 128, H = N / W;
   const q = opt.q, dens = opt.dens, coreF = opt.core;
   const coreV = coreF / (RHO_R - coreF * (RHO_R - 1));
   const coreF2 = opt.coreImp === undefined ? coreF : opt.coreImp;
@@ -100,7 +104,7 @@ function buildRun(opt) {
   const w0 = [0, opt.spin0, 0], w1 = [0, opt.spin1 * Math.sqrt(dens), 0];
   const e1 = bodies.spheroidEcc(Math.abs(opt.spin0)), e2 = bodies.spheroidEcc(Math.abs(opt.spin1));
   const Re1 = R1 / Math.pow(1 - e1 * e1, 1 / 6), Re2 = R2 / Math.pow(1 - e2 * e2, 1 / 6);
-  const rand = bodies.seeded(1234567);
+  const rand = bodies.seeded(opt.seed > 0 ? opt.seed : 1234567);
   const b1 = bodies.onion(n1, s, a, rand, e1), b2 = bodies.onion(n2, s, a, rand, e2);
   const pos = new Float32Array(N * 4), vel = new Float32Array(N * 4), matR = new Float32Array(N), lay = new Uint8Array(N);
   // layers by whole shells: the core the shells nearest its volume share, the crust the outer shell
@@ -183,6 +187,7 @@ function createEngine(opts) {
     const run = buildRun(opt);
     sim = run.sim;
     sim.gen = ++gen;
+    if (ctx) addon.destroy(ctx);   // a rebuild frees the last run's device objects (src/nbody.js:208-245)
     ctx = addon.create(params());
     addon.uploadState(ctx, run.pos, run.vel, sim.matR, new Float32Array(sim.N), sim.N);
     addon.setPPTable(ctx, PP_TAB);
@@ -348,6 +353,8 @@ function createEngine(opts) {
 
   return {
     init() { return this; },
+    // the device objects freed now rather than when the handle is collected
+    destroy() { if (ctx) { addon.destroy(ctx); ctx = null; } },
     build, step, stepMany, settleSome, rigidStep, soften,
     analyze, collect() {}, readState,
     sortNow() { analyses = 0; return analyze(); },

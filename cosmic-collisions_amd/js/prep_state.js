@@ -35,4 +35,5 @@ const keep = ['N', 'W', 'H', 'G', 'n1', 'n2', 'm', 'a', 'cell', 'dt', 'k', 'c', 
 const run = { opt, sim: {}, report: { core: r.core, L: r.L, KE: r.KE, PE: r.PE, EL: r.EL, Q: r.Q, M: r.M, drift: r.drift } };
 for (const k of keep) run.sim[k] = sim[k];
 fs.writeFileSync(path.join(out, 'run.json'), JSON.stringify(run, null, 1));
+NB.destroy();
 log('written ' + out);

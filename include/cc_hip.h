@@ -102,8 +102,19 @@ int cc_sim(cc_ctx* ctx, const cc_sim_args* a);
  * (the crater, chicxulub.html:779-782) no mesh and gravity every step.  box_L is the
  * boxes() uniform for the mesh step (NULL: keep the last); stepNo advances. */
 int cc_step(cc_ctx* ctx, int settle, int kick, const cc_sim_args* a);
-/* n steps without settle or kick; boxL[n_mesh][7] = {LA, LB, boxB} for each mesh step in order (NULL: keep) */
+/* n steps without settle or kick; boxL[n_mesh][7] = {LA, LB, boxB} for each mesh step in order (NULL: keep).
+ * Whole cycles of meshEvery steps from a mesh step replay one captured HIP graph each (captured on first
+ * use for the present params, loose-list count, box B and buffer parities); the rest run eagerly. */
 int cc_step_n(cc_ctx* ctx, int n, const cc_sim_args* a, const double* boxL, int nBoxL);
+/* step_n's cycle graphs on (default) or off (every step launched eagerly; same results) */
+int cc_set_graphs(cc_ctx* ctx, int on);
+/* One pass repeated back to back between HIP events on the ctx's stream, for
+ * measurement: 0 the step pass (no settle, kick or gravity impulse: it
+ * advances the state), 1 the seats, 2 the mesh (exact far field), 3 the
+ * pairwise pass, 4 the mesh as on a mesh step between judges (the learned
+ * far field if it is standing in), 5 the mesh as on a judge step;
+ * *ms = device time per pass. */
+int cc_time_pass(cc_ctx* ctx, int pass, int reps, double* ms);
 int cc_get_step_no(cc_ctx* ctx, int* stepNo);
 int cc_set_step_no(cc_ctx* ctx, int stepNo);
 /* rigidStep's GPU half (RIGID_FS, src/glsl-sim.js:537-570): rot column-major */
@@ -129,6 +140,61 @@ int cc_books(cc_ctx* ctx, double* out);
 int cc_sync(cc_ctx* ctx);
 /* device time of the last cc_step_n call, ms (HIP events on the ctx's stream) */
 int cc_last_step_ms(cc_ctx* ctx, double* ms);
+
+/* The learned far field (no reference counterpart).  A 3D convolutional
+ * operator on the 16^3 coarse mesh -- 4 levels, 32 channels, 3x3x3 kernels,
+ * BF16 weights on the matrix cores with FP32 accumulation -- that stands in
+ * for the block-to-block far field (FAR_FS, src/glsl-sim.js:263-295): coarse
+ * masses and centres of mass in, each block's far acceleration, potential
+ * and tidal tensor out.  Trained on the GPU by the library's own forward and
+ * backward kernels on samples recorded from the exact far field.
+ * Weights: FP32, per layer [27 taps][cin][cout] then [cout] biases, layers in
+ * order (cc_ff_weight_count floats). */
+typedef struct cc_ff cc_ff;
+int cc_ff_create(int capacity, cc_ff** out);     /* capacity: training samples held on the device */
+int cc_ff_destroy(cc_ff* f);
+const char* cc_ff_last_error(cc_ff* f);
+int cc_ff_weight_count(int64_t* n);
+int cc_ff_get_weights(cc_ff* f, float* w, int64_t n);
+int cc_ff_set_weights(cc_ff* f, const float* w, int64_t n);
+/* a sample from box `box` of ctx's last mesh pass, whose far field must be the exact one (cc_mesh) */
+int cc_ff_record(cc_ff* f, cc_ctx* ctx, int box);
+/* samples in the network's own units: in[4096][16] (channels 0-3 mass, centre-of-mass offset), target[4096][16]
+ * (0-2 acceleration, 3 potential, 4-9 tidal xx yy zz xy xz yz), mask[4096] (1: occupied block) */
+int cc_ff_push_sample(cc_ff* f, const float* in, const float* target, const float* mask);
+int cc_ff_read_sample(cc_ff* f, int k, float* in, float* target, float* mask);
+int cc_ff_samples(cc_ff* f, int* count);
+int cc_ff_clear(cc_ff* f);
+/* iters Adam steps (lr) on seeded batches (<= 64) of the record; losses[iters] */
+int cc_ff_train(cc_ff* f, int iters, int batch, double lr, uint32_t seed, double* losses);
+/* the network on samples [first, first+count): outputs (count*4096*16, or NULL) and the relative L2 error of the
+ * accelerations and of all ten channels over occupied blocks */
+int cc_ff_eval(cc_ff* f, int first, int count, float* out, double* relAcc, double* relAll);
+/* the masked mean-squared loss and its gradient wrt every weight (no update), samples [first, first+count) */
+int cc_ff_gradient(cc_ff* f, int first, int count, float* grad, double* loss);
+/* attach (f) or detach (NULL) the network to a ctx's step: every judgeEvery steps (a multiple of meshEvery)
+ * the exact far field makes the force mesh and the network's force mesh is measured against it; the
+ * network stands in on the mesh steps in between while the last relative force error was <= tol */
+int cc_set_far_field(cc_ctx* ctx, cc_ff* f, int judgeEvery, double tol);
+/* per box: last relative force error, judge passes, fallbacks, mesh steps run on the network (out[8]) */
+int cc_ff_stats(cc_ctx* ctx, double* out);
+
+/* Batched sweeps (no reference counterpart: the reference runs one instance
+ * per page).  Independent instances, each its own ctx (any N = W*H up to
+ * 262144), stepped together: every instance's cycle of meshEvery steps is one
+ * branch of one captured HIP graph, so the batch advances a cycle per launch.
+ * Results are those of each instance stepped alone with cc_step_n.  The ctxs
+ * stay owned by the caller and must outlive the batch; all must share
+ * meshEvery and stand at the same step of the mesh cycle. */
+typedef struct cc_batch cc_batch;
+int cc_batch_create(cc_ctx* const* ctxs, int n, cc_batch** out);
+int cc_batch_destroy(cc_batch* b);
+const char* cc_batch_last_error(cc_batch* b);
+/* n steps of every instance; args[n_inst]; boxL: the instances' mesh-step
+ * boxes one after another ({LA, LB, boxB} x nBoxL[i] for instance i), or NULL */
+int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* boxL, const int* nBoxL);
+/* device time of the last cc_batch_step_n, ms */
+int cc_batch_last_ms(cc_batch* b, double* ms);
 
 #ifdef __cplusplus
 }

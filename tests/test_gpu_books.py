@@ -1,0 +1,105 @@
+"""Conservation over 1000 steps (the books of src/nbody.js:831-850 as
+index.html:338-348 reads them against the first look under way), and the
+gravity's error against an FP64 direct sum, on the GPU at 16,384 particles.
+
+The run is the plain theia-like impact (0.89 + 0.11 M_earth at 45 deg,
+1.1 v_esc, 50 deg inclination, 8 h days) driven by the JavaScript engine
+through its own build, settle, approach and kick, then 1000 full steps with
+an analysis every 360 steps as the reference's runTo drives it.  The
+reference keeps momentum, angular momentum and energy "to a fraction of a
+percent" (README.md:71-72): the bars are |dL/L| < 1 %, |energy balance| < 1 %
+of the reference scale, and a barycentre drift under 1 % of v_esc.
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import golden_util as gu
+import oracle
+
+sys.path.insert(0, os.path.join(gu.REPO, "cosmic-collisions_amd"))
+import ccamd  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+JS = os.path.join(gu.REPO, "cosmic-collisions_amd", "js")
+
+RUN = r"""
+const { createEngine } = require(%s);
+const NB = createEngine();
+const sim = NB.build(%s);
+while (sim.phase === 'settle') NB.settleSome(64);
+while (sim.phase === 'approach') NB.rigidStep(0.01);
+const books = [];
+const look = () => {
+  const r = NB.analyze(), ref = sim.ref, E = r.KE + r.PE + r.EL;
+  books.push({ step: sim.stepNo, t: sim.t, L: r.L, dL: (r.L - ref.L) / ref.L, balance: (E + r.Q - ref.EQ) / ref.scale,
+               drift: r.drift, Q: r.Q, KE: r.KE, PE: r.PE, EL: r.EL, unseated: r.unseated, loose: r.pp.loose });
+  console.error(JSON.stringify(books[books.length - 1]));
+};
+look();
+let done = 0;
+while (done < %d) { const n = Math.min(360, %d - done); NB.stepMany(n); done += n; look(); }
+console.log(JSON.stringify({ books, vesc: sim.vesc, N: sim.N }));
+"""
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_books_over_1000_steps(tmp_path):
+    opt = json.loads(str(gu.fixture("c1_build")["opt"]))
+    script = tmp_path / "books.js"
+    script.write_text(RUN % (json.dumps(os.path.join(JS, "nbody.js")), json.dumps(opt), 1000, 1000))
+    print("books run: C1 16k, 1000 steps", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "150", "node", str(script)], capture_output=True, text=True, cwd=JS)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    books = out["books"]
+    for b in books:
+        print(json.dumps(b))
+    last = books[-1]
+    assert last["step"] >= 1000
+    assert abs(last["dL"]) < 1e-2, last["dL"]
+    assert abs(last["balance"]) < 1e-2, last["balance"]
+    assert last["drift"] < 1e-2 * out["vesc"], (last["drift"], out["vesc"])
+    assert last["Q"] > 0, "the impact heats"
+
+
+@pytest.mark.parametrize("name", ["impact", "twobox"])
+def test_force_error_vs_fp64_direct_sum_gpu(name):
+    """The library's gravity (aux.xyz of one step pass, seats + mesh +
+    pairwise rebuilt on the GPU) against the FP64 direct sum softened at a;
+    the same bounds as the reference's own output meets."""
+    fx = gu.fixture("c1_" + name)
+    s = json.loads(str(fx["sim"]))
+    p = ccamd.params(N=s["N"], W=s["W"], G=s["G"], m=s["m"], a=s["a"], cell=s["cell"], dt=s["dt"], k=s["k"],
+                     c=s["c"], creep=s["creep"], damp=s["damp"], tK=s["tK"], kappa=s["kappa"], spin0=s["spin0"],
+                     spin1=s["spin1"])
+    ctx = ccamd.Context(p)
+    try:
+        ctx.upload(fx["pos"], fx["vel"], fx["mat"], fx["q"])
+        u = gu.draws(fx, "dep")[0]
+        ctx.set_boxes([float(x) for x in u["uLA"]], [float(x) for x in u["uLB"]], float(u["uBoxB"][0]) > 0.5)
+        if name == "twobox":
+            ctx.set_mono(1, gu.mesh(fx, 1)["mono"])
+        ctx.set_pp_table(fx["pptab"])
+        ctx.set_pp(fx["pp_index"])
+        ctx.slots()
+        ctx.mesh()
+        ctx.pp()
+        ctx.sim(ccamd.sim_args(gdt=8 * s["dt"]))
+        P, V, A, Q, M = ctx.download()
+    finally:
+        ctx.close()
+    N = fx["pos"].shape[0]
+    d = oracle.direct_f64(fx["pos"], fx["mat"], s["m"], s["a"] ** 2)[:, :3]
+    g = A[:, :3].astype(np.float64)
+    e = np.linalg.norm(g - d, axis=1) / np.linalg.norm(d, axis=1)
+    rms = float(np.sqrt(np.mean(np.sum((g - d) ** 2, 1)) / np.mean(np.sum(d * d, 1))))
+    print(name, "rms", rms, "median", float(np.median(e)), "p99", float(np.percentile(e, 99)))
+    ref = fx["sim_full_aux"].reshape(N, 4)[:, :3]
+    assert gu.rel_err_vec(g, ref).max() < 1e-4
+    assert rms < 0.10 and np.median(e) < 0.08

@@ -13,7 +13,7 @@ import oracle
 
 sys.path.insert(0, os.path.join(gu.REPO, "cosmic-collisions_amd"))
 sys.path.insert(0, gu.REPO)
-import bench  # noqa: E402  (the twins scalars and the lattice state)
+import synthetic as bench  # noqa: E402  (the twins scalars and the lattice state)
 import ccamd  # noqa: E402
 
 pytestmark = pytest.mark.gpu
