@@ -24,6 +24,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(REPO, "cosmic-collisions_amd")
 sys.path.insert(0, PKG)
+import runs  # noqa: E402  (the library's prepared runs; loads the HIP library lazily)
 
 FL = 6.0
 FH = 2 * FL / 64
@@ -61,6 +62,19 @@ def cpu_baseline(sc, pos, vel, mat, q, budget_s=15.0):
     return {"value": n * sc["N"] / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": "%d step passes (contacts, heat, gravity gather, integration) of the oracle on the same "
                       "262,144-particle state, one core, %.1f s" % (n, el)}
+
+
+def measured_traffic():
+    """k_sim's HBM bytes per launch as the committed counter profile states it
+    (FETCH_SIZE + WRITE_SIZE scaled by the same counters on a plain copy of
+    known size), or None."""
+    path = os.path.join(REPO, "profiles", "twins262k_plain_step.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("k_sim HBM traffic per launch:"):
+            return float(line.split(":")[1].split()[0])
+    return None
 
 
 def batch_leg(out_dir, count, steps, warmup, reuse):
@@ -136,8 +150,6 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)
         return 2
-    import runs
-
     run, st = runs.prepare(runs.TWINS, args.state_dir, reuse=args.reuse_state)
     import torch
     R = runs.Runner(run, st)
@@ -153,6 +165,7 @@ def main():
     value = N * args.steps / wall
     # the step pass alone, 200 back to back between HIP events on the library's stream
     sim_ms = R.ctx.time_pass("sim", 200)
+    R.ctx.time_pass("copy", 20)   # a copy of known size, the traffic counters' calibration in a profiled run
     alg_bytes = 92.0 * N     # pos+vel+mat+q read, pos+vel+aux+q written: each particle's own data once
     achieved = alg_bytes / (sim_ms * 1e-3) / 1e9
     out = {
@@ -167,7 +180,7 @@ def main():
                                                               "Morton-sorted, timed from first touch"},
         "device_ms_per_step": dev_ms / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_sim (contacts + heat + gravity gather + symplectic Euler)", "kernel_ms": sim_ms},
     }
     if not args.no_cpu:

@@ -38,7 +38,7 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_read_mesh", "cc_write_mesh", "cc_set_pp_table", "cc_set_pp", "cc_pp", "cc_read_pp", "cc_sim", "cc_step",
            "cc_step_n", "cc_set_graphs", "cc_time_pass", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_couple",
            "cc_diag", "cc_morton", "cc_books", "cc_sync", "cc_last_step_ms", "cc_batch_create", "cc_batch_destroy", "cc_batch_last_error",
-           "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error",
+           "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
            "cc_set_far_field", "cc_ff_stats"]
@@ -90,7 +90,7 @@ def lib():
             "cc_ff_get_weights": ([P, fp, C.c_int64], C.c_int), "cc_ff_set_weights": ([P, fp, C.c_int64], C.c_int),
             "cc_ff_record": ([P, P, C.c_int], C.c_int), "cc_ff_push_sample": ([P, fp, fp, fp], C.c_int),
             "cc_ff_read_sample": ([P, C.c_int, fp, fp, fp], C.c_int), "cc_ff_samples": ([P, ip], C.c_int),
-            "cc_ff_clear": ([P], C.c_int),
+            "cc_ff_clear": ([P], C.c_int), "cc_ff_set_slope": ([P, C.c_double], C.c_int),
             "cc_ff_train": ([P, C.c_int, C.c_int, C.c_double, C.c_uint32, dp], C.c_int),
             "cc_ff_eval": ([P, C.c_int, C.c_int, fp, dp, dp], C.c_int),
             "cc_ff_gradient": ([P, C.c_int, C.c_int, fp, dp], C.c_int),
@@ -265,9 +265,10 @@ class Context:
         return ms.value
 
     def time_pass(self, which, reps):
-        """Device ms per pass: which = 'sim' | 'slots' | 'mesh' | 'pp' | 'mesh_between' | 'mesh_judge' (cc_time_pass)."""
+        """Device ms per pass (cc_time_pass): which = 'sim' | 'slots' | 'mesh' | 'pp' | 'mesh_between' |
+        'mesh_judge' | 'copy'."""
         ms = C.c_double()
-        k = {"sim": 0, "slots": 1, "mesh": 2, "pp": 3, "mesh_between": 4, "mesh_judge": 5}[which]
+        k = {"sim": 0, "slots": 1, "mesh": 2, "pp": 3, "mesh_between": 4, "mesh_judge": 5, "copy": 6}[which]
         self._chk(lib().cc_time_pass(self.h, k, int(reps), C.byref(ms)), "cc_time_pass")
         return ms.value
 
@@ -390,6 +391,10 @@ class FarField:
     def _chk(self, r, what):
         if r:
             raise CCError("%s failed (%d): %s" % (what, r, lib().cc_ff_last_error(self.h).decode()))
+
+    def set_slope(self, slope):
+        """The leaky ReLU's negative slope (1: a linear operator)."""
+        self._chk(lib().cc_ff_set_slope(self.h, float(slope)), "cc_ff_set_slope")
 
     def weights(self):
         w = np.empty(self.nw, np.float32)

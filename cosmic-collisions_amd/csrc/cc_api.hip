@@ -622,13 +622,19 @@ int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nB
 }
 
 int cc_time_pass(cc_ctx* c, int pass, int reps, double* ms) {
-  if (!c || !ms || reps <= 0 || pass < 0 || pass > 5) return CC_EINVAL;
+  if (!c || !ms || reps <= 0 || pass < 0 || pass > 6) return CC_EINVAL;
   cc_sim_args a;
   memset(&a, 0, sizeof a);
   HIPCHK(c, hipEventRecord(c->ev0, c->s));
   for (int i = 0; i < reps; i++) {
-    int r = pass == 0 ? sim_pass(c, &a) : pass == 1 ? cc_slots(c) : pass == 2 ? mesh_all(c) : pass == 3 ? cc_pp(c)
-          : mesh_all(c, pass == 4 ? c->p.meshEvery : 0);
+    int r = 0;
+    if (pass == 6) {   // the state's positions into the spare aux buffer: N x 16 B read, N x 16 B written
+      launch_copy4(c->pos[c->cur], c->aux[c->cur ^ 1], c->N, c->s);
+      LAUNCHCHK(c);
+    } else {
+      r = pass == 0 ? sim_pass(c, &a) : pass == 1 ? cc_slots(c) : pass == 2 ? mesh_all(c) : pass == 3 ? cc_pp(c)
+        : mesh_all(c, pass == 4 ? c->p.meshEvery : 0);
+    }
     if (r) return r;
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->s));

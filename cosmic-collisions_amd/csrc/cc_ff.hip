@@ -42,16 +42,15 @@ __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ float leaky(float x) { return x > 0.0f ? x : 0.25f * x; }
 
 // ------------------------------------------------------------ convolution --
 // in  [M voxels][CIN] bf16 (M = B*S^3), wt [27][CIN/16][COUT][16] bf16,
-// bias [COUT] f32 or null; act 1: leaky ReLU.  Writes out (bf16, [M][COUT])
+// bias [COUT] f32 or null; slope: the leaky ReLU's (1: none).  Writes out (bf16, [M][COUT])
 // and/or outf (f32), and pre (f32 pre-activation) when asked.
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_conv(const uint16_t* __restrict__ in, int B, int S,
                                               const uint16_t* __restrict__ wt, const float* __restrict__ bias,
-                                              int act, uint16_t* out, float* outf, float* pre,
+                                              float slope, uint16_t* out, float* outf, float* pre,
                                               const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -89,7 +88,7 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t* __restrict__ in, i
       float p = acc[n][j] + bb;
       size_t o = (size_t)vo * COUT + co;
       if (pre) pre[o] = p;
-      float yv = act ? leaky(p) : p;
+      float yv = p > 0.0f ? p : slope * p;
       if (out) out[o] = f2bf(yv);
       if (outf) outf[o] = yv;
     }
@@ -203,12 +202,13 @@ __global__ void k_up_bwd(const float* __restrict__ gout, int B, int S, int C, fl
   }
   gcoarse[e] = s;
 }
-// gradient through the leaky ReLU, to BF16 for the matrix cores
-__global__ void k_act_bwd(const float* __restrict__ gout, const float* __restrict__ pre, int n, int act, uint16_t* g) {
+// gradient through the leaky ReLU (slope 1: none), to BF16 for the matrix cores
+__global__ void k_act_bwd(const float* __restrict__ gout, const float* __restrict__ pre, int n, float slope,
+                          uint16_t* g) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   float d = gout[e];
-  if (act && pre[e] <= 0.0f) d *= 0.25f;
+  if (pre[e] <= 0.0f) d *= slope;
   g[e] = f2bf(d);
 }
 // bias gradient: sum over voxels of g (bf16) per channel
@@ -236,11 +236,22 @@ __global__ void k_zero_f32(float* a, size_t n) {
 
 // ------------------------------------------------------ the mesh interface --
 // Network input from a coarse mesh (64x64 atlas of 4x4 slices of 16^2; xyz
-// mass-weighted world positions): mass in units of the box's total mass
-// times 64, and the block's centre of mass less its centre in block lengths
-// (4 FH); channels 4..15 zero.
+// mass-weighted world positions): each block's mass in units of the box's
+// total mass times 64, and its dipole about the block's centre (that mass
+// times the centre of mass less the centre, in block lengths, 4 FH), which
+// is what the far field of the block depends on to first order; channels
+// 4..15 zero.  The network's output is the far field at the block's centre;
+// the centre of mass, where the per-cell pass wants it, is reached by the
+// tidal tensor (acceleration) and the acceleration (potential).
 __device__ __forceinline__ int coarse_texel(int x, int y, int z) { return ((z >> 2) * 16 + y) * 64 + (z & 3) * 16 + x; }
 constexpr float BLOCK_LEN = (float)(4.0 * FH);
+// the block's centre of mass less its centre, world units
+__device__ __forceinline__ void block_offset(float4 c, int x, int y, int z, const float* L, float& dx, float& dy,
+                                             float& dz) {
+  dx = c.x / c.w - ((4 * x + 2) * (float)FH - L[0]);
+  dy = c.y / c.w - ((4 * y + 2) * (float)FH - L[1]);
+  dz = c.z / c.w - ((4 * z + 2) * (float)FH - L[2]);
+}
 __global__ void k_ff_in(const float4* __restrict__ coarse, const float4* __restrict__ mono,
                         const Box* __restrict__ bxp, int which, int inst, uint16_t* in, float* scale,
                         const int* __restrict__ gate) {
@@ -255,10 +266,12 @@ __global__ void k_ff_in(const float4* __restrict__ coarse, const float4* __restr
   uint16_t* o = in + ((size_t)inst * 4096 + v) * 16;
   float f[4] = { 0, 0, 0, 0 };
   if (c.w > 0.0f) {
+    float dx, dy, dz;
+    block_offset(c, x, y, z, L, dx, dy, dz);
     f[0] = c.w / mt * 64.0f;
-    f[1] = (c.x / c.w - ((4 * x + 2) * (float)FH - L[0])) / BLOCK_LEN;
-    f[2] = (c.y / c.w - ((4 * y + 2) * (float)FH - L[1])) / BLOCK_LEN;
-    f[3] = (c.z / c.w - ((4 * z + 2) * (float)FH - L[2])) / BLOCK_LEN;
+    f[1] = f[0] * dx / BLOCK_LEN;
+    f[2] = f[0] * dy / BLOCK_LEN;
+    f[3] = f[0] * dz / BLOCK_LEN;
   }
   for (int k = 0; k < 16; k++) o[k] = f2bf(k < 4 ? f[k] : 0.0f);
   if (v == 0) scale[inst] = mt;
@@ -268,36 +281,58 @@ __global__ void k_ff_in(const float4* __restrict__ coarse, const float4* __restr
 constexpr float FF_A = 1.0f / (BLOCK_LEN * BLOCK_LEN), FF_P = 1.0f / BLOCK_LEN,
                 FF_T = 1.0f / (BLOCK_LEN * BLOCK_LEN * BLOCK_LEN);
 __global__ void k_ff_out(const float* __restrict__ y, const float4* __restrict__ coarse, const float* __restrict__ scale,
-                         int inst, float4* farA, float4* farT0, float4* farT1, const int* __restrict__ gate) {
+                         const Box* __restrict__ bxp, int which, int inst, float4* farA, float4* farT0, float4* farT1,
+                         const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= 4096) return;
   int x = v & 15, yv = (v >> 4) & 15, z = v >> 8;
   int t = coarse_texel(x, yv, z);
-  if (coarse[t].w <= 0.0f) {
+  float4 c = coarse[t];
+  if (c.w <= 0.0f) {
     farA[t] = make_float4(0, 0, 0, 0); farT0[t] = make_float4(0, 0, 0, 0); farT1[t] = make_float4(0, 0, 0, 0);
     return;
   }
+  const Box bx = *bxp;
+  float dx, dy, dz;
+  block_offset(c, x, yv, z, which ? bx.LB : bx.LA, dx, dy, dz);
   const float* o = y + ((size_t)inst * 4096 + v) * 16;
   float m = scale[inst];
-  farA[t] = make_float4(o[0] * m * FF_A, o[1] * m * FF_A, o[2] * m * FF_A, o[3] * m * FF_P);
-  farT0[t] = make_float4(o[4] * m * FF_T, o[5] * m * FF_T, o[6] * m * FF_T, o[7] * m * FF_T);
-  farT1[t] = make_float4(o[8] * m * FF_T, o[9] * m * FF_T, 0.0f, 0.0f);
+  float ax = o[0] * m * FF_A, ay = o[1] * m * FF_A, az = o[2] * m * FF_A, p = o[3] * m * FF_P;
+  float txx = o[4] * m * FF_T, tyy = o[5] * m * FF_T, tzz = o[6] * m * FF_T, txy = o[7] * m * FF_T;
+  float txz = o[8] * m * FF_T, tyz = o[9] * m * FF_T;
+  // from the block's centre to its centre of mass
+  float sx = ax + txx * dx + txy * dy + txz * dz, sy = ay + txy * dx + tyy * dy + tyz * dz,
+        sz = az + txz * dx + tyz * dy + tzz * dz;
+  farA[t] = make_float4(sx, sy, sz, p - (ax * dx + ay * dy + az * dz));
+  farT0[t] = make_float4(txx, tyy, tzz, txy);
+  farT1[t] = make_float4(txz, tyz, 0.0f, 0.0f);
 }
 // Training target from the exact far field, in the output's units; the mask
 // marks occupied blocks.
 __global__ void k_ff_target(const float4* __restrict__ coarse, const float4* __restrict__ farA,
                             const float4* __restrict__ farT0, const float4* __restrict__ farT1,
-                            const float4* __restrict__ mono, int inst, float* tgt, float* mask) {
+                            const float4* __restrict__ mono, const Box* __restrict__ bxp, int which, int inst,
+                            float* tgt, float* mask) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= 4096) return;
   int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
   int t = coarse_texel(x, y, z);
   float* o = tgt + ((size_t)inst * 4096 + v) * 16;
   float m = fmaxf(mono[0].w, 1e-20f);
-  bool occ = coarse[t].w > 0.0f;
+  float4 c = coarse[t];
+  bool occ = c.w > 0.0f;
   float4 A = farA[t], T0 = farT0[t], T1 = farT1[t];
-  float vals[10] = { A.x / (m * FF_A), A.y / (m * FF_A), A.z / (m * FF_A), A.w / (m * FF_P), T0.x / (m * FF_T),
+  // the exact far field at the centre of mass, taken back to the block's centre
+  float dx = 0, dy = 0, dz = 0;
+  if (occ) {
+    const Box bx = *bxp;
+    block_offset(c, x, y, z, which ? bx.LB : bx.LA, dx, dy, dz);
+  }
+  float ax = A.x - (T0.x * dx + T0.w * dy + T1.x * dz), ay = A.y - (T0.w * dx + T0.y * dy + T1.y * dz),
+        az = A.z - (T1.x * dx + T1.y * dy + T0.z * dz);
+  float p = A.w + (ax * dx + ay * dy + az * dz);
+  float vals[10] = { ax / (m * FF_A), ay / (m * FF_A), az / (m * FF_A), p / (m * FF_P), T0.x / (m * FF_T),
                      T0.y / (m * FF_T), T0.z / (m * FF_T), T0.w / (m * FF_T), T1.x / (m * FF_T), T1.y / (m * FF_T) };
   for (int k = 0; k < 16; k++) o[k] = occ && k < 10 ? vals[k] : 0.0f;
   mask[(size_t)inst * 4096 + v] = occ ? 1.0f : 0.0f;
@@ -422,6 +457,7 @@ struct FFNet {
   size_t nw = 0;
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   int adamT = 0;
+  float slope = 0.25f;          // the leaky ReLU's negative slope (1: the network is linear)
   int cap = 0, count = 0;
   uint16_t* dataIn = nullptr;   // [cap][4096][16]
   float* dataTgt = nullptr;     // [cap][4096][16]
@@ -482,18 +518,18 @@ static void acts_free(Acts& a) {
 
 static inline unsigned grid_of(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-static void conv(const Layer& L, const float* bias, const uint16_t* in, int B, int S, uint16_t* out, float* outf,
-                 float* pre, const int* gate, hipStream_t s) {
+static void conv(const Layer& L, float slope, const float* bias, const uint16_t* in, int B, int S, uint16_t* out,
+                 float* outf, float* pre, const int* gate, hipStream_t s) {
   size_t M = (size_t)B * S * S * S;
   unsigned blocks = grid_of((M + 15) / 16 * 64);
   if (L.cin == 16)
-    hipLaunchKernelGGL((k_conv<16, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, L.act, out, outf, pre,
+    hipLaunchKernelGGL((k_conv<16, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, slope, out, outf, pre,
                        gate);
   else if (L.cout == 32)
-    hipLaunchKernelGGL((k_conv<32, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, L.act, out, outf, pre,
+    hipLaunchKernelGGL((k_conv<32, 32>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, slope, out, outf, pre,
                        gate);
   else
-    hipLaunchKernelGGL((k_conv<32, 16>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, L.act, out, outf, pre,
+    hipLaunchKernelGGL((k_conv<32, 16>), dim3(blocks), dim3(256), 0, s, in, B, S, L.fwd, bias, slope, out, outf, pre,
                        gate);
 }
 // the data gradient of a layer: g (bf16, at its output) through the flipped,
@@ -502,13 +538,13 @@ static void conv_dgrad(const Layer& L, const uint16_t* g, int B, int S, float* g
   size_t M = (size_t)B * S * S * S;
   unsigned blocks = grid_of((M + 15) / 16 * 64);
   if (L.cout == 16)   // the output layer: 16 gradient channels in, 32 out
-    hipLaunchKernelGGL((k_conv<16, 32>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 0, nullptr, gin,
+    hipLaunchKernelGGL((k_conv<16, 32>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 1.0f, nullptr, gin,
                        nullptr, nullptr);
   else if (L.cin == 32)
-    hipLaunchKernelGGL((k_conv<32, 32>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 0, nullptr, gin,
+    hipLaunchKernelGGL((k_conv<32, 32>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 1.0f, nullptr, gin,
                        nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_conv<32, 16>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 0, nullptr, gin,
+    hipLaunchKernelGGL((k_conv<32, 16>), dim3(blocks), dim3(256), 0, s, g, B, S, L.bwd, nullptr, 1.0f, nullptr, gin,
                        nullptr, nullptr);
 }
 static void conv_wgrad(const Layer& L, const uint16_t* in, const uint16_t* g, int B, int S, float* dW, float* db,
@@ -529,7 +565,7 @@ static void forward(const FFNet* n, Acts& a, int B, bool training, const int* ga
   const float* w = n->w;
   auto pre = [&](int l) { return training ? a.pre[l] : nullptr; };
   auto C = [&](int l, const uint16_t* in, uint16_t* out) {
-    conv(n->L[l], w + n->L[l].b, in, B, LAYER_S[l], out, nullptr, pre(l), gate, s);
+    conv(n->L[l], n->L[l].act ? n->slope : 1.0f, w + n->L[l].b, in, B, LAYER_S[l], out, nullptr, pre(l), gate, s);
   };
   auto pool = [&](const uint16_t* in, int S, uint16_t* out) {
     hipLaunchKernelGGL(k_pool, dim3(grid_of((size_t)B * (S / 2) * (S / 2) * (S / 2) * 32)), dim3(256), 0, s, in, B, S,
@@ -546,7 +582,7 @@ static void forward(const FFNet* n, Acts& a, int B, bool training, const int* ga
   up(a.bot, a.e2, 4, a.u2); C(8, a.u2, a.d2);
   up(a.d2, a.e1, 8, a.u1); C(9, a.u1, a.d1);
   up(a.d1, a.e0, 16, a.u0); C(10, a.u0, a.d0);
-  conv(n->L[11], w + n->L[11].b, a.d0, B, 16, nullptr, a.y, pre(11), gate, s);
+  conv(n->L[11], 1.0f, w + n->L[11].b, a.d0, B, 16, nullptr, a.y, pre(11), gate, s);
 }
 
 // backward from a.gy (f32, at the output) to the weight gradients n->g
@@ -554,7 +590,8 @@ static void backward(FFNet* n, Acts& a, int B, hipStream_t s) {
   float* g = n->g;
   auto actb = [&](const float* gout, int l) {   // gradient at layer l's output through its activation, to bf16
     size_t cnt = (size_t)B * LAYER_S[l] * LAYER_S[l] * LAYER_S[l] * n->L[l].cout;
-    hipLaunchKernelGGL(k_act_bwd, dim3(grid_of(cnt)), dim3(256), 0, s, gout, a.pre[l], (int)cnt, n->L[l].act, a.gbf);
+    hipLaunchKernelGGL(k_act_bwd, dim3(grid_of(cnt)), dim3(256), 0, s, gout, a.pre[l], (int)cnt,
+                       n->L[l].act ? n->slope : 1.0f, a.gbf);
   };
   auto W = [&](int l, const uint16_t* in) {
     conv_wgrad(n->L[l], in, a.gbf, B, LAYER_S[l], g + n->L[l].w, g + n->L[l].b, s);
@@ -677,7 +714,8 @@ void ff_infer(const FFNet* n, FFWork* wk, const float4* coarse, const float4* mo
   Acts& a = wk->a;
   hipLaunchKernelGGL(k_ff_in, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, 0, a.x0, a.scale, gate);
   forward(n, a, 1, false, gate, s);
-  hipLaunchKernelGGL(k_ff_out, dim3(16), dim3(256), 0, s, a.y, coarse, a.scale, 0, farA, farT0, farT1, gate);
+  hipLaunchKernelGGL(k_ff_out, dim3(16), dim3(256), 0, s, a.y, coarse, a.scale, bx, which, 0, farA, farT0, farT1,
+                     gate);
 }
 
 // one training sample from a box's coarse mesh and its exact far field
@@ -686,8 +724,8 @@ int ff_record(FFNet* n, const float4* coarse, const float4* mono, const Box* bx,
   if (n->count >= n->cap) return -1;
   int k = n->count++;
   hipLaunchKernelGGL(k_ff_in, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, k, n->dataIn, n->dataScale, nullptr);
-  hipLaunchKernelGGL(k_ff_target, dim3(16), dim3(256), 0, s, coarse, farA, farT0, farT1, mono, k, n->dataTgt,
-                     n->dataMask);
+  hipLaunchKernelGGL(k_ff_target, dim3(16), dim3(256), 0, s, coarse, farA, farT0, farT1, mono, bx, which, k,
+                     n->dataTgt, n->dataMask);
   return k;
 }
 
@@ -875,6 +913,12 @@ int cc_ff_set_weights(cc_ff* f, const float* w, int64_t n) {
   if (e == hipSuccess) e = hipStreamSynchronize(f->n->s);
   if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
   f->n->adamT = 0;
+  return 0;
+}
+
+int cc_ff_set_slope(cc_ff* f, double slope) {
+  if (!f || !(slope >= 0.0 && slope <= 1.0)) FFFAIL(f, CC_EINVAL, "the slope must be in [0, 1]");
+  f->n->slope = (float)slope;
   return 0;
 }
 

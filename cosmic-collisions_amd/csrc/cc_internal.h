@@ -96,6 +96,7 @@ struct GridBuf {
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
+void launch_copy4(const float4* src, float4* dst, int n, hipStream_t s);
 void launch_put_box(const Box& b, Box* dst, hipStream_t s);
 void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s);
 void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2,

@@ -309,6 +309,16 @@ void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, in
   hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.occ, m.occN);
 }
 
+// A plain copy of n float4 (read once, written once): the calibration of the
+// HBM traffic counters against a known byte count.
+__global__ void __launch_bounds__(256) k_copy4(const float4* __restrict__ src, float4* dst, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+void launch_copy4(const float4* src, float4* dst, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy4, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n);
+}
+
 // The mesh boxes live in device memory so that a captured step cycle reads
 // the box of the cycle it is replayed for: k_put_box writes one (eager
 // steps), k_box_next takes the next of a call's schedule (graph replays).
@@ -646,11 +656,21 @@ __device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, 
   a.flow += (Vj.w - Ti) * (2.0f * ci * cj / (ci + cj));
 }
 
+// The seat prefilter (seatPos and the reach test, src/glsl-sim.js:109,
+// :426-427): the seat's recorded place, (q + 0.5)/15 of a cell into it, within
+// reach (cell + 0.6a, src/nbody.js:41, :297) of p.  It only prunes: the 0.6a
+// of slack over the link distance is the reference's bound on how far a
+// particle sits from its seat's place, so a seat at the edge of reach holds a
+// particle beyond the link test, and the rounding of the place cannot change
+// which pairs are summed.  The place is one FMA per axis.
+__device__ __forceinline__ float seat_place(uint32_t q, float lo, const SimU& u) {
+  return fmaf((float)q, u.cell * (1.0f / 15.0f), lo + u.cell * (0.5f / 15.0f));
+}
 __device__ __forceinline__ bool seat_passes(uint32_t sv, int me, float lox, float loy, float loz, float px, float py,
-                                            float pz, const float* seatpos, const SimU& u) {
+                                            float pz, const SimU& u) {
   int j = (int)(sv & 0x3FFFFu);
-  float dqx = lox + seatpos[(sv >> 18) & 15u] * u.cell - px, dqy = loy + seatpos[(sv >> 22) & 15u] * u.cell - py,
-        dqz = loz + seatpos[(sv >> 26) & 15u] * u.cell - pz;
+  float dqx = seat_place((sv >> 18) & 15u, lox, u) - px, dqy = seat_place((sv >> 22) & 15u, loy, u) - py,
+        dqz = seat_place((sv >> 26) & 15u, loz, u) - pz;
   return j != me && !(dot3(dqx, dqy, dqz, dqx, dqy, dqz) > u.reach2);
 }
 
@@ -664,9 +684,6 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
                                                    const float4* __restrict__ ppForce, float4* oPos, float4* oVel,
                                                    float4* oAux, float* oQ) {
   __shared__ int cand[CAND_CAP][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts
-  __shared__ float seatpos[16];               // where a seat's particle sat, in cells: (q + 0.5) / 15
-  if (threadIdx.x < 16) seatpos[threadIdx.x] = ((float)threadIdx.x + 0.5f) / 15.0f;
-  __syncthreads();
   int lane = threadIdx.x;
   int me = xcd_block() * blockDim.x + lane;
   if (me >= u.N) return;
@@ -677,47 +694,84 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   Contact acc = { 0, 0, 0, 0, 0, 0 };
   int cx = (int)floorf(px * u.invCell), cy = (int)floorf(py * u.invCell), cz = (int)floorf(pz * u.invCell);
   float c0x = (float)cx * u.cell, c0y = (float)cy * u.cell, c0z = (float)cz * u.cell;
-  // A: candidates in order
+  // A: candidates in order.  A z-plane's nine cells have their seat loads
+  // issued together (each under the cell's cull test), then are walked in order.
   int n = 0;
   bool over = false;
-  for (int dz = -1; dz <= 1; dz++) for (int dy = -1; dy <= 1; dy++) for (int dx = -1; dx <= 1; dx++) {
-    float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
-    float qx = gclamp(px, lox, lox + u.cell) - px, qy = gclamp(py, loy, loy + u.cell) - py,
-          qz = gclamp(pz, loz, loz + u.cell) - pz;
-    if (dot3(qx, qy, qz, qx, qy, qz) > u.link2) continue;
-    int h = hash_cell(u.grid, cx + dx, cy + dy, cz + dz);
-    const uint4* s4 = reinterpret_cast<const uint4*>(seats + (size_t)h * SLOTS);
-    uint4 ea = s4[0], ob = s4[1];
-    uint32_t sv8[8] = { ea.x, ob.x, ea.y, ob.y, ea.z, ob.z, ea.w, ob.w };
-    bool open = true;
+  for (int dz = -1; dz <= 1; dz++) {
+    uint4 sa[9], sb[9];
+    bool ok[9];
 #pragma unroll
-    for (int k = 0; k < SLOTS; k++) {
-      uint32_t sv = sv8[k];
-      open = open && !(sv & 0x80000000u);
-      if (open && seat_passes(sv, me, lox, loy, loz, px, py, pz, seatpos, u)) {
-        if (n < CAND_CAP) cand[n][lane] = (int)(sv & 0x3FFFFu);
-        else over = true;
-        n++;
+    for (int i = 0; i < 9; i++) {
+      int dy = i / 3 - 1, dx = i % 3 - 1;
+      float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
+      float qx = gclamp(px, lox, lox + u.cell) - px, qy = gclamp(py, loy, loy + u.cell) - py,
+            qz = gclamp(pz, loz, loz + u.cell) - pz;
+      ok[i] = !(dot3(qx, qy, qz, qx, qy, qz) > u.link2);
+      sa[i] = make_uint4(0x80000000u, 0, 0, 0);
+      sb[i] = sa[i];
+      if (ok[i]) {
+        const uint4* s4 = reinterpret_cast<const uint4*>(seats + (size_t)hash_cell(u.grid, cx + dx, cy + dy, cz + dz) *
+                                                                   SLOTS);
+        sa[i] = s4[0];
+        sb[i] = s4[1];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      if (!ok[i]) continue;
+      int dy = i / 3 - 1, dx = i % 3 - 1;
+      float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
+      uint32_t sv8[8] = { sa[i].x, sb[i].x, sa[i].y, sb[i].y, sa[i].z, sb[i].z, sa[i].w, sb[i].w };
+      bool open = true;
+#pragma unroll
+      for (int k = 0; k < SLOTS; k++) {
+        uint32_t sv = sv8[k];
+        open = open && !(sv & 0x80000000u);
+        if (open && seat_passes(sv, me, lox, loy, loz, px, py, pz, u)) {
+          if (n < CAND_CAP) cand[n][lane] = (int)(sv & 0x3FFFFu);
+          else over = true;
+          n++;
+        }
       }
     }
   }
   if (!over) {
-    // B: linked pairs counted, touching ones kept (in place, in order)
+    // B: linked pairs counted, touching ones kept (in place, in order); positions fetched eight at a time
     int nt = 0;
-    for (int t = 0; t < n; t++) {
-      int j = cand[t][lane];
-      float4 Pj = pos[j];
-      float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
-      float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
-      if (r2 < u.link2) {
-        acc.nb += 1.0f;
-        if (r2 < touch2r) cand[nt++][lane] = j;
+    for (int t0 = 0; t0 < n; t0 += 8) {
+      float4 Pj[8];
+      int jj[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        jj[k] = t0 + k < n ? cand[t0 + k][lane] : me;
+        Pj[k] = pos[jj[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (t0 + k >= n) break;
+        float ddx = Pj[k].x - px, ddy = Pj[k].y - py, ddz = Pj[k].z - pz;
+        float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
+        if (r2 < u.link2) {
+          acc.nb += 1.0f;
+          if (r2 < touch2r) cand[nt++][lane] = jj[k];
+        }
       }
     }
-    // C: the contacts, in order
-    for (int t = 0; t < nt; t++) {
-      int j = cand[t][lane];
-      contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, pos[j], vel[j], mat[j], acc);
+    // C: the contacts, in order; their data fetched four at a time
+    for (int t0 = 0; t0 < nt; t0 += 4) {
+      float4 Pj[4], Vj[4];
+      float rj[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int j = t0 + k < nt ? cand[t0 + k][lane] : me;
+        Pj[k] = pos[j]; Vj[k] = vel[j]; rj[k] = mat[j];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (t0 + k >= nt) break;
+        contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, Pj[k], Vj[k], rj[k], acc);
+      }
     }
   } else {
     // the direct path: every cell and seat in order, as the reference walks them
@@ -730,7 +784,7 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
       for (int k = 0; k < SLOTS; k++) {
         uint32_t sv = seats[(size_t)h * SLOTS + ((k & 1) ? 4 : 0) + (k >> 1)];
         if (sv & 0x80000000u) break;
-        if (!seat_passes(sv, me, lox, loy, loz, px, py, pz, seatpos, u)) continue;
+        if (!seat_passes(sv, me, lox, loy, loz, px, py, pz, u)) continue;
         int j = (int)(sv & 0x3FFFFu);
         float4 Pj = pos[j];
         float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;

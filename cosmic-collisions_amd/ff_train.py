@@ -39,9 +39,10 @@ def train(ff, iters, batch=16, lr=2e-3, seed=1):
     return np.concatenate([l1, l2])
 
 
-def build(configs, state_root, cycles, iters, capacity=512, reuse=True, log=print):
+def build(configs, state_root, cycles, iters, capacity=512, reuse=True, slope=0.25, log=print):
     """A FarField trained on the named configurations (runs.py): (ff, report)."""
     ff = ccamd.FarField(capacity=capacity)
+    ff.set_slope(slope)
     t0 = time.perf_counter()
     train_set, held = [], []
     for name in configs:
@@ -64,7 +65,7 @@ def build(configs, state_root, cycles, iters, capacity=512, reuse=True, log=prin
     for x, t, m in held:
         ff.push(x, t, m)
     held_err = ff.eval(len(train_set), len(held)) if held else (float("nan"), float("nan"))
-    rep = {"samples": len(train_set), "held_out": len(held), "iters": iters, "collect_s": t1 - t0, "train_s": t2 - t1,
+    rep = {"slope": slope, "samples": len(train_set), "held_out": len(held), "iters": iters, "collect_s": t1 - t0, "train_s": t2 - t1,
            "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-20:])),
            "train_rel_acc_err_init": before, "train_rel_acc_err": train_err[0], "train_rel_err_all": train_err[1],
            "held_rel_acc_err": held_err[0], "held_rel_err_all": held_err[1]}
@@ -78,9 +79,11 @@ def main():
     ap.add_argument("--configs", default="THEIA,TWINS,SHATTER")
     ap.add_argument("--cycles", type=int, default=48)
     ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--slope", type=float, default=0.25, help="the leaky ReLU's negative slope (1: linear)")
     ap.add_argument("--state-root", default=tempfile.gettempdir())
     args = ap.parse_args()
-    ff, rep = build(args.configs.split(","), args.state_root, args.cycles, args.iters, log=lambda m: print(m, flush=True))
+    ff, rep = build(args.configs.split(","), args.state_root, args.cycles, args.iters, slope=args.slope,
+                    log=lambda m: print(m, flush=True))
     np.save(args.out, ff.weights())
     ff.close()
     return 0

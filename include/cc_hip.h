@@ -112,8 +112,9 @@ int cc_set_graphs(cc_ctx* ctx, int on);
  * measurement: 0 the step pass (no settle, kick or gravity impulse: it
  * advances the state), 1 the seats, 2 the mesh (exact far field), 3 the
  * pairwise pass, 4 the mesh as on a mesh step between judges (the learned
- * far field if it is standing in), 5 the mesh as on a judge step;
- * *ms = device time per pass. */
+ * far field if it is standing in), 5 the mesh as on a judge step, 6 a plain
+ * copy of the positions (N x 16 B each way: the calibration of the HBM
+ * traffic counters); *ms = device time per pass. */
 int cc_time_pass(cc_ctx* ctx, int pass, int reps, double* ms);
 int cc_get_step_no(cc_ctx* ctx, int* stepNo);
 int cc_set_step_no(cc_ctx* ctx, int stepNo);
@@ -145,8 +146,9 @@ int cc_last_step_ms(cc_ctx* ctx, double* ms);
  * operator on the 16^3 coarse mesh -- 4 levels, 32 channels, 3x3x3 kernels,
  * BF16 weights on the matrix cores with FP32 accumulation -- that stands in
  * for the block-to-block far field (FAR_FS, src/glsl-sim.js:263-295): coarse
- * masses and centres of mass in, each block's far acceleration, potential
- * and tidal tensor out.  Trained on the GPU by the library's own forward and
+ * masses and dipoles in, each block's far acceleration, potential and tidal
+ * tensor out (at the block's centre, carried to its centre of mass by the
+ * tidal tensor).  Trained on the GPU by the library's own forward and
  * backward kernels on samples recorded from the exact far field.
  * Weights: FP32, per layer [27 taps][cin][cout] then [cout] biases, layers in
  * order (cc_ff_weight_count floats). */
@@ -157,6 +159,8 @@ const char* cc_ff_last_error(cc_ff* f);
 int cc_ff_weight_count(int64_t* n);
 int cc_ff_get_weights(cc_ff* f, float* w, int64_t n);
 int cc_ff_set_weights(cc_ff* f, const float* w, int64_t n);
+/* the leaky ReLU's negative slope, 0.25 by default; 1 makes the network a linear operator of its input */
+int cc_ff_set_slope(cc_ff* f, double slope);
 /* a sample from box `box` of ctx's last mesh pass, whose far field must be the exact one (cc_mesh) */
 int cc_ff_record(cc_ff* f, cc_ctx* ctx, int box);
 /* samples in the network's own units: in[4096][16] (channels 0-3 mass, centre-of-mass offset), target[4096][16]

@@ -14,7 +14,7 @@ import numpy as np
 
 LAYERS = [(16, 32, 1)] + [(32, 32, 1)] * 10 + [(32, 16, 0)]
 LAYER_S = [16, 16, 8, 8, 4, 4, 2, 2, 4, 8, 16, 16]
-SLOPE = 0.25
+SLOPE = 0.25   # the library's default negative slope
 
 
 def weight_count():
