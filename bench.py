@@ -131,6 +131,20 @@ def learned_leg(state_root, steps, warmup, reuse, cycles, iters):
     return out
 
 
+def runto_leg(steps, every=360):
+    """The JavaScript engine driven as the reference's page drives a run: full
+    steps with a synchronous analysis (groups, books, boxes, loose list, a
+    Morton sort every eighth) every 360 steps."""
+    cmd = ["node", os.path.join(PKG, "js", "run_timed.js"), json.dumps(runs.TWINS), str(steps), str(every)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("run_timed.js failed (exit %d):\n%s" % (r.returncode, r.stderr[-3000:]))
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["workload"] = "plain twins through the JavaScript engine (js/nbody.js over the N-API addon), an analysis " \
+                      "every %d steps as index.html's runTo drives it" % every
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +156,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="instances in the batched-sweep leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=240)
     ap.add_argument("--batch-dir", default=os.path.join(tempfile.gettempdir(), "cc_batch_state"))
+    ap.add_argument("--runto", type=int, default=3600, help="steps of the engine-driven leg with analyses (0: skip)")
     ap.add_argument("--learned", type=int, default=1, help="the learned far field leg (0: skip it)")
     ap.add_argument("--learned-steps", type=int, default=1000)
     ap.add_argument("--ff-cycles", type=int, default=48)
@@ -187,6 +202,8 @@ def main():
         P, V, _, Q, M = R.ctx.download()
         out["cpu_baseline"] = cpu_baseline(twins_like(run), P, V, M, Q)
     R.close()
+    if args.runto > 0:
+        out["engine_runto"] = runto_leg(args.runto)
     if args.batch > 0:
         out["batched"] = batch_leg(args.batch_dir, args.batch, args.batch_steps, 24, args.reuse_state)
     if args.learned > 0:

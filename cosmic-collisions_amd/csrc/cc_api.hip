@@ -117,8 +117,8 @@ static int check_params(cc_ctx* c, const cc_params* p) {
   if (!p) FAIL(c, CC_EINVAL, "params is NULL");
   if (p->N <= 0 || p->W <= 0 || p->H <= 0 || (long long)p->W * p->H != p->N)
     FAIL(c, CC_EINVAL, "N must equal W*H (N=%d W=%d H=%d)", p->N, p->W, p->H);
-  if (p->N > (1 << 18))
-    FAIL(c, CC_EINVAL, "N=%d exceeds the seat word's 18-bit particle index (262144)", p->N);
+  if (p->N > (1 << 19))
+    FAIL(c, CC_EINVAL, "N=%d exceeds the seat word's 19-bit particle index (524288)", p->N);
   if (p->G != 64 && p->G != 128) FAIL(c, CC_EINVAL, "contact grid G must be 64 or 128, got %d", p->G);
   if (p->meshEvery <= 0) FAIL(c, CC_EINVAL, "meshEvery must be positive");
   if (!(p->m > 0) || !(p->a > 0) || !(p->cell > 0) || !(p->dt > 0)) FAIL(c, CC_EINVAL, "m, a, cell, dt must be > 0");
@@ -179,6 +179,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
   c->p = *p;
   c->N = p->N;
   grid_make(p->G, c->grid);
+  c->grid.IB = p->N > (1 << 18) ? 19 : 18;
   c->cells = (size_t)c->grid.AW * c->grid.AH;
   hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
   if (e != hipSuccess) { g_err = std::string("hipStreamCreate: ") + hipGetErrorString(e); delete c; return CC_EDEVICE; }
@@ -193,7 +194,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     ALLOC(c->gb.cnt, c->cells * 4);
     ALLOC(c->gb.bucket, c->cells * BUCKET_CAP * 4);
     ALLOC(c->gb.occ[0], N * 4); ALLOC(c->gb.occ[1], N * 4);
-    ALLOC(c->gb.occN, 2 * 4);
+    ALLOC(c->gb.segN, 2 * ((N + 255) / 256) * 4);
     ALLOC(c->gb.ovf, N * 8); ALLOC(c->gb.ovfN, 4);
     ALLOC(c->gb.word, N * 4);
     for (int b = 0; b < 2; b++) {
@@ -217,7 +218,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     // the empty state: seats empty, counters and accumulators zero, meshes zero
     launch_fill_u32(c->gb.seats, c->cells * SLOTS, EMPTY_SEAT, c->s);
     HIPCHK(c, hipMemsetAsync(c->gb.cnt, 0, c->cells * 4, c->s));
-    HIPCHK(c, hipMemsetAsync(c->gb.occN, 0, 8, c->s));
+    HIPCHK(c, hipMemsetAsync(c->gb.segN, 0, 2 * ((N + 255) / 256) * 4, c->s));
     HIPCHK(c, hipMemsetAsync(c->gb.ovfN, 0, 4, c->s));
     for (int b = 0; b < 2; b++) {
       MeshBuf& m = c->mesh[b];

@@ -41,6 +41,11 @@ constexpr double SCR_SCALE = 16777216.0;          // 2^24
 
 struct Grid {
   int G, GShift, GMask, SX, SXShift, SXMask, AW, AH;
+  // bits of the particle index in a seat word: 18 as the reference packs it
+  // (idx | q << 18 | q << 22 | q << 26 | 1 << 30, src/glsl-sim.js:103-110,
+  // :143-144), 19 above 262,144 particles, where the place fields move up one
+  // bit and take bit 30 (no reference counterpart: its index stops at 2^18)
+  int IB;
 };
 
 struct Box {            // the two mesh boxes as the shaders see them
@@ -84,8 +89,8 @@ struct GridBuf {
   uint32_t* seats;           // [cells*8]
   int* cnt;                  // [cells]
   int* bucket;               // [cells*BUCKET_CAP]
-  int* occ[2];               // occupied cells, two lists used in turn
-  int* occN;                 // [2]
+  int* occ[2];               // occupied cells, two lists used in turn, in 256-entry segments (one per work-group)
+  int* segN;                 // [2][blocks]: entries of each segment
   int2* ovf;                 // (cell, particle) past a cell's bucket
   int* ovfN;
   uint32_t* word;            // [N] seat word of each particle

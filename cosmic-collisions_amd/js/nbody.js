@@ -167,7 +167,7 @@ function buildRun(opt) {
 function createEngine(opts) {
   opts = opts || {};
   const addon = require(opts.addon || path.join(__dirname, 'cc_addon.node'));
-  let ctx = null, sim = null, report = null, MESH_EVERY = 8, analyzedAt = 0, analyses = 0, gen = 0;
+  let ctx = null, sim = null, report = null, MESH_EVERY = 8, analyzedAt = 0, analyses = 0, gen = 0, lookMs = null;
   const PP_TAB = pairTable(64, PP_CUT);
   const pp = { count: 0, loose: 0, list: new Int32Array(0) };
   const taps = { sim: [], report: [] };
@@ -325,9 +325,14 @@ function createEngine(opts) {
     sim.gen = ++gen;
     analyzedAt = sim.stepNo;
     const sort = sim.phase === 'full' && analyses++ % 8 === 0;
+    const ms = () => Number(process.hrtime.bigint()) / 1e6;
+    const t0 = ms();
     const D = passDiag();
+    const t1 = ms();
     const st = readState();
+    const t2 = ms();
     const r = analyse(sim.N, sim.m, sim.a, st.pos, st.vel, st.mat, D, sim.k, 2 * sim.a, PP_CAP);
+    const t3 = ms();
     r.t = sim.t;
     report = r; sim.com1 = r.com;
     placeBoxes(r);
@@ -347,6 +352,7 @@ function createEngine(opts) {
       r.perm = perm;
     }
     setPP(list, r.pp.loose);
+    lookMs = { diag: t1 - t0, read: t2 - t1, fof: t3 - t2, rest: ms() - t3, sort };
     emit('report', r);
     return r;
   }
@@ -366,6 +372,7 @@ function createEngine(opts) {
     onSim(f) { taps.sim.push(f); }, onReport(f) { taps.report.push(f); },
     get sim() { return sim; }, get report() { return report; }, get pending() { return null; },
     get analyzedAt() { return analyzedAt; }, get pp() { return pp; }, get ctx() { return ctx; }, get addon() { return addon; },
+    get lookMs() { return lookMs; },   // the last analysis's phases, ms: seats+mesh+pairwise+DIAG, read-back, fof, the rest
     get state() { return { analyzing: false, pending: false, worker: false, analyses, analyzedAt, stepNo: sim && sim.stepNo }; },
     set meshEvery(n) { MESH_EVERY = n; if (ctx) addon.setParams(ctx, params()); },
     set asyncRead(v) {},

@@ -42,6 +42,7 @@ void cco_grid_make(int G, cco_grid* g) {   /* src/gl.js:73-83 */
   g->SXMask = g->SX - 1;
   g->AW = g->SX * G;
   g->AH = (G / g->SX) * G;
+  g->IB = 18;
 }
 
 void cco_hash_at(const cco_grid* g, int cx, int cy, int cz, int* ax, int* ay) {
@@ -53,14 +54,15 @@ void cco_hash_at(const cco_grid* g, int cx, int cy, int cz, int* ax, int* ay) {
   *ay = shl(cz >> g->SXShift, g->GShift) + cy;
 }
 
-static inline uint32_t seat_word(uint32_t me, const float* gq) {
+static inline uint32_t seat_word(uint32_t me, const float* gq, int ib) {
   /* uvec3(clamp(fract(g) * 15.0, 0.0, 14.0)) (src/glsl-sim.js:143-144) */
   uint32_t q[3];
   for (int a = 0; a < 3; a++) {
     float fr = gq[a] - floorf(gq[a]);
     q[a] = (uint32_t)clampf(fr * 15.0f, 0.0f, 14.0f);
   }
-  return me | (q[0] << 18) | (q[1] << 22) | (q[2] << 26) | 0x40000000u;
+  /* 19-bit indices (no reference counterpart) move the place fields up one bit and drop bit 30 */
+  return me | (q[0] << ib) | (q[1] << (ib + 4)) | (q[2] << (ib + 8)) | (ib == 18 ? 0x40000000u : 0u);
 }
 
 /* The depth-peeled seat lists: seat k of a hashed cell holds the k-th lowest
@@ -80,7 +82,7 @@ void cco_slots(const float* pos, int N, float invCell, const cco_grid* g, uint32
     int k = cnt[t];
     if (k >= 8) continue;
     cnt[t] = (unsigned char)(k + 1);
-    uint32_t w = seat_word((uint32_t)i, gq);
+    uint32_t w = seat_word((uint32_t)i, gq, g->IB);
     if (k & 1) seatB[t * 4 + (k >> 1)] = w; else seatA[t * 4 + (k >> 1)] = w;
   }
   free(cnt);
@@ -381,10 +383,11 @@ void cco_sim(const cco_sim_u* u, const float* pos, const float* vel, const float
       for (int k = 0; k < 8; k++) {
         uint32_t sv = seat_of(k, sa, sb);
         if (sv & 0x80000000u) break;
-        int j = (int)(sv & 0x3FFFFu);
+        int j = (int)(sv & ((1u << u->grid.IB) - 1u));
         if (j == me) continue;
-        float sp[3] = { ((float)((sv >> 18) & 15u) + 0.5f) / 15.0f, ((float)((sv >> 22) & 15u) + 0.5f) / 15.0f,
-                        ((float)((sv >> 26) & 15u) + 0.5f) / 15.0f };
+        int B = u->grid.IB;
+        float sp[3] = { ((float)((sv >> B) & 15u) + 0.5f) / 15.0f, ((float)((sv >> (B + 4)) & 15u) + 0.5f) / 15.0f,
+                        ((float)((sv >> (B + 8)) & 15u) + 0.5f) / 15.0f };
         float dq[3];
         for (int a = 0; a < 3; a++) dq[a] = lo[a] + sp[a] * u->cell - p[a];
         if (dot3(dq, dq) > u->reach2) continue;
@@ -578,7 +581,7 @@ void cco_diag(const cco_sim_u* u, const float* pos, const float* q, const uint32
       uint32_t sv = seat_of(k, seatA + t, seatB + t);
       if (sv & 0x80000000u) break;
       taken += 1.0f;
-      if ((int)(sv & 0x3FFFFu) == me) seated = 1.0f;
+      if ((int)(sv & ((1u << u->grid.IB) - 1u)) == me) seated = 1.0f;
     }
     float phi = 0.0f, off = 0.0f, g[3];
     int box = cco_box_of(&u->box, p, g);

@@ -24,6 +24,7 @@ extern "C" {
 
 typedef struct {
   int G, GShift, GMask, SX, SXShift, SXMask, AW, AH;
+  int IB;                 /* seat word index bits: 18 (the reference's); 19 above 262,144 particles */
 } cco_grid;
 
 typedef struct {          /* the two mesh boxes, as the shaders see them */

@@ -20,7 +20,7 @@ i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 
 
 class Grid(C.Structure):
-    _fields_ = [(n, C.c_int) for n in ("G", "GShift", "GMask", "SX", "SXShift", "SXMask", "AW", "AH")]
+    _fields_ = [(n, C.c_int) for n in ("G", "GShift", "GMask", "SX", "SXShift", "SXMask", "AW", "AH", "IB")]
 
 
 class Box(C.Structure):
@@ -75,9 +75,10 @@ def c32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
-def grid(G):
+def grid(G, ib=18):
     g = Grid()
     lib().cco_grid_make(G, C.byref(g))
+    g.IB = ib
     return g
 
 

@@ -153,3 +153,62 @@ console.log(JSON.stringify({ frames: frames, phase: sim.phase, N: sim.N }));
     assert frozen.any() and np.array_equal(f4[frozen, :3], post[frozen, :3]), "the frozen shell moved"
     assert (post[:, 3] == 3).sum() >= (fx["b0_pos"].reshape(N, 4)[:, 3] == 4).sum(), "the impactor is melt"
     assert np.isfinite(f4).all()
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_crater_524k(tmp_path):
+    """The crater at its configuration's stated size, 524,288 grains: past the
+    reference's 262,144, so the seat words carry a 19-bit index (the place
+    fields one bit up).  Seats bit-exact and one settle step at 1e-5 against
+    the oracle with the same seat format."""
+    script = tmp_path / "b.js"
+    script.write_text("""
+const cr = require('%s/crater.js'); const fs = require('fs');
+const r = cr.buildCraterRun({ imp: 14, vimp: 20, angle: 60, sed: 3, dens: 2500, nIdx: 4, rest: 0.25, N: 524288 });
+for (const [k, a] of [['pos', r.pos], ['vel', r.vel], ['mat', r.sim.matR]]) fs.writeFileSync('%s/' + k, Buffer.from(a.buffer));
+const s = {}; for (const k of Object.keys(r.sim)) { const v = r.sim[k]; if (typeof v === 'number') s[k] = v; }
+console.log(JSON.stringify(s));
+""" % (JS, tmp_path))
+    print("crater 524k: build (node)", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "240", "node", str(script)], capture_output=True, text=True, cwd=JS)
+    assert r.returncode == 0, r.stderr[-3000:]
+    s = json.loads(r.stdout.strip().splitlines()[-1])
+    N = s["N"]
+    assert N == 524288 and s["W"] == 512 and s["G"] == 128
+    pos = np.fromfile(tmp_path / "pos", np.float32).reshape(N, 4)
+    vel = np.fromfile(tmp_path / "vel", np.float32).reshape(N, 4)
+    mat = np.fromfile(tmp_path / "mat", np.float32)
+    q = np.zeros(N, np.float32)
+    gfield = (0.0, -9.81 * 10 ** 2 / 1e4, 0.0)
+    ctx = ccamd.Context(ccamd.params(N=N, W=512, G=128, m=1.0, a=s["a"], cell=s["cell"], dt=s["dt"], k=s["k"], c=s["c"],
+                                     creep=s["creep"], damp=s["damp"], tK=s["tK"], kappa=s["kappa"], field=True,
+                                     gField=gfield, frozen=0.0, cpIron=1.0))
+    try:
+        ctx.upload(pos, vel, mat, q)
+        ctx.step_no = 0
+        print("crater 524k: one settle step", flush=True)
+        ctx.step(1, 0, ccamd.SimArgs())
+        a, b = ctx.read_seats()
+        P, V, A, Q, M = ctx.download()
+    finally:
+        ctx.close()
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    u = {"uSize": [512, N // 512], "uInvCell": [f32(1 / s["cell"])], "uCell": [f32(s["cell"])], "uDt": [f32(s["dt"])],
+         "uGDt": [f32(s["dt"])], "uTouch": [f32(2 * s["a"])], "uLink2": [f32(s["cell"] ** 2)],
+         "uReach2": [f32((s["cell"] + 0.6 * s["a"]) ** 2)], "uK": [f32(s["k"])], "uC": [f32(s["c"])], "uInvM": [1.0],
+         "uDamp": [f32(s["damp"])], "uCreep": [f32(s["creep"])], "uTk": [f32(s["tK"])], "uKappa": [f32(s["kappa"])],
+         "uCpIron": [1.0], "uSettle": [1.0], "uKick": [0.0], "uGMask": [127], "uGField": list(gfield), "uFrozen": [0.0]}
+    su = gu.sim_u(u, mode="field")
+    su.grid = oracle.grid(128, ib=19)
+    ra, rb = oracle.slots(pos, su.invCell, su.grid)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb), "seats"
+    seated = np.concatenate([ra.ravel(), rb.ravel()])
+    seated = seated[seated < 0x80000000] & ((1 << 19) - 1)
+    assert seated.max() >= (1 << 18), "indices past 2^18 seated"
+    z4, z1 = np.zeros((513, 512, 4), np.float32), np.zeros(4, np.float32)
+    oP, oV, oA, oQ = oracle.sim(su, pos, vel, mat, q, ra, rb, z4, z1, z4, z1, np.full(N, -1.0, np.float32),
+                                np.zeros((256, 128, 4), np.float32))
+    assert np.array_equal(A[:, 3], oA[:, 3]), "neighbour counts"
+    for mine, ref, what in ((V[:, :3], oV[:, :3], "velocity"), (P[:, :3], oP[:, :3], "position"), (Q, oQ, "heat")):
+        err = gu.rel_err(mine, ref)
+        assert err.max() < 1e-5, (what, err.max())
