@@ -37,11 +37,67 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_set_boxes", "cc_set_mono", "cc_slots", "cc_read_seats", "cc_deposit", "cc_coarse", "cc_cell", "cc_mesh",
            "cc_read_mesh", "cc_write_mesh", "cc_set_pp_table", "cc_set_pp", "cc_pp", "cc_read_pp", "cc_sim", "cc_step",
            "cc_step_n", "cc_set_graphs", "cc_time_pass", "cc_get_step_no", "cc_set_step_no", "cc_rigid", "cc_couple",
-           "cc_diag", "cc_morton", "cc_books", "cc_sync", "cc_last_step_ms", "cc_batch_create", "cc_batch_destroy", "cc_batch_last_error",
+           "cc_diag", "cc_morton", "cc_analyse", "cc_books", "cc_sync", "cc_last_step_ms", "cc_batch_create", "cc_batch_destroy", "cc_batch_last_error",
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
            "cc_set_far_field", "cc_ff_stats"]
+
+
+class AnalyseArgs(C.Structure):
+    _fields_ = [("m", C.c_double), ("a", C.c_double), ("spring", C.c_double), ("touch", C.c_double),
+                ("cap", C.c_int32), ("flags", C.c_int32), ("diag", C.POINTER(C.c_float))]
+
+
+REPLAY_UNIONS = 1   # CC_ANALYSE_REPLAY_UNIONS
+
+_V3 = C.c_double * 3
+
+
+class Second(C.Structure):
+    _fields_ = [("mass", C.c_double), ("dist", C.c_double), ("bound", C.c_int32), ("pad_", C.c_int32),
+                ("imp", C.c_double), ("iron", C.c_double), ("com", _V3), ("core", _V3), ("vel", _V3),
+                ("R", C.c_double), ("Redge", C.c_double), ("Tsurf", C.c_double), ("hotCom", _V3), ("hotR", C.c_double)]
+
+
+class Report(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in ("largest", "largestImp", "largestIron", "day", "tilt")] +
+                [("axis", _V3), ("bodyVel", _V3), ("orbit", C.c_double), ("escape", C.c_double),
+                 ("com", _V3), ("core", _V3), ("vel", _V3), ("R", C.c_double), ("Redge", C.c_double),
+                 ("Tsurf", C.c_double), ("hotCom", _V3), ("hotR", C.c_double), ("impMass", C.c_double),
+                 ("impCom", _V3), ("impRms", C.c_double)] +
+                [(n, C.c_double) for n in ("M", "iron", "drift", "L", "KE", "PE", "EL", "Q", "unseated", "full",
+                                           "jD", "moon")] +
+                [("hasSecond", C.c_int32), ("pad_", C.c_int32), ("second", Second), ("ppCount", C.c_int32),
+                 ("ppLoose", C.c_int32), ("groups", C.c_int32), ("largestN", C.c_int32), ("secondN", C.c_int32),
+                 ("replayed", C.c_int32), ("ms", C.c_double * 3)])
+
+
+def _plain(st):
+    """A ctypes structure as a dict of Python numbers and lists."""
+    out = {}
+    for name, _ in st._fields_:
+        if name.startswith("pad_"):
+            continue
+        v = getattr(st, name)
+        if isinstance(v, C.Structure):
+            v = _plain(v)
+        elif isinstance(v, C.Array):
+            v = list(v)
+        out[name] = v
+    return out
+
+
+def report_dict(rep, lst):
+    """fof()'s report (src/nbody.js:888-892) from a Report and the loose list."""
+    d = _plain(rep)
+    second = d.pop("second")
+    has = d.pop("hasSecond")
+    if has:
+        second["bound"] = bool(second["bound"])
+    d["second"] = second if has else None
+    d["pp"] = {"list": np.array(lst[:rep.ppCount], np.int32), "loose": rep.ppLoose}
+    return d
 
 
 class CoupleArgs(C.Structure):
@@ -80,6 +136,7 @@ def lib():
             "cc_rigid": ([P, dp, dp, dp, dp, dp, dp, C.c_double], C.c_int), "cc_diag": ([P, fp], C.c_int),
             "cc_couple": ([P, C.POINTER(CoupleArgs)], C.c_int),
             "cc_morton": ([P, dp, ip, ip], C.c_int), "cc_books": ([P, dp], C.c_int), "cc_sync": ([P], C.c_int),
+            "cc_analyse": ([P, C.POINTER(AnalyseArgs), C.POINTER(Report), ip], C.c_int),
             "cc_last_step_ms": ([P, dp], C.c_int),
             "cc_batch_create": ([C.POINTER(P), C.c_int, C.POINTER(P)], C.c_int), "cc_batch_destroy": ([P], C.c_int),
             "cc_batch_last_error": ([P], C.c_char_p),
@@ -317,6 +374,19 @@ class Context:
         into = np.empty(self.N, np.int32)
         self._chk(lib().cc_morton(self.h, _d3(core), _ptr(perm, C.c_int32), _ptr(into, C.c_int32)), "cc_morton")
         return perm, into
+
+    def analyse(self, m, a, spring, touch, cap=PP_CAP, diag=None, flags=0):
+        """fof() of src/nbody.js:646-870 on the current state (cc_analyse): the
+        report as a dict with the reference's names, pp.list an int32 array.
+        diag: an N x 4 DIAG read-out to use instead of the one on the device."""
+        d = None if diag is None else _f(diag).reshape(-1)
+        if d is not None and d.size != self.N * 4:
+            raise CCError("diag must hold N x 4 floats")
+        args = AnalyseArgs(float(m), float(a), float(spring), float(touch), int(cap), int(flags), _ptr(d))
+        rep = Report()
+        lst = np.zeros(max(int(cap), 1), np.int32)
+        self._chk(lib().cc_analyse(self.h, C.byref(args), C.byref(rep), _ptr(lst, C.c_int32)), "cc_analyse")
+        return report_dict(rep, lst)
 
     BOOKS = ("M", "px", "py", "pz", "Jx", "Jy", "Jz", "KE", "PE", "Q", "iron", "unseated", "full")
 

@@ -67,6 +67,7 @@ struct cc_ctx {
   float4* ffFar = nullptr;     // 3 x 4096: the network's farA, farT0, farT1
   float4* ffForce = nullptr;   // the force mesh from the network's far field (judge steps)
   int stepNo = 0;
+  FofWork* fof = nullptr;  // the analysis' workspace, made on its first call
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastMs = 0.0;
   std::vector<void*> allocs;
@@ -260,6 +261,7 @@ int cc_destroy(cc_ctx* c) {
   if (c->s) hipStreamSynchronize(c->s);
   for (CycleGraph& g : c->graphs) if (g.exec) hipGraphExecDestroy(g.exec);
   if (c->ffWork) ffwork_destroy(c->ffWork);
+  if (c->fof) fof_destroy(c->fof);
   if (c->boxHost) hipHostFree(c->boxHost);
   for (void* p : c->allocs) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -766,6 +768,20 @@ int cc_morton(cc_ctx* c, const double core[3], int32_t* perm, int32_t* into) {
   if (perm) HIPCHK(c, hipMemcpyAsync(perm, c->mperm, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->s));
   if (into) HIPCHK(c, hipMemcpyAsync(into, c->minto, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
+  return 0;
+}
+
+int cc_analyse(cc_ctx* c, const cc_analyse_args* a, cc_report* out, int32_t* list) {
+  if (!c || !a || !out) return CC_EINVAL;
+  if (a->cap < 0 || a->cap > PP_CAP) FAIL(c, CC_EINVAL, "cap %d outside 0..%d", a->cap, PP_CAP);
+  if (a->cap > 0 && !list) FAIL(c, CC_EINVAL, "list is NULL");
+  if (!(a->a > 0) || !(a->touch >= 0) || !(a->m > 0)) FAIL(c, CC_EINVAL, "m, a and touch must be positive");
+  if (!c->fof) c->fof = fof_create();
+  FofIn in{ c->N, c->pos[c->cur], c->vel[c->cur], c->mat, c->diag, a->diag, a->m, a->a, a->spring, a->touch,
+            a->cap, a->flags };
+  int r = fof_analyse(c->fof, in, out, list, c->s, c->err);
+  if (r) return r;
+  LAUNCHCHK(c);
   return 0;
 }
 

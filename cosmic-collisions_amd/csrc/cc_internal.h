@@ -22,6 +22,7 @@
 #include <string>
 
 struct cc_ff;
+struct cc_report;
 
 namespace cc {
 
@@ -127,6 +128,8 @@ void launch_couple(int N, const CoupleU& c, const float4* pos, const float4* vel
 void launch_diag(const SimU& u, const Box* bx, const float4* pos, const float* q, const uint32_t* seats, const MeshBuf& m0,
                  const MeshBuf& m1, const float* ppSlot, const float4* ppForce, float4* out, hipStream_t s);
 void launch_pp_slot(int N, const int* index, int count, float* slot, hipStream_t s);
+// exclusive prefix sum of data[0..n) in place; n <= 4 * 1024 * 1024, blockSums holds ceil(n / 1024) ints
+void launch_scan_excl(int* data, int n, int* blockSums, hipStream_t s);
 // Morton order: keys from FP64 arithmetic, a counting sort over 2^21 keys made
 // stable by ranking each key's members by index; perm[dst] = src, into[src] = dst
 void launch_morton(const float4* pos, int N, double cx, double cy, double cz, int* keys, int* slot, int* hist,
@@ -155,6 +158,23 @@ int ff_record(FFNet* n, const float4* coarse, const float4* mono, const Box* bx,
 void launch_judge(const float4* force, const float4* ref, double* sums, float tol, float* stats, int* gate,
                   hipStream_t s);
 void launch_count_learned(const int* gate, float* stats, hipStream_t s);
+
+// ---- the analysis (cc_fof.hip) ----
+struct FofWork;
+struct FofIn {
+  int N;
+  const float4* pos;       // device state
+  const float4* vel;
+  const float* mat;
+  const float4* diag;      // the DIAG read-out on the device ...
+  const float* hostDiag;   // ... or this host copy of it when not null
+  double m, a, spring, touch;
+  int cap, flags;
+};
+FofWork* fof_create();
+void fof_destroy(FofWork* w);
+// fills *out (a cc_report) and list; synchronous on stream s
+int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipStream_t s, std::string& err);
 
 }  // namespace cc
 

@@ -1142,6 +1142,13 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_final(int* data, int n, con
   if (i < n) data[i] = sh[threadIdx.x] - v + blockSums[blockIdx.x];
 }
 
+void launch_scan_excl(int* data, int n, int* blockSums, hipStream_t s) {
+  int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BLOCK), 0, s, data, n, blockSums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, blockSums, nb);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_BLOCK), 0, s, data, n, blockSums);
+}
+
 __global__ void __launch_bounds__(256) k_morton_scatter(int N, const int* keys, const int* slot, const int* start,
                                                         int* tmp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1169,10 +1176,7 @@ void launch_morton(const float4* pos, int N, double cx, double cy, double cz, in
   launch_fill_u32(reinterpret_cast<uint32_t*>(hist), (size_t)n, 0u, s);
   int blocks = (N + 255) / 256;
   hipLaunchKernelGGL(k_morton_keys, dim3(blocks), dim3(256), 0, s, pos, N, cx, cy, cz, keys, slot, hist);
-  int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BLOCK), 0, s, hist, n, blockSums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, blockSums, nb);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_BLOCK), 0, s, hist, n, blockSums);
+  launch_scan_excl(hist, n, blockSums, s);
   // perm is used as the scatter's scratch before the ranked write: into holds the arrival list
   hipLaunchKernelGGL(k_morton_scatter, dim3(blocks), dim3(256), 0, s, N, keys, slot, hist, into);
   hipLaunchKernelGGL(k_morton_rank, dim3(blocks), dim3(256), 0, s, N, keys, hist, into, perm, slot);

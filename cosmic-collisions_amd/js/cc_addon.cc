@@ -404,6 +404,87 @@ napi_value Books(napi_env env, napi_callback_info info) {
   return out;
 }
 
+napi_value set_num(napi_env env, napi_value o, const char* k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, o, k, x);
+  return o;
+}
+napi_value set_vec(napi_env env, napi_value o, const char* k, const double* v, int n) {
+  napi_value arr;
+  napi_create_array_with_length(env, n, &arr);
+  for (int i = 0; i < n; i++) {
+    napi_value x;
+    napi_create_double(env, v[i], &x);
+    napi_set_element(env, arr, i, x);
+  }
+  napi_set_named_property(env, o, k, arr);
+  return o;
+}
+
+// analyse(ctx, {m, a, spring, touch, cap, flags}) -> fof()'s report (src/nbody.js:888-892)
+napi_value Analyse(napi_env env, napi_callback_info info) {
+  CTX0();
+  napi_value o = a.argv[1];
+  cc_analyse_args in;
+  memset(&in, 0, sizeof in);
+  in.m = num(env, prop(env, o, "m"));
+  in.a = num(env, prop(env, o, "a"));
+  in.spring = num(env, prop(env, o, "spring"));
+  in.touch = num(env, prop(env, o, "touch"));
+  in.cap = (int32_t)num(env, prop(env, o, "cap"), 8192);
+  in.flags = (int32_t)num(env, prop(env, o, "flags"));
+  in.diag = nullptr;   // the DIAG read-out cc_diag left on the device
+  if (in.cap < 0 || in.cap > 8192) { napi_throw_range_error(env, nullptr, "cap outside 0..8192"); return nullptr; }
+  cc_report r;
+  napi_value ab, list;
+  void* data;
+  NAPI_OK(env, napi_create_arraybuffer(env, (size_t)(in.cap > 0 ? in.cap : 1) * 4, &data, &ab));
+  int code = cc_analyse(c, &in, &r, static_cast<int32_t*>(data));
+  if (code) return throw_cc(env, c, "cc_analyse", code);
+  NAPI_OK(env, napi_create_typedarray(env, napi_int32_array, (size_t)r.ppCount, ab, 0, &list));
+  napi_value out, second, pp, nat;
+  napi_create_object(env, &out);
+  set_num(env, out, "largest", r.largest); set_num(env, out, "largestImp", r.largestImp);
+  set_num(env, out, "largestIron", r.largestIron); set_num(env, out, "day", r.day); set_num(env, out, "tilt", r.tilt);
+  set_vec(env, out, "axis", r.axis, 3); set_vec(env, out, "bodyVel", r.bodyVel, 3);
+  set_num(env, out, "orbit", r.orbit); set_num(env, out, "escape", r.escape);
+  if (r.hasSecond) {
+    const cc_second& q = r.second;
+    napi_create_object(env, &second);
+    set_num(env, second, "mass", q.mass); set_num(env, second, "dist", q.dist);
+    napi_value b;
+    napi_get_boolean(env, q.bound != 0, &b);
+    napi_set_named_property(env, second, "bound", b);
+    set_num(env, second, "imp", q.imp); set_num(env, second, "iron", q.iron);
+    set_vec(env, second, "com", q.com, 3); set_vec(env, second, "core", q.core, 3); set_vec(env, second, "vel", q.vel, 3);
+    set_num(env, second, "R", q.R); set_num(env, second, "Redge", q.Redge); set_num(env, second, "Tsurf", q.Tsurf);
+    set_vec(env, second, "hotCom", q.hotCom, 3); set_num(env, second, "hotR", q.hotR);
+  } else {
+    napi_get_null(env, &second);
+  }
+  napi_set_named_property(env, out, "second", second);
+  set_vec(env, out, "com", r.com, 3); set_vec(env, out, "core", r.core, 3); set_vec(env, out, "vel", r.vel, 3);
+  set_num(env, out, "R", r.R); set_num(env, out, "Redge", r.Redge); set_num(env, out, "Tsurf", r.Tsurf);
+  set_vec(env, out, "hotCom", r.hotCom, 3); set_num(env, out, "hotR", r.hotR);
+  set_num(env, out, "impMass", r.impMass); set_vec(env, out, "impCom", r.impCom, 3); set_num(env, out, "impRms", r.impRms);
+  set_num(env, out, "M", r.M); set_num(env, out, "iron", r.iron); set_num(env, out, "drift", r.drift);
+  set_num(env, out, "L", r.L); set_num(env, out, "KE", r.KE); set_num(env, out, "PE", r.PE); set_num(env, out, "EL", r.EL);
+  set_num(env, out, "Q", r.Q); set_num(env, out, "unseated", r.unseated); set_num(env, out, "full", r.full);
+  napi_create_object(env, &pp);
+  napi_set_named_property(env, pp, "list", list);
+  set_num(env, pp, "loose", r.ppLoose);
+  napi_set_named_property(env, out, "pp", pp);
+  set_num(env, out, "jD", r.jD); set_num(env, out, "moon", r.moon);
+  // what the reference's report does not carry: group count and sizes, whether the unions were replayed, timing
+  napi_create_object(env, &nat);
+  set_num(env, nat, "groups", r.groups); set_num(env, nat, "largestN", r.largestN);
+  set_num(env, nat, "secondN", r.secondN); set_num(env, nat, "replayed", r.replayed);
+  set_vec(env, nat, "ms", r.ms, 3);
+  napi_set_named_property(env, out, "analysis", nat);
+  return out;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char* name; napi_callback cb; } fns[] = {
     { "create", Create }, { "destroy", Destroy }, { "setParams", SetParams }, { "uploadState", Upload }, { "downloadState", Download },
@@ -412,7 +493,7 @@ napi_value Init(napi_env env, napi_value exports) {
     { "writeMesh", WriteMesh }, { "setPPTable", SetPPTable }, { "setPP", SetPP }, { "pp", PP }, { "readPP", ReadPP },
     { "sim", Sim }, { "step", Step }, { "stepN", StepN }, { "getStepNo", GetStepNo }, { "setStepNo", SetStepNo },
     { "setGraphs", SetGraphs },
-    { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "books", Books }, { "sync", Sync },
+    { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "analyse", Analyse }, { "books", Books }, { "sync", Sync },
   };
   for (auto& f : fns) {
     napi_value fn;

@@ -9,7 +9,6 @@
 'use strict';
 const path = require('path');
 const bodies = require('./bodies');
-const { analyse } = require('./fof');
 
 const T_UNIT = 805, V_UNIT = 7.91, E_KG = 6.256e7, CP_ROCK = 1200, CP_IRON = 450, L_EM = 0.1163;
 const JAM = 0.96;
@@ -305,12 +304,14 @@ function createEngine(opts) {
     addon.downloadState(ctx, st.pos, st.vel, st.aux, st.q, st.mat, N);
     return st;
   }
-  // the read-outs for the books from seats, mesh and pairwise pass rebuilt now (src/nbody.js:488-507)
-  function passDiag() {
+  // the read-outs for the books from seats, mesh and pairwise pass rebuilt now (src/nbody.js:488-507);
+  // read back to the host unless `keep` (the analysis reads them on the device)
+  function passDiag(keep) {
     boxes();
     addon.slots(ctx);
     addon.mesh(ctx);
     addon.pp(ctx);
+    if (keep) { addon.diag(ctx, null); return null; }
     const D = new Float32Array(sim.N * 4);
     addon.diag(ctx, D);
     return D;
@@ -327,12 +328,10 @@ function createEngine(opts) {
     const sort = sim.phase === 'full' && analyses++ % 8 === 0;
     const ms = () => Number(process.hrtime.bigint()) / 1e6;
     const t0 = ms();
-    const D = passDiag();
+    passDiag(true);
     const t1 = ms();
-    const st = readState();
+    const r = addon.analyse(ctx, { m: sim.m, a: sim.a, spring: sim.k, touch: 2 * sim.a, cap: PP_CAP });
     const t2 = ms();
-    const r = analyse(sim.N, sim.m, sim.a, st.pos, st.vel, st.mat, D, sim.k, 2 * sim.a, PP_CAP);
-    const t3 = ms();
     r.t = sim.t;
     report = r; sim.com1 = r.com;
     placeBoxes(r);
@@ -352,7 +351,8 @@ function createEngine(opts) {
       r.perm = perm;
     }
     setPP(list, r.pp.loose);
-    lookMs = { diag: t1 - t0, read: t2 - t1, fof: t3 - t2, rest: ms() - t3, sort };
+    const am = r.analysis.ms;
+    lookMs = { diag: t1 - t0, analyse: t2 - t1, cells: am[0], copy: am[1], host: am[2], rest: ms() - t2, sort };
     emit('report', r);
     return r;
   }
@@ -372,7 +372,9 @@ function createEngine(opts) {
     onSim(f) { taps.sim.push(f); }, onReport(f) { taps.report.push(f); },
     get sim() { return sim; }, get report() { return report; }, get pending() { return null; },
     get analyzedAt() { return analyzedAt; }, get pp() { return pp; }, get ctx() { return ctx; }, get addon() { return addon; },
-    get lookMs() { return lookMs; },   // the last analysis's phases, ms: seats+mesh+pairwise+DIAG, read-back, fof, the rest
+    // the last analysis's phases, ms: seats+mesh+pairwise+DIAG; cc_analyse, of it the device's cells and
+    // groups, the read-back, the host's sums; the rest (boxes, Morton sort, loose list)
+    get lookMs() { return lookMs; },
     get state() { return { analyzing: false, pending: false, worker: false, analyses, analyzedAt, stepNo: sim && sim.stepNo }; },
     set meshEvery(n) { MESH_EVERY = n; if (ctx) addon.setParams(ctx, params()); },
     set asyncRead(v) {},

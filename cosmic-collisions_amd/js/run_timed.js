@@ -17,7 +17,7 @@ while (sim.phase === 'settle') NB.settleSome(64);
 while (sim.phase === 'approach') NB.rigidStep(0.01);
 NB.analyze();
 let stepMs = 0, lookMs = 0, done = 0, looks = 0;
-const phases = { diag: 0, read: 0, fof: 0, rest: 0 };
+const phases = { diag: 0, analyse: 0, cells: 0, copy: 0, host: 0, rest: 0 };
 const t0 = process.hrtime.bigint();
 while (done < steps) {
   const n = Math.min(every, steps - done);

@@ -137,6 +137,46 @@ int cc_morton(cc_ctx* ctx, const double core[3], int32_t* perm, int32_t* into);
 /* the FP64 books over the state and the last DIAG (src/nbody.js:831-850):
  * out[13] = M, px, py, pz, Jx, Jy, Jz, KE, PE, Q, iron mass, unseated count, full count */
 int cc_books(cc_ctx* ctx, double* out);
+
+/* The analysis of the current state: fof() of src/nbody.js:646-870 without its Morton sort
+ * (cc_morton does that from out->core).  Friends-of-friends groups at linking length 2.4 a
+ * (2^20 hashed cells, union-find), the largest group's and the second's statistics, the
+ * largest body's spin, the disk and the moon, the books and the loose list (at most cap
+ * entries, farthest first past the cap).  Every field equals the reference's bit for bit:
+ * the cells, the linked pairs and the components are found on the device; every FP64 sum
+ * runs on the host in the reference's particle-index order. */
+typedef struct {
+  double m;           /* mref: mass of a unit of mat (sim.m) */
+  double a;           /* particle radius (sim.a) */
+  double spring;      /* contact stiffness for the elastic energy (sim.k) */
+  double touch;       /* contact distance (2 a) */
+  int32_t cap;        /* loose-list capacity (PP_CAP, at most 8192) */
+  int32_t flags;      /* CC_ANALYSE_* */
+  const float* diag;  /* N x 4 DIAG read-out to use, or NULL for the one cc_diag left on the device */
+} cc_analyse_args;
+/* replay the reference's union order on the host even when no tie among the largest
+ * groups needs it (the replay is otherwise done only when it can change the result) */
+#define CC_ANALYSE_REPLAY_UNIONS 1
+typedef struct {      /* the second group, when it is a body apart (src/nbody.js:811-816) */
+  double mass, dist;
+  int32_t bound, pad_;
+  double imp, iron, com[3], core[3], vel[3], R, Redge, Tsurf, hotCom[3], hotR;
+} cc_second;
+typedef struct cc_report {   /* fof()'s report (src/nbody.js:888-892), same names */
+  double largest, largestImp, largestIron, day, tilt, axis[3], bodyVel[3], orbit, escape;
+  double com[3], core[3], vel[3], R, Redge, Tsurf, hotCom[3], hotR, impMass, impCom[3], impRms;
+  double M, iron, drift, L, KE, PE, EL, Q, unseated, full, jD, moon;
+  int32_t hasSecond, pad_;
+  cc_second second;
+  int32_t ppCount;    /* entries written to list */
+  int32_t ppLoose;    /* loose particles before the cap (pp.loose) */
+  int32_t groups;     /* number of groups (no reference counterpart) */
+  int32_t largestN, secondN;   /* member counts of the largest and second group */
+  int32_t replayed;   /* 1 when the union order was replayed on the host */
+  double ms[3];       /* device, transfer and host milliseconds of this call */
+} cc_report;
+int cc_analyse(cc_ctx* ctx, const cc_analyse_args* args, cc_report* out, int32_t* list);
+
 /* wait for the ctx's stream */
 int cc_sync(cc_ctx* ctx);
 /* device time of the last cc_step_n call, ms (HIP events on the ctx's stream) */

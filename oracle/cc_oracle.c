@@ -760,3 +760,85 @@ void cco_direct_f64(int N, const float* pos, const float* mat, double gm, double
     out[i * 4] = ax * gm; out[i * 4 + 1] = ay * gm; out[i * 4 + 2] = az * gm; out[i * 4 + 3] = pot * gm;
   }
 }
+
+/* ---- friends-of-friends linking (src/nbody.js:647-684) ----
+ * The reference's own sequence: cells of width 2.4a hashed into 2^20 buckets
+ * (products in FP64, each operand of ^ taken to int32 as JavaScript does),
+ * a counting sort in index order, then for i ascending the 27 buckets dz, dy,
+ * dx from -1 to 1 and each bucket's members in order, uniting every pair
+ * j > i closer than 2.4a with the reference's union-find (uf[ri] = rj).
+ * Outputs: root[i] = that union-find's root, label[i] = the smallest index in
+ * i's group, e2[] = the touching pairs' squared distances in pair order (the
+ * count is returned; e2 may be NULL or short, then only counted), the elastic
+ * energy summed as the reference sums it, and when linkOff is given every
+ * linked partner in order (linkOff[N+1], partner[] of room `pcap`). */
+static int32_t js_int32(double d) {
+  if (!(d == d) || isinf(d)) return 0;
+  if (d > -9.2e18 && d < 9.2e18) return (int32_t)(uint32_t)(uint64_t)(int64_t)d;
+  double m = fmod(trunc(d), 4294967296.0);
+  if (m < 0) m += 4294967296.0;
+  return (int32_t)(uint32_t)m;
+}
+static int fof_hash(double x, double y, double z) {
+  return (js_int32(x * 73856093.0) ^ js_int32(y * 19349663.0) ^ js_int32(z * 83492791.0)) & ((1 << 20) - 1);
+}
+static int uf_find(int* uf, int x) {
+  while (uf[x] != x) { uf[x] = uf[uf[x]]; x = uf[x]; }
+  return x;
+}
+long cco_fof_link(int N, const float* P, double a, double touch, double spring, int32_t* root, int32_t* label,
+                  double* e2out, long e2cap, double* elastic, int32_t* linkOff, int32_t* partner, long pcap) {
+  const int T = 1 << 20;
+  double L = 2.4 * a, L2 = L * L, inv = 1 / L, touch2 = touch * touch, el = 0;
+  int32_t *cx = malloc(sizeof(int32_t) * N), *cy = malloc(sizeof(int32_t) * N), *cz = malloc(sizeof(int32_t) * N);
+  int32_t *key = malloc(sizeof(int32_t) * N), *cnt = calloc((size_t)T + 1, sizeof(int32_t));
+  int32_t *fill = malloc(sizeof(int32_t) * T), *order = malloc(sizeof(int32_t) * N), *uf = malloc(sizeof(int32_t) * N);
+  for (int i = 0; i < N; i++) {
+    double qx = floor((double)P[i * 4] * inv), qy = floor((double)P[i * 4 + 1] * inv), qz = floor((double)P[i * 4 + 2] * inv);
+    cx[i] = js_int32(qx); cy[i] = js_int32(qy); cz[i] = js_int32(qz);
+    key[i] = fof_hash(qx, qy, qz);
+    cnt[key[i] + 1]++;
+  }
+  for (int k = 0; k < T; k++) cnt[k + 1] += cnt[k];
+  memcpy(fill, cnt, sizeof(int32_t) * T);
+  for (int i = 0; i < N; i++) order[fill[key[i]]++] = i;
+  for (int i = 0; i < N; i++) uf[i] = i;
+  long nt = 0, np = 0;
+  for (int i = 0; i < N; i++) {
+    double px = P[i * 4], py = P[i * 4 + 1], pz = P[i * 4 + 2];
+    if (linkOff) linkOff[i] = (int32_t)np;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int k = fof_hash((double)cx[i] + dx, (double)cy[i] + dy, (double)cz[i] + dz);
+          for (int s = cnt[k]; s < cnt[k + 1]; s++) {
+            int j = order[s];
+            if (j <= i) continue;
+            double ex = (double)P[j * 4] - px, ey = (double)P[j * 4 + 1] - py, ez = (double)P[j * 4 + 2] - pz;
+            double e2 = ex * ex + ey * ey + ez * ez;
+            if (e2 < L2) {
+              int ri = uf_find(uf, i), rj = uf_find(uf, j);
+              if (ri != rj) uf[ri] = rj;
+              if (linkOff && np < pcap) partner[np] = j;
+              np++;
+              if (e2 < touch2) {
+                double ov = touch - sqrt(e2);
+                el += 0.5 * spring * ov * ov;
+                if (e2out && nt < e2cap) e2out[nt] = e2;
+                nt++;
+              }
+            }
+          }
+        }
+  }
+  if (linkOff) linkOff[N] = (int32_t)np;
+  for (int i = 0; i < N; i++) root[i] = uf_find(uf, i);
+  /* smallest member of each root's group */
+  int32_t* small = malloc(sizeof(int32_t) * N);
+  for (int i = 0; i < N; i++) small[i] = -1;
+  for (int i = 0; i < N; i++) if (small[root[i]] < 0) small[root[i]] = i;
+  for (int i = 0; i < N; i++) label[i] = small[root[i]];
+  *elastic = el;
+  free(small); free(cx); free(cy); free(cz); free(key); free(cnt); free(fill); free(order); free(uf);
+  return nt;
+}

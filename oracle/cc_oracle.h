@@ -103,6 +103,10 @@ uint32_t cco_mulberry_next(uint32_t* state, double* out);
 /* A10 meshPairTable, src/nbody.js:56-109 (bins x {force r^2, potential r}) */
 void cco_mesh_pair_table(int bins, double cut, float* f, float* p);
 
+/* friends-of-friends linking in the reference's order (src/nbody.js:647-684); returns the touching-pair count */
+long cco_fof_link(int N, const float* P, double a, double touch, double spring, int32_t* root, int32_t* label,
+                  double* e2out, long e2cap, double* elastic, int32_t* linkOff, int32_t* partner, long pcap);
+
 #ifdef __cplusplus
 }
 #endif

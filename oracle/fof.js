@@ -1,12 +1,14 @@
-// The host read-out of a run (src/nbody.js:646-893): friends-of-friends
+// CPU restatement of the reference's host analysis (fof, src/nbody.js:646-893)
+// -- TEST INFRASTRUCTURE, the checker only: the tests run it beside the
+// library's analysis (cc_analyse) on the same state.  Friends-of-friends
 // groups, the largest and second body's statistics, spin, disk and moon, the
-// conservation books and the loose list the pairwise pass corrects.  The
-// Morton order that the reference also computes here is the GPU's
-// (cc_morton), given this analysis' core centre.
+// conservation books and the loose list the pairwise pass corrects (the
+// Morton order the reference also computes there is cc_morton's).
 //
 // Every float64 sum runs in particle-index order, and the union-find walks
 // the same hashed cells in the same order, so group choice, the core centre
-// and the loose list agree with the reference's analysis bit for bit.
+// and the loose list agree with the reference's analysis bit for bit; the
+// reference's own reports on the golden states pin it (tests/test_js_host.py).
 'use strict';
 
 const HASH_BITS = 20;

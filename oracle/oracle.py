@@ -67,6 +67,9 @@ def lib():
         L.cco_mulberry_next.restype = C.c_uint32
         L.cco_direct_f64.argtypes = [C.c_int, f32p, f32p, C.c_double, C.c_double, f64p]
         L.cco_mesh_pair_table.argtypes = [C.c_int, C.c_double, f32p, f32p]
+        L.cco_fof_link.argtypes = [C.c_int, f32p, C.c_double, C.c_double, C.c_double, i32p, i32p, C.c_void_p, C.c_long,
+                                   C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_long]
+        L.cco_fof_link.restype = C.c_long
         _LIB = L
     return _LIB
 
@@ -243,3 +246,27 @@ def mesh_pair_table(bins=64, cut=5 * 12.0 / 64):
     p = np.empty(bins, np.float32)
     lib().cco_mesh_pair_table(bins, cut, f, p)
     return np.stack([f, p], axis=1)
+
+
+def fof_link(pos, a, touch, spring, pairs=False):
+    """Friends-of-friends linking as the reference runs it (src/nbody.js:647-684):
+    {root, label (smallest index of the group), e2 (touching pairs' squared
+    distances in pair order), elastic, and with pairs linkOff/partner}."""
+    P = c32(pos).ravel()
+    N = P.size // 4
+    root = np.empty(N, np.int32)
+    label = np.empty(N, np.int32)
+    el = C.c_double()
+    nt = lib().cco_fof_link(N, P, a, touch, spring, root, label, None, 0, C.byref(el), None, None, 0)
+    e2 = np.empty(max(nt, 1), np.float64)
+    off = np.empty(N + 1, np.int32) if pairs else None
+    lib().cco_fof_link(N, P, a, touch, spring, root, label, e2.ctypes.data, nt, C.byref(el),
+                       off.ctypes.data if pairs else None, None, 0)
+    out = {"root": root, "label": label, "e2": e2[:nt], "elastic": el.value}
+    if pairs:
+        npairs = int(off[N])
+        partner = np.empty(max(npairs, 1), np.int32)
+        lib().cco_fof_link(N, P, a, touch, spring, root, label, None, 0, C.byref(el), off.ctypes.data,
+                           partner.ctypes.data, npairs)
+        out["linkOff"], out["partner"] = off, partner[:npairs]
+    return out

@@ -13,6 +13,7 @@ import pytest
 import golden_util as gu
 
 JS = os.path.join(gu.REPO, "cosmic-collisions_amd", "js")
+ORACLE = os.path.join(gu.REPO, "oracle")
 NODE = shutil.which("node")
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
@@ -57,8 +58,9 @@ const nb = require('%s/nbody.js'); require('fs').writeFileSync('%s/tab', Buffer.
 
 @pytest.mark.parametrize("name", ["impact", "twobox"])
 def test_analysis_matches_reference(tmp_path, name):
-    """The host analysis on the golden state (positions, velocities, masses and
-    the reference's own DIAG read-out) against the reference analysis."""
+    """The oracle's restatement of the host analysis (oracle/fof.js) on the
+    golden state (positions, velocities, masses and the reference's own DIAG
+    read-out) against the reference analysis."""
     fx = gu.fixture("c1_" + name)
     sim = json.loads(str(fx["sim"]))
     for k, a in (("P", fx["pos"]), ("V", fx["vel"]), ("R", fx["mat"]), ("D", fx["diag"].reshape(-1, 4))):
@@ -69,7 +71,7 @@ const rd = (k) => { const b = fs.readFileSync('%s/' + k); return new Float32Arra
 const r = analyse(%d, %r, %r, rd('P'), rd('V'), rd('R'), rd('D'), %r, %r, 8192);
 fs.writeFileSync('%s/list', Buffer.from(r.pp.list.buffer));
 delete r.pp; console.log(JSON.stringify(r));
-""" % (JS, tmp_path, sim["N"], sim["m"], sim["a"], sim["k"], 2 * sim["a"], tmp_path), tmp_path)
+""" % (ORACLE, tmp_path, sim["N"], sim["m"], sim["a"], sim["k"], 2 * sim["a"], tmp_path), tmp_path)
     mine = json.loads(out)
     ref = json.loads(str(fx["report"]))
     for k in ("largest", "largestImp", "largestIron", "day", "tilt", "axis", "bodyVel", "orbit", "escape", "com", "core",
