@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <thread>
 #include <vector>
 
 namespace cc {
@@ -131,17 +133,30 @@ struct GroupSums {
   }
 };
 
-// the rest of stats(): the core walk, the surface temperature, the hot spot,
-// the impactor's spread and the edge (src/nbody.js:701-760)
-static Group finish_group(const GroupSums& g, double mref, const F4* P, const F4* V, const float* Rm) {
+// what stats() takes from a group's first pass alone: mass, centre, velocity,
+// shares, radius, the impactor's centre (src/nbody.js:701-702)
+static Group group_basics(const GroupSums& g) {
   Group o;
-  const std::vector<int>& mem = g.members;
-  int n = (int)mem.size();
-  o.n = n;
+  o.n = (int)g.members.size();
   o.mass = g.mass;
   o.com[0] = g.x / g.mass; o.com[1] = g.y / g.mass; o.com[2] = g.z / g.mass;
-  double Rg = js_cbrt(g.mass);
-  double icx = g.imp > 0 ? g.ix / g.imp : 0, icy = g.imp > 0 ? g.iy / g.imp : 0, icz = g.imp > 0 ? g.iz / g.imp : 0;
+  o.vel[0] = g.vx / g.mass; o.vel[1] = g.vy / g.mass; o.vel[2] = g.vz / g.mass;
+  o.imp = g.imp / g.mass;
+  o.iron = g.iron / g.mass;
+  o.R = js_cbrt(g.mass);
+  o.impMass = g.imp;
+  o.impCom[0] = g.imp > 0 ? g.ix / g.imp : 0;
+  o.impCom[1] = g.imp > 0 ? g.iy / g.imp : 0;
+  o.impCom[2] = g.imp > 0 ? g.iz / g.imp : 0;
+  return o;
+}
+
+// the rest of stats(): the core walk, the surface temperature, the hot spot,
+// the impactor's spread and the edge (src/nbody.js:712-760), over the members
+static void group_walks(const GroupSums& g, Group& o, double mref, const F4* P, const F4* V, const float* Rm) {
+  const std::vector<int>& mem = g.members;
+  const int n = o.n;
+  const double Rg = o.R, icx = o.impCom[0], icy = o.impCom[1], icz = o.impCom[2];
   double tm = g.mass - g.imp;
   double core[3];
   if (g.imp > tm) { core[0] = icx; core[1] = icy; core[2] = icz; }
@@ -202,18 +217,12 @@ static Group finish_group(const GroupSums& g, double mref, const F4* P, const F4
       if ((b + 0.5) * dr > 0.85 * Rb && refN > 0 && rho[b] < 0.35 * refS / refN) { Redge = b * dr; break; }
   }
   for (int k = 0; k < 3; k++) { o.core[k] = core[k]; o.hotCom[k] = hotCom[k]; }
-  o.vel[0] = g.vx / g.mass; o.vel[1] = g.vy / g.mass; o.vel[2] = g.vz / g.mass;
-  o.imp = g.imp / g.mass;
-  o.iron = g.iron / g.mass;
-  o.R = Rg;
   o.Redge = Redge;
   o.Tsurf = ws > 0 ? Ts / ws : 0;
   o.hotR = hotR;
-  o.impMass = g.imp;
-  o.impCom[0] = icx; o.impCom[1] = icy; o.impCom[2] = icz;
   o.impRms = g.imp > 0 ? std::sqrt(irms / g.imp) : 0;
-  return o;
 }
+
 
 
 FofPick fof_pick(const FofHostIn& h) {
@@ -277,6 +286,22 @@ void fof_pick_replay(const FofHostIn& h, const int* linkOff, const int* partner,
   k.replay = true;
 }
 
+// the tasks at once, each but the last on a thread of its own; a task whose
+// thread cannot be had runs on the caller's, in turn
+static void run_together(std::vector<std::function<void()>>& tasks) {
+  std::vector<std::thread> th;
+  size_t k = 0;
+  for (; k + 1 < tasks.size(); k++) {
+    try {
+      th.emplace_back(tasks[k]);
+    } catch (...) {
+      break;
+    }
+  }
+  for (size_t r = k; r < tasks.size(); r++) tasks[r]();
+  for (auto& t : th) t.join();
+}
+
 void fof_report(const FofHostIn& h, const FofPick& k, cc_report* out, int32_t* list) {
   const int N = h.N;
   const F4 *P = h.P, *V = h.V, *D = h.D;
@@ -285,13 +310,6 @@ void fof_report(const FofHostIn& h, const FofPick& k, cc_report* out, int32_t* l
   const double mref = h.m;
   const int g1 = k.g1, g2 = k.g2;
 
-  // the contacts' spring energy, summed in the reference's pair order (src/nbody.js:674)
-  double elastic = 0;
-  for (size_t q = 0; q < h.nTouch; q++) {
-    double ov = h.touch - std::sqrt(h.e2Touch[q]);
-    elastic += 0.5 * h.spring * ov * ov;
-  }
-
   // stats() of the largest and the second group: the first pass of both in one
   // walk, each group's sums in index order (src/nbody.js:688-700)
   GroupSums s1, s2;
@@ -299,40 +317,20 @@ void fof_report(const FofHostIn& h, const FofPick& k, cc_report* out, int32_t* l
     if (lab[t] == g1) s1.add(t, mref, P, V, Rm);
     else if (lab[t] == g2) s2.add(t, mref, P, V, Rm);
   }
-  Group A = finish_group(s1, mref, P, V, Rm), B;
-  bool haveB = g2 >= 0;
-  if (haveB) B = finish_group(s2, mref, P, V, Rm);
+  const bool haveB = g2 >= 0;
+  Group A = group_basics(s1), B;
+  if (haveB) B = group_basics(s2);
+  const bool bigB = haveB && B.mass > 0.05;
 
-  // the largest body's spin (src/nbody.js:770-795): its members in index order
-  const double rbq = 1.15 * js_cbrt(js_max(A.impMass, A.mass - A.impMass)), rb2 = rbq * rbq;
-  const double c0 = A.core[0], c1 = A.core[1], c2 = A.core[2];
-  double bm = 0, bvx = 0, bvy = 0, bvz = 0;
-  for (int t : s1.members) {
-    double dx = (double)P[t].x - c0, dy = (double)P[t].y - c1, dz = (double)P[t].z - c2;
-    if (dx * dx + dy * dy + dz * dz > rb2) continue;
-    double m = mref * (double)Rm[t];
-    bm += m; bvx += m * (double)V[t].x; bvy += m * (double)V[t].y; bvz += m * (double)V[t].z;
-  }
-  if (bm > 0) { bvx /= bm; bvy /= bm; bvz /= bm; }
-  double Lx = 0, Ly = 0, Lz = 0, I = 0;
-  for (int t : s1.members) {
-    double dx = (double)P[t].x - c0, dy = (double)P[t].y - c1, dz = (double)P[t].z - c2;
-    if (dx * dx + dy * dy + dz * dz > rb2) continue;
-    double m = mref * (double)Rm[t];
-    double ux = (double)V[t].x - bvx, uy = (double)V[t].y - bvy, uz = (double)V[t].z - bvz;
-    Lx += m * (dy * uz - dz * uy); Ly += m * (dz * ux - dx * uz); Lz += m * (dx * uy - dy * ux);
-  }
-  double Lm = std::sqrt(Lx * Lx + Ly * Ly + Lz * Lz);
-  double lx = Lx / (Lm + 1e-30), ly = Ly / (Lm + 1e-30), lz = Lz / (Lm + 1e-30);
-  double tilt = js_acos(js_max(-1, js_min(1, ly))) * 180 / M_PI;
-  for (int t : s1.members) {
-    double dx = (double)P[t].x - c0, dy = (double)P[t].y - c1, dz = (double)P[t].z - c2;
-    if (dx * dx + dy * dy + dz * dz > rb2) continue;
-    double m = mref * (double)Rm[t];
-    double along = dx * lx + dy * ly + dz * lz;
-    I += m * (dx * dx + dy * dy + dz * dz - along * along);
-  }
-  double day = Lm > 1e-9 ? 2 * M_PI * I / Lm * 805 / 3600 : INFINITY;
+  // Every sum below is its own chain in the reference's order; chains that do
+  // not read each other's results run side by side: the elastic energy, the
+  // second group's walks, the disk with the books, the loose list, and on this
+  // thread the largest group's walks and then its spin.
+  double elastic = 0;
+  double orbit = 0, esc = 0, Dx = 0, Dy = 0, Dz = 0;
+  double Mt = 0, px = 0, py = 0, pz = 0, Jx = 0, Jy = 0, Jz = 0, ke = 0, pe = 0, Q = 0, ironAll = 0;
+  int unseated = 0, full = 0, nLoose = 0, wr = 0;
+  double bvx = 0, bvy = 0, bvz = 0, lx = 0, ly = 0, lz = 0, tilt = 0, day = 0;
 
   // energy in the largest body's frame (src/nbody.js:797-801)
   auto energy = [&](double x, double y, double z, double vx, double vy, double vz) {
@@ -342,34 +340,99 @@ void fof_report(const FofHostIn& h, const FofPick& k, cc_report* out, int32_t* l
                  (vz - A.vel[2]) * (vz - A.vel[2]);
     return 0.5 * dv2 - A.mass / js_max(r, A.R);
   };
-  // the disk (src/nbody.js:807-815) and the books (src/nbody.js:831-850), one walk
-  double orbit = 0, esc = 0, Dx = 0, Dy = 0, Dz = 0;
-  double Mt = 0, px = 0, py = 0, pz = 0, Jx = 0, Jy = 0, Jz = 0, ke = 0, pe = 0, Q = 0, ironAll = 0;
-  int unseated = 0, full = 0;
-  for (int i = 0; i < N; i++) {
-    double m = mref * (double)Rm[i];
-    double bx = P[i].x, by = P[i].y, bz = P[i].z, bu = V[i].x, bv = V[i].y, bw = V[i].z;
-    if (lab[i] != g1) {
-      if (energy(bx, by, bz, bu, bv, bw) < 0) {
-        orbit += m;
-        double dx = bx - A.com[0], dy = by - A.com[1], dz = bz - A.com[2];
-        double wx = bu - A.vel[0], wy = bv - A.vel[1], wz = bw - A.vel[2];
-        Dx += m * (dy * wz - dz * wy); Dy += m * (dz * wx - dx * wz); Dz += m * (dx * wy - dy * wx);
-      } else {
-        esc += m;
-      }
+  auto far2 = [&](const Group& b, int i) {
+    double ex = (double)P[i].x - b.com[0], ey = (double)P[i].y - b.com[1], ez = (double)P[i].z - b.com[2];
+    return ex * ex + ey * ey + ez * ez;
+  };
+
+  std::vector<std::function<void()>> tasks;
+  // the contacts' spring energy, summed in the reference's pair order (src/nbody.js:674)
+  tasks.push_back([&] {
+    for (size_t q = 0; q < h.nTouch; q++) {
+      double ov = h.touch - std::sqrt(h.e2Touch[q]);
+      elastic += 0.5 * h.spring * ov * ov;
     }
-    Mt += m; px += m * bu; py += m * bv; pz += m * bw;
-    if (is_iron(P[i].w)) ironAll += m;
-    Jx += m * (by * bw - bz * bv); Jy += m * (bz * bu - bx * bw); Jz += m * (bx * bv - by * bu);
-    ke += 0.5 * m * (bu * bu + bv * bv + bw * bw);
-    double flag = D[i].y;
-    bool off = flag >= 2;
-    pe += (off ? 1 : 0.5) * m * (double)D[i].x;
-    Q += 0.5 * (double)D[i].w;
-    if (flag - (off ? 2 : 0) < 0.5) unseated++;
-    if ((double)D[i].z >= 7.5) full++;
-  }
+  });
+  if (haveB) tasks.push_back([&] { group_walks(s2, B, mref, P, V, Rm); });
+  // the disk (src/nbody.js:807-815) and the books (src/nbody.js:831-850), one walk
+  tasks.push_back([&] {
+    for (int i = 0; i < N; i++) {
+      double m = mref * (double)Rm[i];
+      double bx = P[i].x, by = P[i].y, bz = P[i].z, bu = V[i].x, bv = V[i].y, bw = V[i].z;
+      if (lab[i] != g1) {
+        if (energy(bx, by, bz, bu, bv, bw) < 0) {
+          orbit += m;
+          double dx = bx - A.com[0], dy = by - A.com[1], dz = bz - A.com[2];
+          double wx = bu - A.vel[0], wy = bv - A.vel[1], wz = bw - A.vel[2];
+          Dx += m * (dy * wz - dz * wy); Dy += m * (dz * wx - dx * wz); Dz += m * (dx * wy - dy * wx);
+        } else {
+          esc += m;
+        }
+      }
+      Mt += m; px += m * bu; py += m * bv; pz += m * bw;
+      if (is_iron(P[i].w)) ironAll += m;
+      Jx += m * (by * bw - bz * bv); Jy += m * (bz * bu - bx * bw); Jz += m * (bx * bv - by * bu);
+      ke += 0.5 * m * (bu * bu + bv * bv + bw * bw);
+      double flag = D[i].y;
+      bool off = flag >= 2;
+      pe += (off ? 1 : 0.5) * m * (double)D[i].x;
+      Q += 0.5 * (double)D[i].w;
+      if (flag - (off ? 2 : 0) < 0.5) unseated++;
+      if ((double)D[i].z >= 7.5) full++;
+    }
+  });
+  // the loose list (src/nbody.js:857-870): beyond 1.3 radii of the largest
+  // body, and of the second when it is a body of 0.05 or more; farthest first
+  // past the cap
+  tasks.push_back([&] {
+    auto loose = [&](int i) { return far2(A, i) > 1.69 * A.R * A.R && !(bigB && far2(B, i) <= 1.69 * B.R * B.R); };
+    double least = 0;
+    std::vector<double> ds;
+    for (int i = 0; i < N; i++)
+      if (loose(i)) { nLoose++; ds.push_back(far2(A, i)); }
+    if (nLoose > h.cap) {
+      std::nth_element(ds.begin(), ds.begin() + (nLoose - h.cap), ds.end());
+      least = ds[nLoose - h.cap];
+    }
+    int count = std::min(nLoose, h.cap);
+    for (int i = 0; i < N && wr < count; i++)
+      if (loose(i) && far2(A, i) >= least) list[wr++] = i;
+  });
+  // the largest group's walks, then its body's spin (src/nbody.js:770-795), its members in index order
+  tasks.push_back([&] {
+    group_walks(s1, A, mref, P, V, Rm);
+    const double rbq = 1.15 * js_cbrt(js_max(A.impMass, A.mass - A.impMass)), rb2 = rbq * rbq;
+    const double c0 = A.core[0], c1 = A.core[1], c2 = A.core[2];
+    double bm = 0;
+    for (int t : s1.members) {
+      double dx = (double)P[t].x - c0, dy = (double)P[t].y - c1, dz = (double)P[t].z - c2;
+      if (dx * dx + dy * dy + dz * dz > rb2) continue;
+      double m = mref * (double)Rm[t];
+      bm += m; bvx += m * (double)V[t].x; bvy += m * (double)V[t].y; bvz += m * (double)V[t].z;
+    }
+    if (bm > 0) { bvx /= bm; bvy /= bm; bvz /= bm; }
+    double Lx = 0, Ly = 0, Lz = 0, I = 0;
+    for (int t : s1.members) {
+      double dx = (double)P[t].x - c0, dy = (double)P[t].y - c1, dz = (double)P[t].z - c2;
+      if (dx * dx + dy * dy + dz * dz > rb2) continue;
+      double m = mref * (double)Rm[t];
+      double ux = (double)V[t].x - bvx, uy = (double)V[t].y - bvy, uz = (double)V[t].z - bvz;
+      Lx += m * (dy * uz - dz * uy); Ly += m * (dz * ux - dx * uz); Lz += m * (dx * uy - dy * ux);
+    }
+    double Lm = std::sqrt(Lx * Lx + Ly * Ly + Lz * Lz);
+    lx = Lx / (Lm + 1e-30); ly = Ly / (Lm + 1e-30); lz = Lz / (Lm + 1e-30);
+    tilt = js_acos(js_max(-1, js_min(1, ly))) * 180 / M_PI;
+    for (int t : s1.members) {
+      double dx = (double)P[t].x - c0, dy = (double)P[t].y - c1, dz = (double)P[t].z - c2;
+      if (dx * dx + dy * dy + dz * dz > rb2) continue;
+      double m = mref * (double)Rm[t];
+      double along = dx * lx + dy * ly + dz * lz;
+      I += m * (dx * dx + dy * dy + dz * dz - along * along);
+    }
+    day = Lm > 1e-9 ? 2 * M_PI * I / Lm * 805 / 3600 : INFINITY;
+  });
+  run_together(tasks);
+
   double jD = orbit > 0 ? std::sqrt(Dx * Dx + Dy * Dy + Dz * Dz) / (orbit * std::sqrt(A.mass * 2.9)) : 0;
   double moon = js_min(js_max(0, orbit * (1.9 * jD - 1.1 - 1.9 * 0.05)), 0.95 * orbit);
 
@@ -387,28 +450,6 @@ void fof_report(const FofHostIn& h, const FofPick& k, cc_report* out, int32_t* l
       for (int k = 0; k < 3; k++) { q.com[k] = B.com[k]; q.core[k] = B.core[k]; q.vel[k] = B.vel[k]; q.hotCom[k] = B.hotCom[k]; }
     }
   }
-
-  // the loose list (src/nbody.js:857-870): beyond 1.3 radii of the largest
-  // body, and of the second when it is a body of 0.05 or more; farthest first
-  // past the cap
-  const bool bigB = haveB && B.mass > 0.05;
-  auto far2 = [&](const Group& b, int i) {
-    double ex = (double)P[i].x - b.com[0], ey = (double)P[i].y - b.com[1], ez = (double)P[i].z - b.com[2];
-    return ex * ex + ey * ey + ez * ez;
-  };
-  auto loose = [&](int i) { return far2(A, i) > 1.69 * A.R * A.R && !(bigB && far2(B, i) <= 1.69 * B.R * B.R); };
-  int nLoose = 0;
-  double least = 0;
-  std::vector<double> ds;
-  for (int i = 0; i < N; i++)
-    if (loose(i)) { nLoose++; ds.push_back(far2(A, i)); }
-  if (nLoose > h.cap) {
-    std::nth_element(ds.begin(), ds.begin() + (nLoose - h.cap), ds.end());
-    least = ds[nLoose - h.cap];
-  }
-  int count = std::min(nLoose, h.cap), wr = 0;
-  for (int i = 0; i < N && wr < count; i++)
-    if (loose(i) && far2(A, i) >= least) list[wr++] = i;
 
   out->largest = A.mass; out->largestImp = A.imp; out->largestIron = A.iron; out->day = day; out->tilt = tilt;
   out->axis[0] = lx; out->axis[1] = ly; out->axis[2] = lz;
