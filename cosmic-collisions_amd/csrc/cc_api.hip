@@ -139,6 +139,12 @@ static SimU sim_uniforms(const cc_ctx* c, const cc_sim_args* a) {
   u.link2 = (float)(p.cell * p.cell);
   double reach = p.cell + p.seatSlack * p.a;
   u.reach2 = (float)(reach * reach);
+  {
+    // the prefilter on the 1/480-cell lattice: the reach, plus the half-unit per
+    // axis (sqrt(3)/2 in all) that rounding the particle onto the lattice moves it
+    double rq = reach * SEAT_Q / p.cell + 1.0;
+    u.reachQ2 = (int)(rq * rq);
+  }
   u.k = (float)p.k;
   u.c = (float)p.c;
   u.invM = (float)(1.0 / p.m);

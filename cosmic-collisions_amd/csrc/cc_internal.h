@@ -27,6 +27,7 @@ struct cc_report;
 namespace cc {
 
 constexpr int SLOTS = 8;             // seats per contact cell
+constexpr int SEAT_Q = 480;          // the seat prefilter's lattice: 1/480 of a cell (32 per fifteenth)
 constexpr int BUCKET_CAP = 16;       // arrivals kept per cell before the overflow list
 constexpr int PP_CAP = 8192;
 constexpr int FINE_TEXELS = 513 * 512;
@@ -66,6 +67,7 @@ struct SimU {           // the step pass's uniforms, already FP32
   float gField[3];
   int frozenOn;
   float frozen;
+  int reachQ2;          // the seat prefilter's bound in 1/480-cell units, squared (see k_sim)
 };
 
 struct CoupleU {        // COUPLE_FS's uniforms (chicxulub.html:260-276), FP32

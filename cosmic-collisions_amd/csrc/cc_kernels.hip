@@ -673,7 +673,7 @@ __device__ __forceinline__ void cross_at(float4 mo, float4 mm, float gm, float e
 //   C  the touching pairs' velocities and masses, and the sums.
 // A lane whose list would overflow finishes alone on the direct path.
 constexpr int SIM_BLOCK = 64;
-constexpr int CAND_CAP = 40;
+constexpr int CAND_CAP = 39;
 
 struct Contact {
   float pushx, pushy, pushz, nb, lost, flow;
@@ -701,19 +701,20 @@ __device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, 
 // :426-427): the seat's recorded place, (q + 0.5)/15 of a cell into it, within
 // reach (cell + 0.6a, src/nbody.js:41, :297) of p.  It only prunes: the 0.6a
 // of slack over the link distance is the reference's bound on how far a
-// particle sits from its seat's place, so a seat at the edge of reach holds a
-// particle beyond the link test, and the rounding of the place cannot change
-// which pairs are summed.  The place is one FMA per axis.
-__device__ __forceinline__ float seat_place(uint32_t q, float lo, const SimU& u) {
-  return fmaf((float)q, u.cell * (1.0f / 15.0f), lo + u.cell * (0.5f / 15.0f));
-}
-__device__ __forceinline__ bool seat_passes(uint32_t sv, int me, float lox, float loy, float loz, float px, float py,
-                                            float pz, const SimU& u) {
-  int B = u.grid.IB;
-  int j = (int)(sv & ((1u << B) - 1u));
-  float dqx = seat_place((sv >> B) & 15u, lox, u) - px, dqy = seat_place((sv >> (B + 4)) & 15u, loy, u) - py,
-        dqz = seat_place((sv >> (B + 8)) & 15u, loz, u) - pz;
-  return j != me && !(dot3(dqx, dqy, dqz, dqx, dqy, dqz) > u.reach2);
+// particle sits from its seat's place (a fifteenth of a cell and a step of
+// motion), so a seat near the edge of reach holds a particle beyond the link
+// test, and how the reach test rounds cannot change which pairs are summed.
+// Here it runs in integers on a lattice of 1/480 cell: the seat's place is
+// exactly 32 q + 16 from its cell's corner, the particle is rounded onto the
+// lattice (half a unit per axis at most) and the bound widened by that
+// (reachQ2), so the test keeps every seat the reference keeps and the few
+// more it admits are dropped by the link test.  Three shifts, three
+// multiply-adds and a compare, where the places in FP32 took a dozen.
+__device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int bz) {
+  int ex = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B, 4u) << 5) + bx;
+  int ey = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B + 4u, 4u) << 5) + by;
+  int ez = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B + 8u, 4u) << 5) + bz;
+  return __mul24(ex, ex) + __mul24(ey, ey) + __mul24(ez, ez);
 }
 
 __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict__ bxp, float touch2r,
@@ -725,7 +726,8 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
                                                    const float* __restrict__ ppSlot,
                                                    const float4* __restrict__ ppForce, float4* oPos, float4* oVel,
                                                    float4* oAux, float* oQ) {
-  __shared__ int cand[CAND_CAP][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts
+  __shared__ int cand[CAND_CAP + 1][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts;
+                                                  // the last row takes the writes past the list's end
   int lane = threadIdx.x;
   int me = xcd_block() * blockDim.x + lane;
   if (me >= u.N) return;
@@ -735,9 +737,18 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   int cx = (int)floorf(px * u.invCell), cy = (int)floorf(py * u.invCell), cz = (int)floorf(pz * u.invCell);
   float c0x = (float)cx * u.cell, c0y = (float)cy * u.cell, c0z = (float)cz * u.cell;
   // A: candidates in order.  A z-plane's nine cells have their seat loads
-  // issued together (each under the cell's cull test), then are walked in order.
+  // issued together (each under the cell's cull test), then are walked in
+  // order.  Every seat's index is written at the list's end and kept by
+  // counting it when it passes; a wavefront leaves a cell at the first seat
+  // that is empty for all its lanes.
+  const float qScale = (float)SEAT_Q / u.cell;
+  // (the particle lies in its own cell, 0..480 on the lattice; the clamp only tells the compiler the range)
+  const int pqx = min(max((int)rintf((px - c0x) * qScale), -1024), 1024),
+            pqy = min(max((int)rintf((py - c0y) * qScale), -1024), 1024),
+            pqz = min(max((int)rintf((pz - c0z) * qScale), -1024), 1024);
+  const int B = u.grid.IB;
+  const uint32_t idxMask = (1u << B) - 1u;
   int n = 0;
-  bool over = false;
   for (int dz = -1; dz <= 1; dz++) {
     uint4 sa[9], sb[9];
     bool ok[9];
@@ -757,25 +768,26 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
         sb[i] = s4[1];
       }
     }
+    const int bz = SEAT_Q * dz + 16 - pqz;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      if (!ok[i]) continue;
-      int dy = i / 3 - 1, dx = i % 3 - 1;
-      float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
+      if (__ballot(ok[i]) == 0ull) continue;
+      const int bx = SEAT_Q * (i % 3 - 1) + 16 - pqx, by = SEAT_Q * (i / 3 - 1) + 16 - pqy;
       uint32_t sv8[8] = { sa[i].x, sb[i].x, sa[i].y, sb[i].y, sa[i].z, sb[i].z, sa[i].w, sb[i].w };
-      bool open = true;
+      bool open = ok[i];
 #pragma unroll
       for (int k = 0; k < SLOTS; k++) {
         uint32_t sv = sv8[k];
-        open = open && !(sv & 0x80000000u);
-        if (open && seat_passes(sv, me, lox, loy, loz, px, py, pz, u)) {
-          if (n < CAND_CAP) cand[n][lane] = (int)(sv & ((1u << u.grid.IB) - 1u));
-          else over = true;
-          n++;
-        }
+        open = open & ((int)sv >= 0);
+        if (__ballot(open) == 0ull) break;
+        int j = (int)(sv & idxMask);
+        bool pass = open & (j != me) & (seat_d2(sv, B, bx, by, bz) <= u.reachQ2);
+        cand[min(n, CAND_CAP)][lane] = j;
+        n += (int)pass;
       }
     }
   }
+  const bool over = n > CAND_CAP;
   // the particle's own velocity, temperature and mass, needed from the contacts on
   float4 V = vel[me];
   float ri = mat[me];
@@ -826,11 +838,12 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
             qz = gclamp(pz, loz, loz + u.cell) - pz;
       if (dot3(qx, qy, qz, qx, qy, qz) > u.link2) continue;
       int h = hash_cell(u.grid, cx + dx, cy + dy, cz + dz);
+      const int bx = SEAT_Q * dx + 16 - pqx, by = SEAT_Q * dy + 16 - pqy, bz = SEAT_Q * dz + 16 - pqz;
       for (int k = 0; k < SLOTS; k++) {
         uint32_t sv = seats[(size_t)h * SLOTS + ((k & 1) ? 4 : 0) + (k >> 1)];
         if (sv & 0x80000000u) break;
-        if (!seat_passes(sv, me, lox, loy, loz, px, py, pz, u)) continue;
-        int j = (int)(sv & ((1u << u.grid.IB) - 1u));
+        int j = (int)(sv & idxMask);
+        if (j == me || seat_d2(sv, B, bx, by, bz) > u.reachQ2) continue;
         float4 Pj = pos[j];
         float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
         float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
