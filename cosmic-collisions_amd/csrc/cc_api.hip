@@ -112,6 +112,8 @@ static void grid_make(int G, Grid& g) {   // src/gl.js:73-83
   g.SXMask = g.SX - 1;
   g.AW = g.SX * G;
   g.AH = (G / g.SX) * G;
+  g.AWShift = G == 128 ? 11 : 9;
+  g.cells = g.AW * g.AH;
 }
 
 static int check_params(cc_ctx* c, const cc_params* p) {
@@ -197,12 +199,12 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       ALLOC(c->pos[b], N * 16); ALLOC(c->vel[b], N * 16); ALLOC(c->aux[b], N * 16); ALLOC(c->q[b], N * 4);
     }
     ALLOC(c->mat, N * 4); ALLOC(c->matTmp, N * 4);
-    ALLOC(c->gb.seats, c->cells * SLOTS * 4);
+    ALLOC(c->gb.seats, (c->cells + 1) * SLOTS * 4);
     ALLOC(c->gb.cnt, c->cells * 4);
     ALLOC(c->gb.bucket, c->cells * BUCKET_CAP * 4);
     ALLOC(c->gb.occ[0], N * 4); ALLOC(c->gb.occ[1], N * 4);
     ALLOC(c->gb.segN, 2 * ((N + 255) / 256) * 4);
-    ALLOC(c->gb.ovf, N * 8); ALLOC(c->gb.ovfN, 4);
+    ALLOC(c->gb.ovf, N * 8); ALLOC(c->gb.ovfN, 8);
     ALLOC(c->gb.word, N * 4);
     for (int b = 0; b < 2; b++) {
       MeshBuf& m = c->mesh[b];
@@ -214,7 +216,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     }
     ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
-    ALLOC(c->ppPartial, (size_t)4 * (PP_CAP / 4 / 256) * PP_CAP * 16);
+    ALLOC(c->ppPartial, pp_partial_bytes());
     ALLOC(c->diag, N * 16);
     ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
     ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4);
@@ -223,10 +225,10 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     ALLOC(c->boxSched, (size_t)SCHED_CAP * sizeof(Box));
     HIPCHK(c, hipHostMalloc((void**)&c->boxHost, (size_t)SCHED_CAP * sizeof(Box), hipHostMallocDefault));
     // the empty state: seats empty, counters and accumulators zero, meshes zero
-    launch_fill_u32(c->gb.seats, c->cells * SLOTS, EMPTY_SEAT, c->s);
+    launch_fill_u32(c->gb.seats, (c->cells + 1) * SLOTS, EMPTY_SEAT, c->s);
     HIPCHK(c, hipMemsetAsync(c->gb.cnt, 0, c->cells * 4, c->s));
     HIPCHK(c, hipMemsetAsync(c->gb.segN, 0, 2 * ((N + 255) / 256) * 4, c->s));
-    HIPCHK(c, hipMemsetAsync(c->gb.ovfN, 0, 4, c->s));
+    HIPCHK(c, hipMemsetAsync(c->gb.ovfN, 0, 8, c->s));
     for (int b = 0; b < 2; b++) {
       MeshBuf& m = c->mesh[b];
       HIPCHK(c, hipMemsetAsync(m.acc, 0, (size_t)FINE_TEXELS * 32, c->s));

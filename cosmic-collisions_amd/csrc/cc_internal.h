@@ -43,6 +43,8 @@ constexpr double SCR_SCALE = 16777216.0;          // 2^24
 
 struct Grid {
   int G, GShift, GMask, SX, SXShift, SXMask, AW, AH;
+  int AWShift;          // log2 AW (the atlas is G x SX wide, both powers of two)
+  int cells;            // AW x AH; the seat array holds one more cell, always empty, after them
   // bits of the particle index in a seat word: 18 as the reference packs it
   // (idx | q << 18 | q << 22 | q << 26 | 1 << 30, src/glsl-sim.js:103-110,
   // :143-144), 19 above 262,144 particles, where the place fields move up one
@@ -89,13 +91,13 @@ struct MeshBuf {
 };
 
 struct GridBuf {
-  uint32_t* seats;           // [cells*8]
+  uint32_t* seats;           // [(cells + 1) * 8]: the last cell stays empty (a safe target for skipped loads)
   int* cnt;                  // [cells]
   int* bucket;               // [cells*BUCKET_CAP]
   int* occ[2];               // occupied cells, two lists used in turn, in 256-entry segments (one per work-group)
   int* segN;                 // [2][blocks]: entries of each segment
   int2* ovf;                 // (cell, particle) past a cell's bucket
-  int* ovfN;
+  int* ovfN;                 // [2]: the overflow count of each rebuild parity
   uint32_t* word;            // [N] seat word of each particle
   int cur;                   // which occupied list is the current one
 };
@@ -115,6 +117,8 @@ void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s);
 void launch_far(const float4* coarse, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, hipStream_t s);
+// the pairwise pass's workspace: each block pair's sums for both of its blocks
+size_t pp_partial_bytes();
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm,
                float cut, float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force,
                hipStream_t s);

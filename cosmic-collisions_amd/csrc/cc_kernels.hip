@@ -80,7 +80,7 @@ __device__ __forceinline__ int hash_cell(const Grid& g, int cx, int cy, int cz) 
   cx &= g.GMask; cy &= g.GMask; cz &= g.GMask;
   int ax = shl(cz & g.SXMask, g.GShift) + cx;
   int ay = shl(cz >> g.SXShift, g.GShift) + cy;
-  return ay * g.AW + ax;
+  return (ay << g.AWShift) + ax;
 }
 
 // src/glsl-sim.js:25-37 boxOf
@@ -137,28 +137,28 @@ void launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
 }
 
 // ----------------------------------------------------- contact grid (A1-A3) --
-// Seats of the previous rebuild's occupied cells back to empty; the counters
-// of the new rebuild to zero.  Every other cell's seats are empty already.
-// The seats of the cells the last rebuild listed back to empty.  The lists
-// are per work-group segments (segment b: occ[b*256 ...], segN[b] entries),
-// so no kernel of the rebuild appends through one global counter.
-__global__ void k_slots_clear(uint32_t* seats, const int* __restrict__ occPrev, const int* __restrict__ segPrev,
-                              int N, int* ovfN) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) *ovfN = 0;
-  if (t >= N || (t & 255) >= segPrev[t >> 8]) return;
-  uint4 e = make_uint4(EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT);
-  uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)occPrev[t] * SLOTS);
-  sv[0] = e; sv[1] = e;
-}
-
-// Per particle: its hashed cell, its seat word (index, place in the cell to a
-// fifteenth, bit 30; src/glsl-sim.js:133-147), an arrival in the cell's bucket.
+// One pass of a rebuild, two independent jobs (they touch disjoint arrays):
+//  - the seats of the cells the last rebuild listed back to empty (its lists
+//    are per work-group segments: segment b is occ[b*256 ...], segN[b]
+//    entries), so every cell's seats are empty but the new ones';
+//  - per particle: its hashed cell, its seat word (index, place in the cell
+//    to a fifteenth, bit 30; src/glsl-sim.js:133-147), an arrival in the
+//    cell's bucket, the cell listed in the work-group's segment on its first
+//    arrival.  Arrivals past a bucket go to the overflow list, whose counter
+//    (one of two used in turn) the previous rebuild's seat pass set to zero.
 __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ pos, int N, float invCell, Grid g,
                                                      int* cnt, int* bucket, int* occ, int* segN, int2* ovf, int* ovfN,
-                                                     uint32_t* word) {
+                                                     uint32_t* word, uint32_t* seats,
+                                                     const int* __restrict__ occPrev,
+                                                     const int* __restrict__ segPrev) {
   __shared__ int listed;
   if (threadIdx.x == 0) listed = 0;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N && (t & 255) < segPrev[t >> 8]) {
+    uint4 e = make_uint4(EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT);
+    uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)occPrev[t] * SLOTS);
+    sv[0] = e; sv[1] = e;
+  }
   __syncthreads();
   int b = xcd_block();
   int i = b * blockDim.x + threadIdx.x;
@@ -189,9 +189,10 @@ __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ 
 __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN, int N,
                                                     int* cnt,
                                                     const int* __restrict__ bucket, const int2* __restrict__ ovf,
-                                                    const int* ovfN, const uint32_t* __restrict__ word,
-                                                    uint32_t* seats) {
+                                                    const int* ovfN, int* ovfNOther,
+                                                    const uint32_t* __restrict__ word, uint32_t* seats) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *ovfNOther = 0;   // the overflow counter of the next rebuild
   if (t >= N || (t & 255) >= segN[t >> 8]) return;
   int h = occ[t];
   int n = cnt[h];
@@ -234,11 +235,10 @@ void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBu
   int prev = gb.cur, next = gb.cur ^ 1, blocks = (N + 255) / 256;
   int* segPrev = gb.segN + (size_t)prev * blocks;
   int* segNext = gb.segN + (size_t)next * blocks;
-  hipLaunchKernelGGL(k_slots_clear, dim3(blocks), dim3(256), 0, s, gb.seats, gb.occ[prev], segPrev, N, gb.ovfN);
   hipLaunchKernelGGL(k_slots_count, dim3(blocks), dim3(256), 0, s, pos, N, invCell, g, gb.cnt, gb.bucket,
-                     gb.occ[next], segNext, gb.ovf, gb.ovfN, gb.word);
+                     gb.occ[next], segNext, gb.ovf, gb.ovfN + next, gb.word, gb.seats, gb.occ[prev], segPrev);
   hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, gb.occ[next], segNext, N, gb.cnt, gb.bucket,
-                     gb.ovf, gb.ovfN, gb.word, gb.seats);
+                     gb.ovf, gb.ovfN + next, gb.ovfN + prev, gb.word, gb.seats);
   gb.cur = next;
 }
 
@@ -613,6 +613,8 @@ __global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ parti
   force[slice * PP_CAP + me] = s;
 }
 
+size_t pp_partial_bytes() { return (size_t)4 * PP_MAXCHUNK * PP_CAP * sizeof(float4); }
+
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm, float cut,
                float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force, hipStream_t s) {
   hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, b, index, list, box);
@@ -710,6 +712,21 @@ __device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, 
 // (reachQ2), so the test keeps every seat the reference keeps and the few
 // more it admits are dropped by the link test.  Three shifts, three
 // multiply-adds and a compare, where the places in FP32 took a dozen.
+// Buffer-resource loads (a 32-bit byte offset from a base in scalar
+// registers, no 64-bit address arithmetic per load); gfx9's descriptor word 3
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint4 bload_u4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ float4 bload_f4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ float bload_f1(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 __device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int bz) {
   int ex = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B, 4u) << 5) + bx;
   int ey = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B + 4u, 4u) << 5) + by;
@@ -741,6 +758,9 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   // order.  Every seat's index is written at the list's end and kept by
   // counting it when it passes; a wavefront leaves a cell at the first seat
   // that is empty for all its lanes.
+  const __amdgpu_buffer_rsrc_t seatR = buf(seats, (uint32_t)(u.grid.cells + 1) * (SLOTS * 4));
+  const __amdgpu_buffer_rsrc_t posR = buf(pos, (uint32_t)u.N * 16), velR = buf(vel, (uint32_t)u.N * 16),
+                               matR = buf(mat, (uint32_t)u.N * 4);
   const float qScale = (float)SEAT_Q / u.cell;
   // (the particle lies in its own cell, 0..480 on the lattice; the clamp only tells the compiler the range)
   const int pqx = min(max((int)rintf((px - c0x) * qScale), -1024), 1024),
@@ -759,14 +779,10 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
       float qx = gclamp(px, lox, lox + u.cell) - px, qy = gclamp(py, loy, loy + u.cell) - py,
             qz = gclamp(pz, loz, loz + u.cell) - pz;
       ok[i] = !(dot3(qx, qy, qz, qx, qy, qz) > u.link2);
-      sa[i] = make_uint4(0x80000000u, 0, 0, 0);
-      sb[i] = sa[i];
-      if (ok[i]) {
-        const uint4* s4 = reinterpret_cast<const uint4*>(seats + (size_t)hash_cell(u.grid, cx + dx, cy + dy, cz + dz) *
-                                                                   SLOTS);
-        sa[i] = s4[0];
-        sb[i] = s4[1];
-      }
+      // a culled cell reads the always-empty cell past the grid: no branch, and its lanes' addresses coincide
+      uint32_t off = (uint32_t)(ok[i] ? hash_cell(u.grid, cx + dx, cy + dy, cz + dz) : u.grid.cells) * (SLOTS * 4);
+      sa[i] = bload_u4(seatR, off);
+      sb[i] = bload_u4(seatR, off + 16);
     }
     const int bz = SEAT_Q * dz + 16 - pqz;
 #pragma unroll
@@ -802,7 +818,7 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
 #pragma unroll
       for (int k = 0; k < 8; k++) {
         jj[k] = t0 + k < n ? cand[t0 + k][lane] : me;
-        Pj[k] = pos[jj[k]];
+        Pj[k] = bload_f4(posR, (uint32_t)jj[k] << 4);
       }
 #pragma unroll
       for (int k = 0; k < 8; k++) {
@@ -821,8 +837,8 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
       float rj[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        int j = t0 + k < nt ? cand[t0 + k][lane] : me;
-        Pj[k] = pos[j]; Vj[k] = vel[j]; rj[k] = mat[j];
+        uint32_t j = (uint32_t)(t0 + k < nt ? cand[t0 + k][lane] : me);
+        Pj[k] = bload_f4(posR, j << 4); Vj[k] = bload_f4(velR, j << 4); rj[k] = bload_f1(matR, j << 2);
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
