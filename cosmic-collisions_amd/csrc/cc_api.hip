@@ -210,6 +210,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       MeshBuf& m = c->mesh[b];
       ALLOC(m.acc, (size_t)FINE_TEXELS * 4 * 8);
       ALLOC(m.fine, (size_t)FINE_TEXELS * 16); ALLOC(m.force, (size_t)FINE_TEXELS * 16);
+      ALLOC(m.com, (size_t)FINE_TEXELS * 16); ALLOC(m.ccom, 4096 * 16);
       ALLOC(m.coarse, 4096 * 16); ALLOC(m.farA, 4096 * 16); ALLOC(m.farT0, 4096 * 16); ALLOC(m.farT1, 4096 * 16);
       ALLOC(m.mono, 16);
       ALLOC(m.occ, (size_t)SCRATCH * 4); ALLOC(m.occN, 4);
@@ -233,6 +234,8 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       MeshBuf& m = c->mesh[b];
       HIPCHK(c, hipMemsetAsync(m.acc, 0, (size_t)FINE_TEXELS * 32, c->s));
       HIPCHK(c, hipMemsetAsync(m.fine, 0, (size_t)FINE_TEXELS * 16, c->s));
+      HIPCHK(c, hipMemsetAsync(m.com, 0, (size_t)FINE_TEXELS * 16, c->s));
+      HIPCHK(c, hipMemsetAsync(m.ccom, 0, 4096 * 16, c->s));
       HIPCHK(c, hipMemsetAsync(m.force, 0, (size_t)FINE_TEXELS * 16, c->s));
       for (float4* t : { m.coarse, m.farA, m.farT0, m.farT1 }) HIPCHK(c, hipMemsetAsync(t, 0, 4096 * 16, c->s));
       HIPCHK(c, hipMemsetAsync(m.mono, 0, 16, c->s));
@@ -354,6 +357,12 @@ int cc_write_mesh(cc_ctx* c, int box, int which, const float* in) {
   if (!p) FAIL(c, CC_EINVAL, "unknown mesh texture %d", which);
   HIPCHK(c, hipMemcpyAsync(p, in, bytes, hipMemcpyHostToDevice, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
+  if (which == 0 || which == 1) {   // a written fine or coarse mesh: its centres of mass as a deposit would leave them
+    MeshBuf& m = c->mesh[box];
+    if (which == 0) launch_centres(m.fine, m.com, FINE_TEXELS, c->s);
+    else launch_centres(m.coarse, m.ccom, 4096, c->s);
+    LAUNCHCHK(c);
+  }
   if (which == 0) {   // a written fine mesh: its occupied list as a deposit would leave it
     std::vector<int> occ;
     const float* f = in;
@@ -418,13 +427,13 @@ static int mesh_all(cc_ctx* c, int stepIdx = -1) {
       launch_cell(m, eps2, gm, c->s);
     } else if (stepIdx % c->ffJudge == 0) {
       float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
-      launch_far(m.coarse, m.farA, m.farT0, m.farT1, nullptr, c->s);
+      launch_far(m, m.farA, m.farT0, m.farT1, nullptr, c->s);
       launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->s);
       ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, A, T0, T1, nullptr, c->s);
       launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->s);
       launch_judge(c->ffForce, m.force, c->ffSums + 2 * b, c->ffTol, c->ffStats + 4 * b, c->ffGate + b, c->s);
     } else {
-      launch_far(m.coarse, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
+      launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
       ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
       launch_count_learned(c->ffGate + b, c->ffStats + 4 * b, c->s);
       launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->s);

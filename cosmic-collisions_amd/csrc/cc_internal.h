@@ -80,8 +80,10 @@ struct CoupleU {        // COUPLE_FS's uniforms (chicxulub.html:260-276), FP32
 struct MeshBuf {
   unsigned long long* acc;   // int64 two's complement, [FINE_TEXELS*4]
   float4* fine;
+  float4* com;               // each fine texel's centre of mass and mass (fine.xyz / fine.w, fine.w)
   float4* force;
   float4* coarse;
+  float4* ccom;              // each coarse block's centre of mass and mass
   float4* farA;
   float4* farT0;
   float4* farT1;
@@ -114,7 +116,9 @@ void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, in
 void launch_coarse(MeshBuf& m, hipStream_t s);
 void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s);
 // the exact far field (skipped while *gate is nonzero) and the per-cell pass from given far-field buffers
-void launch_far(const float4* coarse, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
+void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
+// the centres of mass of a mesh written from outside: com[t] for mesh[t], t < n
+void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, hipStream_t s);
 // the pairwise pass's workspace: each block pair's sums for both of its blocks

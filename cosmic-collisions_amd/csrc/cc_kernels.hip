@@ -260,6 +260,7 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
                                                  const float4* __restrict__ mono, float eps2, unsigned long long* acc,
                                                  int* occN) {
   __shared__ unsigned long long tile[TILE * TILE * TILE * 4];
+  __shared__ unsigned long long stray[4];   // the work-group's strays, summed before one atomic per channel
   __shared__ int lo[3], hi[3];
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i == 0) *occN = 0;   // the occupied list k_deposit_finish appends to
@@ -278,6 +279,7 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
     cx = (int)fx; cy = (int)fy; cz = (int)fz;
   }
   if (threadIdx.x < 3) { lo[threadIdx.x] = 1 << 20; hi[threadIdx.x] = -(1 << 20); }
+  if (threadIdx.x < 4) stray[threadIdx.x] = 0ull;
   for (int t = threadIdx.x; t < TILE * TILE * TILE * 4; t += blockDim.x) tile[t] = 0ull;
   __syncthreads();
   if (dep) {
@@ -311,11 +313,12 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
     float rx = mo.x / gmax(mo.w, 1e-6f) - p.x, ry = mo.y / gmax(mo.w, 1e-6f) - p.y, rz = mo.z / gmax(mo.w, 1e-6f) - p.z;
     float inv = rsq(dot3(rx, ry, rz, rx, ry, rz) + eps2);
     float s = mr * inv * inv * inv;
-    unsigned long long* d = acc + (size_t)SCRATCH * 4;
-    atomicAdd(d, to_fixed(rx * s, SCR_SCALE)); atomicAdd(d + 1, to_fixed(ry * s, SCR_SCALE));
-    atomicAdd(d + 2, to_fixed(rz * s, SCR_SCALE)); atomicAdd(d + 3, to_fixed(mr, SCR_SCALE));
+    atomicAdd(&stray[0], to_fixed(rx * s, SCR_SCALE)); atomicAdd(&stray[1], to_fixed(ry * s, SCR_SCALE));
+    atomicAdd(&stray[2], to_fixed(rz * s, SCR_SCALE)); atomicAdd(&stray[3], to_fixed(mr, SCR_SCALE));
   }
   __syncthreads();
+  // integer sums: the same whatever order the work-groups' adds land in
+  if (threadIdx.x < 4 && stray[threadIdx.x] != 0ull) atomicAdd(acc + (size_t)SCRATCH * 4 + threadIdx.x, stray[threadIdx.x]);
   if (!tiled) return;
   for (int l = threadIdx.x; l < TILE * TILE * TILE; l += blockDim.x) {
     unsigned long long w = tile[l * 4 + 3], a0 = tile[l * 4], a1 = tile[l * 4 + 1], a2 = tile[l * 4 + 2];
@@ -327,8 +330,18 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
   }
 }
 
-// The accumulator to the FP32 mesh (and back to zero); the occupied texels listed.
-__global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc, float4* fine, int* occ, int* occN) {
+// A mesh texel's centre of mass (its mass-weighted sums over its mass, each
+// quotient as the cell and far-field passes of the reference take it) and its
+// mass: computed once per deposit, where every cell pass used to recompute it
+// for each of the ~100 texels that read it.
+__device__ __forceinline__ float4 centre_of(float4 v) {
+  return v.w > 0.0f ? make_float4(v.x / v.w, v.y / v.w, v.z / v.w, v.w) : make_float4(0.0f, 0.0f, 0.0f, v.w);
+}
+
+// The accumulator to the FP32 mesh (and back to zero), each texel's centre of
+// mass; the occupied texels listed.
+__global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc, float4* fine, float4* com, int* occ,
+                                                        int* occN) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= FINE_TEXELS) return;
   unsigned long long* a = acc + (size_t)t * 4;
@@ -337,7 +350,16 @@ __global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc,
                          (float)((double)(long long)a[2] * sc), (float)((double)(long long)a[3] * sc));
   a[0] = 0; a[1] = 0; a[2] = 0; a[3] = 0;
   fine[t] = v;
+  com[t] = centre_of(v);
   if (t < SCRATCH && v.w > 0.0f) occ[atomicAdd(occN, 1)] = t;
+}
+// the centres of a mesh written from outside (fine or coarse)
+__global__ void __launch_bounds__(256) k_centres(const float4* __restrict__ mesh, float4* com, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) com[t] = centre_of(mesh[t]);
+}
+void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_centres, dim3((n + 255) / 256), dim3(256), 0, s, mesh, com, n);
 }
 
 
@@ -345,7 +367,8 @@ void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, in
                     hipStream_t s) {
   int blocks = (N + 255) / 256;
   hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, b, box, m.mono, eps2, m.acc, m.occN);
-  hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.occ, m.occN);
+  hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.com, m.occ,
+                     m.occN);
 }
 
 // A plain copy of n float4 (read once, written once): the calibration of the
@@ -373,7 +396,7 @@ void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s) {
 }
 
 // ------------------------------------------------------ coarse + mono (A5) --
-__global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine, float4* coarse) {
+__global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine, float4* coarse, float4* ccom) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4096) return;
   int mx = t & 63, my = t >> 6;
@@ -384,6 +407,7 @@ __global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine,
     s.x += f.x; s.y += f.y; s.z += f.z; s.w += f.w;
   }
   coarse[t] = s;
+  ccom[t] = centre_of(s);
 }
 
 __global__ void __launch_bounds__(256) k_mono(const float4* __restrict__ coarse, float4* mono) {
@@ -405,7 +429,7 @@ __global__ void __launch_bounds__(256) k_mono(const float4* __restrict__ coarse,
 
 // -------------------------------------------------------- far field (A6) --
 // One wavefront per coarse block; its lanes share the 4096 source blocks.
-__global__ void __launch_bounds__(256) k_far(const float4* __restrict__ coarse, float4* farA, float4* farT0,
+__global__ void __launch_bounds__(256) k_far(const float4* __restrict__ ccom, float4* farA, float4* farT0,
                                              float4* farT1, const int* __restrict__ gate) {
   if (gate && *gate != 0) return;   // the learned far field is standing in
   int lane = threadIdx.x & 63;
@@ -413,22 +437,22 @@ __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ coarse, 
   if (me >= 4096) return;
   int mx = me & 63, my = me >> 6;
   int c0 = mx & 15, c1 = my & 15, c2 = (my >> 4) * 4 + (mx >> 4);
-  float4 mc = coarse[me];
+  float4 mc = ccom[me];
   if (mc.w <= 0.0f) {
     if (lane == 0) { farA[me] = make_float4(0, 0, 0, 0); farT0[me] = make_float4(0, 0, 0, 0); farT1[me] = make_float4(0, 0, 0, 0); }
     return;
   }
-  float x0 = mc.x / mc.w, x1 = mc.y / mc.w, x2 = mc.z / mc.w;
+  float x0 = mc.x, x1 = mc.y, x2 = mc.z;
   float a0 = 0, a1 = 0, a2 = 0, pot = 0, d0 = 0, d1 = 0, d2 = 0, o0 = 0, o1 = 0, o2 = 0;
   for (int k = lane; k < 4096; k += 64) {
     int kx = k & 15, ky = (k >> 4) & 15, kz = k >> 8;
     int dx = abs(kx - c0), dy = abs(ky - c1), dz = abs(kz - c2);
     int dm = max(dx, max(dy, dz));
     if (dm <= 2) continue;
-    float4 m = coarse[coarse_at(kx, ky, kz)];
+    float4 m = ccom[coarse_at(kx, ky, kz)];
     if (m.w <= 0.0f) continue;
-    float r0 = m.x / m.w - x0, r1 = m.y / m.w - x1, r2 = m.z / m.w - x2;
-    float inv2 = 1.0f / dot3(r0, r1, r2, r0, r1, r2), inv = sqrtf(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
+    float r0 = m.x - x0, r1 = m.y - x1, r2 = m.z - x2;
+    float inv2 = qdiv(1.0f, dot3(r0, r1, r2, r0, r1, r2)), inv = qsqrt(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
     float s = m.w * inv3;
     a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
     pot -= m.w * inv;
@@ -453,7 +477,8 @@ __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ coarse, 
 // One wavefront per occupied fine cell; lane l takes fine cell l of each of
 // the 5^3 surrounding coarse blocks; the far field comes from the block's
 // tidal tensor (src/glsl-sim.js:303-352).
-__global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, const float4* __restrict__ coarse,
+__global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, const float4* __restrict__ com,
+                                              const float4* __restrict__ coarse, const float4* __restrict__ ccom,
                                               const float4* __restrict__ farA, const float4* __restrict__ farT0,
                                               const float4* __restrict__ farT1, const int* __restrict__ occ,
                                               const int* occN, float eps2, float gm, float4* force) {
@@ -468,8 +493,8 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
     int me = occ[w];
     int mx = me & 511, my = me >> 9;
     int c0 = mx & 63, c1 = my & 63, c2 = (my >> 6) * 8 + (mx >> 6);
-    float4 mc = fine[me];
-    float x0 = mc.x / mc.w, x1 = mc.y / mc.w, x2 = mc.z / mc.w;
+    float4 mc = com[me];
+    float x0 = mc.x, x1 = mc.y, x2 = mc.z;
     int q0 = c0 >> 2, q1 = c1 >> 2, q2 = c2 >> 2;
     float a0 = 0, a1 = 0, a2 = 0, pot = 0;
     for (int kz = -2; kz <= 2; kz++) for (int ky = -2; ky <= 2; ky++) for (int kx = -2; kx <= 2; kx++) {
@@ -478,10 +503,10 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
       if (coarse[coarse_at(k0, k1, k2)].w <= 0.0f) continue;
       int f0 = k0 * 4 + fx, f1 = k1 * 4 + fy, f2 = k2 * 4 + fz;
       if (f0 == c0 && f1 == c1 && f2 == c2) continue;
-      float4 mf = fine[fine_at(f0, f1, f2)];
+      float4 mf = com[fine_at(f0, f1, f2)];
       if (mf.w <= 0.0f) continue;
-      float r0 = mf.x / mf.w - x0, r1 = mf.y / mf.w - x1, r2 = mf.z / mf.w - x2;
-      float inv = rsq(dot3(r0, r1, r2, r0, r1, r2) + eps2);
+      float r0 = mf.x - x0, r1 = mf.y - x1, r2 = mf.z - x2;
+      float inv = qdiv(1.0f, qsqrt(dot3(r0, r1, r2, r0, r1, r2) + eps2));
       float s = mf.w * inv * inv * inv;
       a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
       pot -= mf.w * inv;
@@ -489,8 +514,8 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
     a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); pot = wave_sum(pot);
     if (lane == 0) {
       int ca = coarse_at(q0, q1, q2);
-      float4 mb = coarse[ca], fa = farA[ca], t0 = farT0[ca], t1 = farT1[ca];
-      float d0 = x0 - mb.x / mb.w, d1 = x1 - mb.y / mb.w, d2 = x2 - mb.z / mb.w;
+      float4 mb = ccom[ca], fa = farA[ca], t0 = farT0[ca], t1 = farT1[ca];
+      float d0 = x0 - mb.x, d1 = x1 - mb.y, d2 = x2 - mb.z;
       float T0 = t0.x * d0 + t0.w * d1 + t1.x * d2;
       float T1 = t0.w * d0 + t0.y * d1 + t1.y * d2;
       float T2 = t1.x * d0 + t1.y * d1 + t0.z * d2;
@@ -502,20 +527,20 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
 }
 
 void launch_coarse(MeshBuf& m, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarse, dim3(16), dim3(256), 0, s, m.fine, m.coarse);
+  hipLaunchKernelGGL(k_coarse, dim3(16), dim3(256), 0, s, m.fine, m.coarse, m.ccom);
   hipLaunchKernelGGL(k_mono, dim3(1), dim3(256), 0, s, m.coarse, m.mono);
 }
-void launch_far(const float4* coarse, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
-  hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, coarse, farA, farT0, farT1, gate);
+void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, m.ccom, farA, farT0, farT1, gate);
 }
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, hipStream_t s) {
   launch_fill_f32(reinterpret_cast<float*>(force), (size_t)FINE_TEXELS * 4, 0.0f, s);
-  hipLaunchKernelGGL(k_cell, dim3(2048), dim3(256), 0, s, m.fine, m.coarse, farA, farT0, farT1, m.occ, m.occN, eps2,
-                     gm, force);
+  hipLaunchKernelGGL(k_cell, dim3(2048), dim3(256), 0, s, m.fine, m.com, m.coarse, m.ccom, farA, farT0, farT1, m.occ,
+                     m.occN, eps2, gm, force);
 }
 void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s) {
-  launch_far(m.coarse, m.farA, m.farT0, m.farT1, nullptr, s);
+  launch_far(m, m.farA, m.farT0, m.farT1, nullptr, s);
   launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, s);
 }
 
