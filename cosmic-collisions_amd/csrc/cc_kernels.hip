@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
                                                  int* occN) {
   __shared__ unsigned long long tile[TILE * TILE * TILE * 4];
   __shared__ unsigned long long stray[4];   // the work-group's strays, summed before one atomic per channel
-  __shared__ int lo[3], hi[3];
+  __shared__ int sum[4];                    // the deposited particles' cells, summed, and their count
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i == 0) *occN = 0;   // the occupied list k_deposit_finish appends to
   bool live = i < N;
@@ -278,18 +278,17 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
     tx = gx - fx; ty = gy - fy; tz = gz - fz;
     cx = (int)fx; cy = (int)fy; cz = (int)fz;
   }
-  if (threadIdx.x < 3) { lo[threadIdx.x] = 1 << 20; hi[threadIdx.x] = -(1 << 20); }
-  if (threadIdx.x < 4) stray[threadIdx.x] = 0ull;
+  if (threadIdx.x < 4) { sum[threadIdx.x] = 0; stray[threadIdx.x] = 0ull; }
   for (int t = threadIdx.x; t < TILE * TILE * TILE * 4; t += blockDim.x) tile[t] = 0ull;
   __syncthreads();
-  if (dep) {
-    atomicMin(&lo[0], cx); atomicMin(&lo[1], cy); atomicMin(&lo[2], cz);
-    atomicMax(&hi[0], cx); atomicMax(&hi[1], cy); atomicMax(&hi[2], cz);
-  }
+  if (dep) { atomicAdd(&sum[0], cx); atomicAdd(&sum[1], cy); atomicAdd(&sum[2], cz); atomicAdd(&sum[3], 1); }
   __syncthreads();
-  int ox = lo[0], oy = lo[1], oz = lo[2];
-  // corners span [lo, hi+1]: the tile holds them when that is within TILE
-  bool tiled = hi[0] + 1 - ox < TILE && hi[1] + 1 - oy < TILE && hi[2] + 1 - oz < TILE;
+  // The tile is centred on the work-group's mean cell; a particle whose corners
+  // fall outside it (an outlier of a work-group the motion has spread since the
+  // last Morton sort) adds to the mesh directly, the rest through the tile.
+  int nd = max(sum[3], 1);
+  int ox = sum[0] / nd - (TILE / 2 - 1), oy = sum[1] / nd - (TILE / 2 - 1), oz = sum[2] / nd - (TILE / 2 - 1);
+  bool tiled = dep && cx >= ox && cy >= oy && cz >= oz && cx + 1 < ox + TILE && cy + 1 < oy + TILE && cz + 1 < oz + TILE;
   if (dep) {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -319,7 +318,6 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
   __syncthreads();
   // integer sums: the same whatever order the work-groups' adds land in
   if (threadIdx.x < 4 && stray[threadIdx.x] != 0ull) atomicAdd(acc + (size_t)SCRATCH * 4 + threadIdx.x, stray[threadIdx.x]);
-  if (!tiled) return;
   for (int l = threadIdx.x; l < TILE * TILE * TILE; l += blockDim.x) {
     unsigned long long w = tile[l * 4 + 3], a0 = tile[l * 4], a1 = tile[l * 4 + 1], a2 = tile[l * 4 + 2];
     if ((w | a0 | a1 | a2) == 0ull) continue;
