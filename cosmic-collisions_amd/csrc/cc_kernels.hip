@@ -442,24 +442,35 @@ __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ ccom, fl
   }
   float x0 = mc.x, x1 = mc.y, x2 = mc.z;
   float a0 = 0, a1 = 0, a2 = 0, pot = 0, d0 = 0, d1 = 0, d2 = 0, o0 = 0, o1 = 0, o2 = 0;
-  for (int k = lane; k < 4096; k += 64) {
-    int kx = k & 15, ky = (k >> 4) & 15, kz = k >> 8;
-    int dx = abs(kx - c0), dy = abs(ky - c1), dz = abs(kz - c2);
-    int dm = max(dx, max(dy, dz));
-    if (dm <= 2) continue;
-    float4 m = ccom[coarse_at(kx, ky, kz)];
-    if (m.w <= 0.0f) continue;
-    float r0 = m.x - x0, r1 = m.y - x1, r2 = m.z - x2;
-    float inv2 = qdiv(1.0f, dot3(r0, r1, r2, r0, r1, r2)), inv = qsqrt(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
-    float s = m.w * inv3;
-    a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
-    pot -= m.w * inv;
-    d0 += m.w * (3.0f * r0 * r0 * inv5 - inv3);
-    d1 += m.w * (3.0f * r1 * r1 * inv5 - inv3);
-    d2 += m.w * (3.0f * r2 * r2 * inv5 - inv3);
-    o0 += m.w * (3.0f * (r0 * r1) * inv5);
-    o1 += m.w * (3.0f * (r0 * r2) * inv5);
-    o2 += m.w * (3.0f * (r1 * r2) * inv5);
+  // four source blocks' loads in flight at a time; each lane's sum in the same block order
+  for (int k4 = lane; k4 < 4096; k4 += 256) {
+    float4 mq[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int k = k4 + 64 * u;
+      mq[u] = ccom[coarse_at(k & 15, (k >> 4) & 15, k >> 8)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      int k = k4 + 64 * u;
+      int kx = k & 15, ky = (k >> 4) & 15, kz = k >> 8;
+      int dx = abs(kx - c0), dy = abs(ky - c1), dz = abs(kz - c2);
+      int dm = max(dx, max(dy, dz));
+      if (dm <= 2) continue;
+      float4 m = mq[u];
+      if (m.w <= 0.0f) continue;
+      float r0 = m.x - x0, r1 = m.y - x1, r2 = m.z - x2;
+      float inv2 = qdiv(1.0f, dot3(r0, r1, r2, r0, r1, r2)), inv = qsqrt(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
+      float s = m.w * inv3;
+      a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
+      pot -= m.w * inv;
+      d0 += m.w * (3.0f * r0 * r0 * inv5 - inv3);
+      d1 += m.w * (3.0f * r1 * r1 * inv5 - inv3);
+      d2 += m.w * (3.0f * r2 * r2 * inv5 - inv3);
+      o0 += m.w * (3.0f * (r0 * r1) * inv5);
+      o1 += m.w * (3.0f * (r0 * r2) * inv5);
+      o2 += m.w * (3.0f * (r1 * r2) * inv5);
+    }
   }
   a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); pot = wave_sum(pot);
   d0 = wave_sum(d0); d1 = wave_sum(d1); d2 = wave_sum(d2);
@@ -495,10 +506,19 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
     float x0 = mc.x, x1 = mc.y, x2 = mc.z;
     int q0 = c0 >> 2, q1 = c1 >> 2, q2 = c2 >> 2;
     float a0 = 0, a1 = 0, a2 = 0, pot = 0;
-    for (int kz = -2; kz <= 2; kz++) for (int ky = -2; ky <= 2; ky++) for (int kx = -2; kx <= 2; kx++) {
-      int k0 = q0 + kx, k1 = q1 + ky, k2 = q2 + kz;
-      if (k0 < 0 || k1 < 0 || k2 < 0 || k0 > 15 || k1 > 15 || k2 > 15) continue;
-      if (coarse[coarse_at(k0, k1, k2)].w <= 0.0f) continue;
+    // The 5^3 coarse blocks around, b = (kz+2)*25 + (ky+2)*5 + (kx+2): which hold
+    // mass, read all at once (two blocks a lane) into two wavefront masks, so
+    // the walk below does not wait on one block's load before the next.
+    auto held = [&](int b) {
+      if (b >= 125) return false;
+      int k0 = q0 + b % 5 - 2, k1 = q1 + (b / 5) % 5 - 2, k2 = q2 + b / 25 - 2;
+      if (k0 < 0 || k1 < 0 || k2 < 0 || k0 > 15 || k1 > 15 || k2 > 15) return false;
+      return coarse[coarse_at(k0, k1, k2)].w > 0.0f;
+    };
+    const unsigned long long held0 = __ballot(held(lane)), held1 = __ballot(held(lane + 64));
+    for (int half = 0; half < 2; half++) for (unsigned long long m = half ? held1 : held0; m; m &= m - 1) {
+      int b = (__ffsll((long long)m) - 1) + 64 * half;
+      int k0 = q0 + b % 5 - 2, k1 = q1 + (b / 5) % 5 - 2, k2 = q2 + b / 25 - 2;
       int f0 = k0 * 4 + fx, f1 = k1 * 4 + fy, f2 = k2 * 4 + fz;
       if (f0 == c0 && f1 == c1 && f2 == c2) continue;
       float4 mf = com[fine_at(f0, f1, f2)];
