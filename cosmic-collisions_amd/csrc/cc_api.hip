@@ -67,6 +67,9 @@ struct cc_ctx {
   float4* ffFar = nullptr;     // 3 x 4096: the network's farA, farT0, farT1
   float4* ffForce = nullptr;   // the force mesh from the network's far field (judge steps)
   int stepNo = 0;
+  // the pairwise pass of a mesh step runs on its own stream beside the seats and the mesh
+  hipStream_t s2 = nullptr;
+  hipEvent_t ppFork = nullptr, ppJoin = nullptr;
   FofWork* fof = nullptr;  // the analysis' workspace, made on its first call
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastMs = 0.0;
@@ -254,6 +257,9 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     LAUNCHCHK(c);
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ppFork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ppJoin, hipEventDisableTiming));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
   };
@@ -277,6 +283,10 @@ int cc_destroy(cc_ctx* c) {
   for (void* p : c->allocs) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->s2) hipStreamSynchronize(c->s2);
+  if (c->ppFork) hipEventDestroy(c->ppFork);
+  if (c->ppJoin) hipEventDestroy(c->ppJoin);
+  if (c->s2) hipStreamDestroy(c->s2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
   return 0;
@@ -505,6 +515,20 @@ int cc_sim(cc_ctx* c, const cc_sim_args* a) {
 static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   int n = c->stepNo++;
   int r;
+  bool mesh = !c->p.field && n % c->p.meshEvery == 0;
+  // The pairwise pass reads the positions, masses and boxes and writes only its
+  // own buffers, as the seats and the mesh do theirs: on a mesh step it runs on
+  // the second stream beside them (a fork and a join, which a captured cycle
+  // keeps as graph edges), and the step pass waits for all of them.
+  bool side = mesh && c->ppCount > 0;
+  if (side) {
+    HIPCHK(c, hipEventRecord(c->ppFork, c->s));
+    HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
+    launch_pp(c->pos[c->cur], c->mat, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+              (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppPartial, c->ppForce, c->s2);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
+  }
   if (n % 2 == 0 && (r = cc_slots(c))) return r;
   if (c->p.field) {   // constant gravity: no mesh, and the field acts on every step
     cc_sim_args b = *a;
@@ -513,9 +537,8 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
     b.gdt = c->p.dt;
     return sim_pass(c, &b);
   }
-  bool mesh = n % c->p.meshEvery == 0;
   if (mesh && (r = mesh_all(c, n))) return r;
-  if (mesh && (r = cc_pp(c))) return r;
+  if (side) HIPCHK(c, hipStreamWaitEvent(c->s, c->ppJoin, 0));
   cc_sim_args b = *a;
   b.settle = settle;
   b.kick = kick;
