@@ -153,7 +153,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--state-dir", default=os.path.join(tempfile.gettempdir(), "cc_twins_state"))
     ap.add_argument("--reuse-state", action="store_true", help="use a prepared state in --state-dir if there is one")
-    ap.add_argument("--batch", type=int, default=64, help="instances in the batched-sweep leg (0: skip it)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instances in the batched-sweep leg (0, the default: skip it; 64 is the sweep's size)")
     ap.add_argument("--batch-steps", type=int, default=240)
     ap.add_argument("--batch-dir", default=os.path.join(tempfile.gettempdir(), "cc_batch_state"))
     ap.add_argument("--runto", type=int, default=3600, help="steps of the engine-driven leg with analyses (0: skip)")

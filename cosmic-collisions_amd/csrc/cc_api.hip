@@ -276,6 +276,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
 int cc_destroy(cc_ctx* c) {
   if (!c) return 0;
   if (c->s) hipStreamSynchronize(c->s);
+  if (c->s2) hipStreamSynchronize(c->s2);   // both streams idle before their buffers go
   for (CycleGraph& g : c->graphs) if (g.exec) hipGraphExecDestroy(g.exec);
   if (c->ffWork) ffwork_destroy(c->ffWork);
   if (c->fof) fof_destroy(c->fof);
@@ -283,7 +284,6 @@ int cc_destroy(cc_ctx* c) {
   for (void* p : c->allocs) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->s2) hipStreamSynchronize(c->s2);
   if (c->ppFork) hipEventDestroy(c->ppFork);
   if (c->ppJoin) hipEventDestroy(c->ppJoin);
   if (c->s2) hipStreamDestroy(c->s2);
@@ -588,7 +588,11 @@ static int cycle_graph(cc_ctx* c, const cc_sim_args* a, CycleGraph** out) {
     if (g.exec && g.key == key) { *out = &g; return 0; }
   CycleGraph& g = c->graphs[c->nextGraph];
   c->nextGraph = (c->nextGraph + 1) % 4;
-  if (g.exec) { hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+  if (g.exec) {   // its last launch may still run: let it end first
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
   int stepNo = c->stepNo, cur = c->cur, gcur = c->gb.cur;
   HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
   launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
@@ -884,7 +888,11 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args) {
     key.insert(key.end(), k.begin(), k.end());
   }
   if (b->exec && key == b->key) return 0;
-  if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->exec) {   // its last launch may still run: let it end first
+    BCHK(b, hipStreamSynchronize(b->s));
+    hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
   size_t n = b->ctx.size();
   std::vector<int> stepNo(n), cur(n), gcur(n);
   for (size_t i = 0; i < n; i++) { stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; }

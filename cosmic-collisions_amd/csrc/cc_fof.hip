@@ -324,25 +324,30 @@ int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipS
   FofHostIn h{ N, reinterpret_cast<const F4*>(w->hP), reinterpret_cast<const F4*>(w->hV),
                reinterpret_cast<const F4*>(w->hD), w->hR, w->hLab, w->hE2, nT, in.m, in.a, in.spring, in.touch,
                in.cap, in.flags };
-  FofPick pick = fof_pick(h);
-  if (pick.replay) {
-    // every linked partner in the reference's order, for the host to replay its unions
-    launch_scan_excl(w->nLink, N + 1, w->blockSums, s);
-    FCHK(hipGetLastError());
-    FCHK(hipMemcpyAsync(w->hCounts + 1, w->nLink + N, 4, hipMemcpyDeviceToHost, s));
-    FCHK(hipStreamSynchronize(s));
-    const int nLk = w->hCounts[1];
-    if ((r = grow_dev((void**)&w->partner, w->partnerCap, (size_t)nLk, 4, err))) return r;
-    hipLaunchKernelGGL(k_fof_pairs<1>, dim3(blocks), dim3(256), 0, s, in.pos, N, w->cell, w->end, w->order, L2,
-                       touch2, w->parent, nullptr, nullptr, w->nLink, w->nTouch, w->partner, w->e2);
-    FCHK(hipGetLastError());
-    std::vector<int> off(N + 1), partner((size_t)nLk);
-    FCHK(hipMemcpyAsync(off.data(), w->nLink, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, s));
-    if (nLk) FCHK(hipMemcpyAsync(partner.data(), w->partner, (size_t)nLk * 4, hipMemcpyDeviceToHost, s));
-    FCHK(hipStreamSynchronize(s));
-    fof_pick_replay(h, off.data(), partner.data(), pick);
+  try {
+    FofPick pick = fof_pick(h);
+    if (pick.replay) {
+      // every linked partner in the reference's order, for the host to replay its unions
+      launch_scan_excl(w->nLink, N + 1, w->blockSums, s);
+      FCHK(hipGetLastError());
+      FCHK(hipMemcpyAsync(w->hCounts + 1, w->nLink + N, 4, hipMemcpyDeviceToHost, s));
+      FCHK(hipStreamSynchronize(s));
+      const int nLk = w->hCounts[1];
+      if ((r = grow_dev((void**)&w->partner, w->partnerCap, (size_t)nLk, 4, err))) return r;
+      hipLaunchKernelGGL(k_fof_pairs<1>, dim3(blocks), dim3(256), 0, s, in.pos, N, w->cell, w->end, w->order, L2,
+                         touch2, w->parent, nullptr, nullptr, w->nLink, w->nTouch, w->partner, w->e2);
+      FCHK(hipGetLastError());
+      std::vector<int> off(N + 1), partner((size_t)nLk);
+      FCHK(hipMemcpyAsync(off.data(), w->nLink, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, s));
+      if (nLk) FCHK(hipMemcpyAsync(partner.data(), w->partner, (size_t)nLk * 4, hipMemcpyDeviceToHost, s));
+      FCHK(hipStreamSynchronize(s));
+      fof_pick_replay(h, off.data(), partner.data(), pick);
+    }
+    fof_report(h, pick, out, list);
+  } catch (const std::exception& x) {
+    err = std::string("the analysis' host half: ") + x.what();
+    return CC_EDEVICE;
   }
-  fof_report(h, pick, out, list);
   out->ms[0] = msCells;
   out->ms[1] = msCopy;
   out->ms[2] = ms_since(tC);

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <thread>
 #include <vector>
@@ -287,19 +288,30 @@ void fof_pick_replay(const FofHostIn& h, const int* linkOff, const int* partner,
 }
 
 // the tasks at once, each but the last on a thread of its own; a task whose
-// thread cannot be had runs on the caller's, in turn
+// thread cannot be had runs on the caller's, in turn.  A task's exception
+// (an allocation that failed) is passed on to the caller once all have ended.
 static void run_together(std::vector<std::function<void()>>& tasks) {
+  std::vector<std::exception_ptr> err(tasks.size());
+  auto guarded = [&](size_t k) {
+    try {
+      tasks[k]();
+    } catch (...) {
+      err[k] = std::current_exception();
+    }
+  };
   std::vector<std::thread> th;
   size_t k = 0;
   for (; k + 1 < tasks.size(); k++) {
     try {
-      th.emplace_back(tasks[k]);
+      th.emplace_back(guarded, k);
     } catch (...) {
       break;
     }
   }
-  for (size_t r = k; r < tasks.size(); r++) tasks[r]();
+  for (size_t r = k; r < tasks.size(); r++) guarded(r);
   for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 void fof_report(const FofHostIn& h, const FofPick& k, cc_report* out, int32_t* list) {
