@@ -40,6 +40,20 @@ __device__ __forceinline__ float qdiv(float a, float b) {
   r = fmaf(-b, q, a);
   return fmaf(r, y, q);
 }
+// a / b correctly rounded for a divisor whose correctly rounded reciprocal y
+// is at hand (a uniform divisor): one residual correction of a * y, exact by
+// Markstein's theorem (y within half an ulp of 1/b and a * y within one ulp of
+// a / b make q + (a - b q) y round to a / b) while the residual a - b q does
+// not underflow, i.e. for dividends above about 2^-100.  Checked exhaustively
+// on the CPU (tests/markstein_check.c) for the pairwise pass's two divisors
+// and every dividend in [2^-80, 1] and 0; its dividends are a distance (0 or
+// above 2^-75: the root of a sum of squares) and the cut minus a distance (0
+// or above 2^-24).
+__device__ __forceinline__ float qdiv_by(float a, float b, float y) {
+  float q = a * y;
+  float r = fmaf(-b, q, a);
+  return fmaf(r, y, q);
+}
 // The correctly rounded square root of a normal x > 0 (or of 0) without the
 // scaling for tiny x: the hardware root, then the neighbour above or below
 // where its residual says the root lies.  Checked bit for bit against the
@@ -583,63 +597,155 @@ __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
 }
 
 // Each reference slice (a quarter of the list, src/glsl-sim.js:712-754) is cut
-// into chunks of PP_CHUNK entries: a thread sums one seat against one chunk,
-// staged through LDS, and k_pp_sum adds a slice's chunks in order.  Four
-// slices alone would be 512 wavefronts; chunks make it up to 4096.
+// into ranges of PP_CHUNK entries: a seat's sum over one range, in the
+// reference's order, is one partial, and k_pp_sum adds a slice's partials in
+// order.  The 32 ranges (4 slices x 8) pair up into 496 work-groups, one per
+// pair of ranges A < B, and 128 for the ranges with themselves, each taking a
+// quarter of the range's rows (these have no column sums, so their rows part
+// freely; 624 work-groups of about 2.1 pairs' work per CU where 528 equal ones
+// left some CUs three).  A pair's terms for its two ends differ only in the
+// mass they are scaled by and the sign of the direction (the distance, the
+// kernel, the taper and the box test are the same numbers), so the work-group
+// for A < B evaluates each pair (i in A, j in B) once and sums it into both
+// i's partial over B and j's partial over A.  Both sums keep the reference's
+// order: the pairs are taken in 32 x 32 tiles, A's tile outer and B's inner,
+// each tile's terms set out in LDS, and a lane per (particle, component) adds a
+// tile's 32 terms in order to its running sum -- i's over B's tiles in turn,
+// j's (kept in LDS across the outer loop) over A's tiles in turn.  A work-group
+// with A = B evaluates its pairs in both directions, as the reference does.
 constexpr int PP_CHUNK = 256;
 constexpr int PP_MAXCHUNK = PP_CAP / 4 / PP_CHUNK;   // 8
-__global__ void __launch_bounds__(256) k_pp_force(const float4* __restrict__ list, const float* __restrict__ box,
+constexpr int PP_RANGES = 4 * PP_MAXCHUNK;           // 32
+constexpr int PP_TILE = 32;
+constexpr int PP_PAD = PP_TILE + 1;                  // float4 row stride of a term tile (no bank conflicts)
+constexpr int PP_PAIRS = PP_RANGES * (PP_RANGES - 1) / 2;   // 496 range pairs A < B
+
+// range g = slice * PP_MAXCHUNK + chunk of a list of count seats: [lo, hi), empty when lo >= hi
+__device__ __forceinline__ void pp_range(int g, int count, int& lo, int& hi) {
+  int share = (count + 3) >> 2, s = g / PP_MAXCHUNK, c = g % PP_MAXCHUNK;
+  int j0 = s * share, j1 = min(j0 + share, count);
+  lo = j0 + c * PP_CHUNK;
+  hi = min(lo + PP_CHUNK, j1);
+}
+
+// One pair's terms, each operation the reference's (src/glsl-sim.js:712-754):
+// for the seat at P (mass mi, box bi) from the entry at Q (mass Q.w, box bj),
+// and for Q's seat from P; zero when the pair is beyond the cut.
+__device__ __forceinline__ void pp_terms(float4 P, float bi, float4 Q, float bj, float cut, float cut2, float a2, float a,
+                                         float yCut, float yTap, float tapD, const float2* st, float4& ti, float4& tj) {
+  ti = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  tj = ti;
+  float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
+  float r2 = dot3(d0, d1, d2, d0, d1, d2);
+  if (r2 >= cut2) return;
+  float r = qsqrt(r2);
+  float x = gclamp(qdiv_by(r, cut, yCut) * 64.0f - 0.5f, 0.0f, 63.0f);
+  int k = (int)x, k1 = min(k + 1, 63);
+  float t = x - (float)k;
+  float invN = qdiv(1.0f, qsqrt(r2 + a2));
+  // smoothstep(cut, 0.8 cut, r): (r - cut) / (0.8 cut - cut), both signs turned (exactly) to divide by a positive
+  float taper = gclamp(qdiv_by(cut - r, tapD, yTap), 0.0f, 1.0f);
+  taper = taper * taper * (3.0f - 2.0f * taper);
+  float same = (bi >= 0.0f && bj == bi) ? 1.0f : 0.0f;
+  float2 tk = st[k], tk1 = st[k1];
+  float Tx = gmix(tk.x, tk1.x, t) * same, Ty = gmix(tk.y, tk1.y, t) * same;
+  float fn = invN * invN * invN * r - qdiv(Tx, gmax(r2, a2));
+  float rr = gmax(r, 1e-6f);
+  float n0 = qdiv(d0, rr), n1 = qdiv(d1, rr), n2 = qdiv(d2, rr);
+  float ph = -invN - qdiv(Ty, gmax(r, a));
+  float si = Q.w * fn * taper, sj = P.w * fn * taper;
+  ti = make_float4(n0 * si, n1 * si, n2 * si, Q.w * ph * taper);
+  tj = make_float4(-(n0 * sj), -(n1 * sj), -(n2 * sj), P.w * ph * taper);
+}
+
+__global__ void __launch_bounds__(256) k_pp_pairs(const float4* __restrict__ list, const float* __restrict__ box,
                                                   int count, float cut, float a2, const float2* __restrict__ tab,
                                                   float4* partial) {
-  __shared__ float4 sl[PP_CHUNK];
-  __shared__ float sb[PP_CHUNK];
+  __shared__ float4 tRow[PP_TILE * PP_PAD];   // [i][j]: the A side's terms of a tile
+  __shared__ float4 tCol[PP_TILE * PP_PAD];   // [j][i]: the B side's terms
+  __shared__ float4 lB[PP_CHUNK];
+  __shared__ float bB[PP_CHUNK];
+  __shared__ float colSum[PP_CHUNK * 4];      // B's running sums, [j][component]
   __shared__ float2 st[64];
-  int slice = blockIdx.y, chunk = blockIdx.z;
-  int me = blockIdx.x * blockDim.x + threadIdx.x;
-  if (threadIdx.x < 64) st[threadIdx.x] = tab[threadIdx.x];
-  int share = (count + 3) >> 2, j0 = slice * share, j1 = min(j0 + share, count);
-  int c0 = j0 + chunk * PP_CHUNK, c1 = min(c0 + PP_CHUNK, j1);
-  float4* out = partial + ((size_t)slice * PP_MAXCHUNK + chunk) * PP_CAP;
-  if (c0 >= c1) {   // past the slice's end: an empty partial
-    if (me < PP_CAP) out[me] = make_float4(0, 0, 0, 0);
-    return;
+  // work-groups 0..495: the range pairs A < B; then four per range A = B, each a quarter of its rows
+  int id = blockIdx.x, gA = 0, gB, quarter = -1;
+  if (id < PP_PAIRS) {
+    while (id >= PP_RANGES - 1 - gA) { id -= PP_RANGES - 1 - gA; gA++; }
+    gB = gA + 1 + id;
+  } else {
+    id -= PP_PAIRS;
+    gA = gB = id >> 2;
+    quarter = id & 3;
   }
-  float4 P = me < PP_CAP ? list[me] : make_float4(0, 0, 0, 0);
-  bool active = me < count && P.w != 0.0f;
-  float myBox = me < PP_CAP ? box[me] : -1.0f;
-  int j = c0 + threadIdx.x;
-  if (j < c1) { sl[threadIdx.x] = list[j]; sb[threadIdx.x] = box[j]; }
+  int loA, hiA, loB, hiB;
+  pp_range(gA, count, loA, hiA);
+  pp_range(gB, count, loB, hiB);
+  if (loA >= hiA || loB >= hiB) return;
+  const bool both = gA != gB;
+  const int nA = hiA - loA, nB = hiB - loB, t = threadIdx.x;
+  if (t < 64) st[t] = tab[t];
+  lB[t] = t < nB ? list[loB + t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bB[t] = t < nB ? box[loB + t] : -1.0f;
+  for (int k = t; k < PP_CHUNK * 4; k += 256) colSum[k] = 0.0f;
   __syncthreads();
-  float cut2 = cut * cut, a = sqrtf(a2);
-  float a0 = 0, a1 = 0, a2s = 0, pot = 0;
-  if (active) {
-    int nt = c1 - c0;
-    for (int q = 0; q < nt; q++) {
-      if (c0 + q == me) continue;
-      float4 Q = sl[q];
-      float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
-      float r2 = dot3(d0, d1, d2, d0, d1, d2);
-      if (r2 >= cut2) continue;
-      float r = qsqrt(r2);
-      float x = gclamp(qdiv(r, cut) * 64.0f - 0.5f, 0.0f, 63.0f);
-      int k = (int)x, k1 = min(k + 1, 63);
-      float t = x - (float)k;
-      float invN = qdiv(1.0f, qsqrt(r2 + a2));
-      // smoothstep(cut, 0.8 cut, r): (r - cut) / (0.8 cut - cut), both signs turned (exactly) to divide by a positive
-      float taper = gclamp(qdiv(cut - r, cut - 0.8f * cut), 0.0f, 1.0f);
-      taper = taper * taper * (3.0f - 2.0f * taper);
-      float same = (myBox >= 0.0f && sb[q] == myBox) ? 1.0f : 0.0f;
-      float2 tk = st[k], tk1 = st[k1];
-      float Tx = gmix(tk.x, tk1.x, t) * same, Ty = gmix(tk.y, tk1.y, t) * same;
-      float fn = invN * invN * invN * r - qdiv(Tx, gmax(r2, a2)), mj = Q.w;
-      float s = mj * fn * taper, rr = gmax(r, 1e-6f);
-      a0 += qdiv(d0, rr) * s; a1 += qdiv(d1, rr) * s; a2s += qdiv(d2, rr) * s;
-      pot += mj * (-invN - qdiv(Ty, gmax(r, a))) * taper;
+  const float cut2 = cut * cut, a = sqrtf(a2), tapD = cut - 0.8f * cut;
+  const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD);
+  // evaluation: lane il of a tile's rows against four consecutive columns
+  const int il = t & 31, jl0 = (t >> 6) * 8 + ((t >> 5) & 1) * 4;
+  // sums: lanes 0..127 a row's component, lanes 128..255 a column's
+  const bool rowLane = t < 128;
+  const int sl = (t & 127) >> 2, comp = t & 3;
+  const int tilesA = (nA + PP_TILE - 1) / PP_TILE, tilesB = (nB + PP_TILE - 1) / PP_TILE;
+  const int ta0 = both ? 0 : quarter * 2, ta1 = both ? tilesA : min(ta0 + 2, tilesA);
+  for (int ta = ta0; ta < ta1; ta++) {
+    const int i = ta * PP_TILE + il;
+    const float4 P = i < nA ? list[loA + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float bi = i < nA ? box[loA + i] : -1.0f;
+    float rowRun = 0.0f;
+    for (int tb = 0; tb < tilesB; tb++) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {   // the tile's terms
+        const int jl = jl0 + k, j = tb * PP_TILE + jl;
+        float4 ti, tj;
+        if (i < nA && j < nB && (both || i != j)) {
+          pp_terms(P, bi, lB[j], bB[j], cut, cut2, a2, a, yCut, yTap, tapD, st, ti, tj);
+        } else {
+          ti = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          tj = ti;
+        }
+        tRow[il * PP_PAD + jl] = ti;
+        if (both) tCol[jl * PP_PAD + il] = tj;
+      }
+      __syncthreads();
+      if (rowLane) {   // added in order to the running sums: a row's over B's tiles in turn ...
+        const float* R = reinterpret_cast<const float*>(tRow) + sl * PP_PAD * 4 + comp;
+        float v[PP_TILE];
+#pragma unroll
+        for (int j = 0; j < PP_TILE; j++) v[j] = R[j * 4];
+#pragma unroll
+        for (int j = 0; j < PP_TILE; j++) rowRun += v[j];
+      } else if (both) {   // ... a column's over A's tiles in turn
+        const float* Cc = reinterpret_cast<const float*>(tCol) + sl * PP_PAD * 4 + comp;
+        float v[PP_TILE];
+#pragma unroll
+        for (int k = 0; k < PP_TILE; k++) v[k] = Cc[k * 4];
+        float& run = colSum[(tb * PP_TILE + sl) * 4 + comp];
+        float c = run;
+#pragma unroll
+        for (int k = 0; k < PP_TILE; k++) c += v[k];
+        run = c;
+      }
+      __syncthreads();
     }
+    const int ir = ta * PP_TILE + sl;   // a row's sum over B is complete
+    if (rowLane && ir < nA) reinterpret_cast<float*>(partial + (size_t)gB * PP_CAP + loA + ir)[comp] = rowRun;
   }
-  if (me < PP_CAP) out[me] = make_float4(a0, a1, a2s, pot);
+  if (both && t < nB) {   // j's sum over A is complete
+    const float* c = colSum + t * 4;
+    partial[(size_t)gA * PP_CAP + loB + t] = make_float4(c[0], c[1], c[2], c[3]);
+  }
 }
-// a slice's chunks in order, times G m
+// a slice's partials in order, times G m
 __global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ partial, const float4* __restrict__ list,
                                                 int count, float gm, float4* force) {
   int me = blockIdx.x * blockDim.x + threadIdx.x, slice = blockIdx.y;
@@ -648,6 +754,9 @@ __global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ parti
   float4 s = make_float4(0, 0, 0, 0);
   if (active) {
     for (int c = 0; c < PP_MAXCHUNK; c++) {
+      int lo, hi;
+      pp_range(slice * PP_MAXCHUNK + c, count, lo, hi);
+      if (lo >= hi) break;
       float4 v = partial[((size_t)slice * PP_MAXCHUNK + c) * PP_CAP + me];
       s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
     }
@@ -656,14 +765,14 @@ __global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ parti
   force[slice * PP_CAP + me] = s;
 }
 
-size_t pp_partial_bytes() { return (size_t)4 * PP_MAXCHUNK * PP_CAP * sizeof(float4); }
+size_t pp_partial_bytes() { return (size_t)PP_RANGES * PP_CAP * sizeof(float4); }
 
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm, float cut,
                float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force, hipStream_t s) {
   hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, b, index, list, box);
-  int tb = (count + 255) / 256;   // seats past count only need the zero that k_pp_sum writes
-  if (tb < 1) tb = 1;
-  hipLaunchKernelGGL(k_pp_force, dim3(tb, 4, PP_MAXCHUNK), dim3(256), 0, s, list, box, count, cut, a2, tab, partial);
+  if (count > 0)
+    hipLaunchKernelGGL(k_pp_pairs, dim3(PP_PAIRS + 4 * PP_RANGES), dim3(256), 0, s, list, box, count, cut, a2, tab,
+                       partial);
   hipLaunchKernelGGL(k_pp_sum, dim3(PP_CAP / 256, 4), dim3(256), 0, s, partial, list, count, gm, force);
 }
 

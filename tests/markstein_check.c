@@ -1,0 +1,51 @@
+/* The pairwise pass divides by two fixed divisors -- the cut, 5 FH = 0.9375,
+ * and the taper's width, cut - 0.8 cut in FP32 -- with one residual
+ * correction of a * y, y the divisor's correctly rounded reciprocal
+ * (qdiv_by in cosmic-collisions_amd/csrc/cc_kernels.hip).  This checks that
+ * against the correctly rounded quotient for 0 and every FP32 dividend in
+ * [lo, hi] (arguments; 2^-80 and 1 by default), the quotient taken in FP64
+ * (a quotient of two FP32 numbers rounds the same from FP64).  Prints the
+ * count of differences per divisor; exit status 1 if any. */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static float f_of(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static uint32_t u_of(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+
+int main(int argc, char** argv) {
+  float lo = argc > 1 ? strtof(argv[1], NULL) : 0x1p-80f;
+  float hi = argc > 2 ? strtof(argv[2], NULL) : 1.0f;
+  volatile float cut = (float)(5.0 * (2.0 * 6.0 / 64.0));
+  volatile float c8 = 0.8f * cut;
+  float divisors[2] = { cut, cut - c8 };
+  long total = 0;
+  for (int k = 0; k < 2; k++) {
+    float b = divisors[k];
+    float y = (float)(1.0 / (double)b);
+    long bad = 0;
+    for (uint32_t u = u_of(lo); u <= u_of(hi); u++) {
+      float a = f_of(u);
+      float q = a * y;
+      float r = fmaf(-b, q, a);
+      float m = fmaf(r, y, q);
+      float want = (float)((double)a / (double)b);
+      if (m != want) bad++;
+    }
+    float a0 = 0.0f, q0 = a0 * y, m0 = fmaf(fmaf(-b, q0, a0), y, q0);
+    if (m0 != 0.0f) bad++;
+    printf("divisor %.9g: %ld differences\n", b, bad);
+    total += bad;
+  }
+  return total ? 1 : 0;
+}
