@@ -145,6 +145,21 @@ def runto_leg(steps, every=360):
     return out
 
 
+def build_leg():
+    """Body setup on the host: the twins build with its bodies relaxed in
+    JavaScript and by the library, and the batched sweep's 64 instances'
+    bodies relaxed in one library call (js/time_build.js)."""
+    cmd = ["node", os.path.join(PKG, "js", "time_build.js"), json.dumps(runs.TWINS)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("time_build.js failed (exit %d):\n%s" % (r.returncode, r.stderr[-3000:]))
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["workload"] = "body setup (onion shells laid in JavaScript, the overlap relaxation of src/bodies.js:97-146 in " \
+                      "JavaScript or by cc_relax_piles, bit-identical): the twins build, and every body of the batched " \
+                      "sweep's 64 instances in one call"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +174,7 @@ def main():
     ap.add_argument("--batch-dir", default=os.path.join(tempfile.gettempdir(), "cc_batch_state"))
     ap.add_argument("--runto", type=int, default=3600, help="steps of the engine-driven leg with analyses (0: skip)")
     ap.add_argument("--learned", type=int, default=1, help="the learned far field leg (0: skip it)")
+    ap.add_argument("--build", type=int, default=1, help="the body setup leg on the host (0: skip it)")
     ap.add_argument("--learned-steps", type=int, default=1000)
     ap.add_argument("--ff-cycles", type=int, default=48)
     ap.add_argument("--ff-iters", type=int, default=3000)
@@ -207,6 +223,8 @@ def main():
         out["engine_runto"] = runto_leg(args.runto)
     if args.batch > 0:
         out["batched"] = batch_leg(args.batch_dir, args.batch, args.batch_steps, 24, args.reuse_state)
+    if args.build > 0:
+        out["body_build"] = build_leg()
     if args.learned > 0:
         out["learned"] = learned_leg(tempfile.gettempdir(), args.learned_steps, 200, args.reuse_state, args.ff_cycles,
                                      args.ff_iters)

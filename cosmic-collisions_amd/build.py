@@ -17,7 +17,8 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["csrc/cc_kernels.hip", "csrc/cc_api.hip", "csrc/cc_ff.hip", "csrc/cc_fof.hip", "csrc/cc_fof_host.cpp"]
+SOURCES = ["csrc/cc_kernels.hip", "csrc/cc_api.hip", "csrc/cc_ff.hip", "csrc/cc_fof.hip", "csrc/cc_fof_host.cpp",
+           "csrc/cc_bodies.cpp"]
 
 
 def run(cmd, cwd):

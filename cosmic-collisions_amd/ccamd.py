@@ -41,7 +41,20 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
-           "cc_set_far_field", "cc_ff_stats"]
+           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error"]
+
+
+class Pile(C.Structure):
+    """cc_pile: n x 3 FP32 positions relaxed in place (relax(), src/bodies.js:97-146)."""
+    _fields_ = [("p", C.POINTER(C.c_float)), ("n", C.c_int32), ("sweeps", C.c_int32), ("a", C.c_double),
+                ("A", C.c_double), ("C", C.c_double), ("confine", C.c_int32), ("pad_", C.c_int32)]
+
+
+class PileReport(C.Structure):
+    _fields_ = [("sweeps", C.c_int32), ("pad_", C.c_int32), ("worst", C.c_double)]
+
+
+PILE_SPHEROID, PILE_GROUND = 0, 1
 
 
 class AnalyseArgs(C.Structure):
@@ -152,6 +165,8 @@ def lib():
             "cc_ff_eval": ([P, C.c_int, C.c_int, fp, dp, dp], C.c_int),
             "cc_ff_gradient": ([P, C.c_int, C.c_int, fp, dp], C.c_int),
             "cc_set_far_field": ([P, P, C.c_int, C.c_double], C.c_int), "cc_ff_stats": ([P, dp], C.c_int),
+            "cc_relax_piles": ([C.POINTER(Pile), C.c_int, C.POINTER(PileReport)], C.c_int),
+            "cc_bodies_last_error": ([], C.c_char_p),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -525,3 +540,21 @@ class FarField:
         except Exception:
             pass
 
+
+def relax_piles(piles):
+    """Relaxes each (xyz float32 array of shape (n, 3), a, A, C, confine, sweeps) in place, the
+    piles side by side (cc_relax_piles); returns [(sweeps, worst)]."""
+    for q in piles:
+        a = q[0]
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous and a.ndim == 2
+                and a.shape[1] == 3):
+            raise CCError("each pile must be a C-contiguous float32 (n, 3) array, relaxed in place")
+    ps = (Pile * max(len(piles), 1))()
+    for k, q in enumerate(piles):
+        a = q[0]
+        ps[k] = Pile(_ptr(a), len(a), int(q[5]), float(q[1]), float(q[2]), float(q[3]), int(q[4]), 0)
+    reps = (PileReport * max(len(piles), 1))()
+    r = lib().cc_relax_piles(ps, len(piles), reps)
+    if r:
+        raise CCError("cc_relax_piles failed (%d): %s" % (r, lib().cc_bodies_last_error().decode()))
+    return [(reps[k].sweeps, reps[k].worst) for k in range(len(piles))]

@@ -36,8 +36,10 @@ function shellArea(A, C, NB) {
 // shells (eccentricity e) outermost first, each holding its volume's share,
 // laid by the golden-angle spiral; the innermost remainder fills a seed.
 // Returns xyz (Float32Array, inside-out order) with .bounds (each shell's
-// first index, ascending) and .relax (the relaxation's report).
-function onion(n, s, a, rand, e) {
+// first index, ascending) and .pile, what relaxing it takes ({p, n, a, A, C}:
+// the spheroid it is held in).  The relaxation draws no random numbers, so a
+// caller may lay several bodies first and relax them together.
+function onionLay(n, s, a, rand, e) {
   const R = Math.cbrt(3 * n * s * s * s / (16 * Math.PI));
   const ae = R / Math.pow(1 - e * e, 1 / 6), ce = ae * Math.sqrt(1 - e * e);
   const vp = s * s * s / 4, unit = 3 * vp / (4 * Math.PI), jit = 0.06 * s;
@@ -87,8 +89,38 @@ function onion(n, s, a, rand, e) {
   firsts.push(0);
   firsts.reverse();
   xyz.bounds = firsts;
-  xyz.relax = relaxPile(xyz, n, a, ae - h * 0.5, ce - h * 0.5);
+  xyz.pile = { p: xyz, n, a, A: ae - h * 0.5, C: ce - h * 0.5, confine: SPHEROID, sweeps: 0 };
   return xyz;
+}
+
+// onion() of the reference (src/bodies.js:23-76): laid and relaxed, with
+// .relax the relaxation's report.
+function onion(n, s, a, rand, e) {
+  const xyz = onionLay(n, s, a, rand, e);
+  xyz.relax = relaxPiles([xyz.pile])[0];
+  return xyz;
+}
+
+// Piles relaxed (relaxPile below): with the native addon all in one call,
+// side by side on the host's threads (cc_relax_piles, the same float64
+// arithmetic in the same order, so the same bits); without it, here, in turn.  A pile is
+// {p, n, a, A, C, confine, sweeps}: confine SPHEROID holds it in the spheroid
+// of semi-axes A and C, GROUND in the crater's ground patch (radius A, depth C).
+const SPHEROID = 0, GROUND = 1;
+function relaxPiles(piles, native) {
+  if (native) return native.relaxPiles(piles);
+  return piles.map((q) => relaxPile(q.p, q.n, q.a, q.A, q.C, q.confine === GROUND ? groundConfine(q.A, q.C) : undefined,
+                                    q.sweeps || undefined));
+}
+
+// the crater's ground patch (chicxulub.html:543-548): inside the wall, below the flat sky, above the floor
+function groundConfine(R, D) {
+  const RR = R * R;
+  return (held, x, y, z) => {
+    const q2 = x * x + z * z;
+    if (q2 > RR) { const f = R / Math.sqrt(q2); x *= f; z *= f; }
+    held[0] = x; held[1] = Math.max(Math.min(y, 0), -D); held[2] = z;
+  };
 }
 
 // Inertia-free relaxation (src/bodies.js:97-146): pairs closer than the target
@@ -178,4 +210,4 @@ function rotation(w, dt) {
           t * x * z + s * y, t * y * z - s * x, t * z * z + c];
 }
 
-module.exports = { seeded, onion, relaxPile, spheroidEcc, rotation };
+module.exports = { seeded, onion, onionLay, relaxPile, relaxPiles, groundConfine, SPHEROID, GROUND, spheroidEcc, rotation };

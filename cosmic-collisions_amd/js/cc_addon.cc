@@ -485,6 +485,52 @@ napi_value Analyse(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// relaxPiles([{p: Float32Array (n x 3), n, a, A, C, confine, sweeps}, ...]) -> [{sweeps, worst}, ...]:
+// relax() of src/bodies.js:97-146 on each pile, the piles side by side (cc_relax_piles)
+napi_value RelaxPiles(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  bool isArr = false;
+  if (a.argc < 1 || napi_is_array(env, a.argv[0], &isArr) != napi_ok || !isArr) {
+    napi_throw_type_error(env, nullptr, "relaxPiles takes an array of piles");
+    return nullptr;
+  }
+  uint32_t count = 0;
+  napi_get_array_length(env, a.argv[0], &count);
+  std::vector<cc_pile> piles(count);
+  std::vector<cc_pile_report> reps(count);
+  for (uint32_t k = 0; k < count; k++) {
+    napi_value o;
+    napi_get_element(env, a.argv[0], k, &o);
+    cc_pile& pl = piles[k];
+    memset(&pl, 0, sizeof pl);
+    pl.n = (int32_t)num(env, prop(env, o, "n"));
+    bool ok = true;
+    pl.p = static_cast<float*>(typed(env, prop(env, o, "p"), napi_float32_array, (size_t)(pl.n > 0 ? pl.n : 0) * 3, &ok));
+    if (!ok) return nullptr;
+    pl.a = num(env, prop(env, o, "a"));
+    pl.A = num(env, prop(env, o, "A"));
+    pl.C = num(env, prop(env, o, "C"));
+    pl.confine = (int32_t)num(env, prop(env, o, "confine"));
+    pl.sweeps = (int32_t)num(env, prop(env, o, "sweeps"));
+  }
+  int code = cc_relax_piles(piles.data(), (int)count, reps.data());
+  if (code) {
+    std::string msg = "cc_relax_piles failed (" + std::to_string(code) + "): " + cc_bodies_last_error();
+    napi_throw_error(env, nullptr, msg.c_str());
+    return nullptr;
+  }
+  napi_value out;
+  napi_create_array_with_length(env, count, &out);
+  for (uint32_t k = 0; k < count; k++) {
+    napi_value r;
+    napi_create_object(env, &r);
+    set_num(env, r, "sweeps", reps[k].sweeps);
+    set_num(env, r, "worst", reps[k].worst);
+    napi_set_element(env, out, k, r);
+  }
+  return out;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char* name; napi_callback cb; } fns[] = {
     { "create", Create }, { "destroy", Destroy }, { "setParams", SetParams }, { "uploadState", Upload }, { "downloadState", Download },
@@ -494,6 +540,7 @@ napi_value Init(napi_env env, napi_value exports) {
     { "sim", Sim }, { "step", Step }, { "stepN", StepN }, { "getStepNo", GetStepNo }, { "setStepNo", SetStepNo },
     { "setGraphs", SetGraphs },
     { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "analyse", Analyse }, { "books", Books }, { "sync", Sync },
+    { "relaxPiles", RelaxPiles },
   };
   for (auto& f : fns) {
     napi_value fn;

@@ -33,7 +33,14 @@ function seismicMagnitude(E) { return 0.67 * Math.log10(E) - 5.87; }
 // A jittered face-centred lattice cut to a cylinder of radius R and depth D,
 // relaxed inside its walls, floor and flat top (chicxulub.html:512-550); the
 // spacing is searched for, sites are dropped at random to leave nWant.
+// latticeLay stops short of the relaxation (which draws no random numbers) and
+// returns it as .pile, so the ground and the impactor can be relaxed together.
 function latticeFill(nWant, R, D, rand) {
+  const lat = latticeLay(nWant, R, D, rand);
+  lat.relax = bodies.relaxPiles([lat.pile])[0];
+  return lat;
+}
+function latticeLay(nWant, R, D, rand) {
   let s = Math.cbrt(4 * Math.PI * R * R * D / nWant), pts = null;
   for (let attempt = 0; attempt < 8; attempt++) {
     pts = [];
@@ -59,20 +66,15 @@ function latticeFill(nWant, R, D, rand) {
     out[q * 3 + 1] = pts[o + 1] + (rand() - 0.5) * jit;
     out[q * 3 + 2] = pts[o + 2] + (rand() - 0.5) * jit;
   }
-  const a = JAM * s / (2 * Math.SQRT2), RR = R * R;
-  bodies.relaxPile(out, nWant, a, R, D, (held, x, y, z) => {
-    const q2 = x * x + z * z;
-    if (q2 > RR) { const f = R / Math.sqrt(q2); x *= f; z *= f; }
-    held[0] = x; held[1] = Math.max(Math.min(y, 0), -D); held[2] = z;
-  }, 16);
-  return { p: out, s, a };
+  const a = JAM * s / (2 * Math.SQRT2);
+  return { p: out, s, a, pile: { p: out, n: nWant, a, A: R, C: D, confine: bodies.GROUND, sweeps: 16 } };
 }
 
 // The crater run (chicxulub.html:567-674): ground, impactor, and the run's constants
 // S.N (no reference counterpart) asks for a grain count outside N_CHOICES: a
 // multiple of 512 up to 524,288, 512 wide (the seat words then carry a
 // 19-bit index above 262,144).
-function buildCraterRun(S) {
+function buildCraterRun(S, native) {
   const N = S.N > 0 ? S.N : N_CHOICES[S.nIdx];
   if (S.N > 0 && (N % 512 !== 0 || N > 524288 || N < 16384)) throw new Error('N must be a multiple of 512 in [16384, 524288]');
   const W = S.N > 0 || N >= 131072 ? 512 : N >= 32768 ? 256 : 128, H = N / W;
@@ -86,9 +88,10 @@ function buildCraterRun(S) {
   const vImpactor = 4 / 3 * Math.PI * aImp * aImp * aImp * (rhoI / rhoT);
   const n2 = Math.max(1, Math.min(Math.round(N * vImpactor / (vBox + vImpactor)), Math.floor(N / 4)));
   const n1 = N - n2;
-  const lat = latticeFill(n1, Rbox, Dbox, rand);
+  const lat = latticeLay(n1, Rbox, Dbox, rand);
   const s = lat.s, a = JAM * s / (2 * Math.SQRT2);
-  const b2 = bodies.onion(n2, s, a, rand, 0);
+  const b2 = bodies.onionLay(n2, s, a, rand, 0);
+  [lat.relax, b2.relax] = bodies.relaxPiles([lat.pile, b2.pile], native);
   const pos = new Float32Array(N * 4), vel = new Float32Array(N * 4), matR = new Float32Array(N);
   const sedTop = -S.sed * 1000 / L_UNIT, wallR = Rbox - froz, floorY = -(Dbox - froz);
   for (let i = 0; i < n1; i++) {
@@ -143,7 +146,7 @@ function createCraterEngine(opts) {
              frozenOn: 1, frozen: MAT_FROZEN };
   }
   function build(S) {
-    const run = buildCraterRun(Object.assign({}, DEFAULTS, S || {}));
+    const run = buildCraterRun(Object.assign({}, DEFAULTS, S || {}), addon);
     sim = run.sim;
     if (ctx) addon.destroy(ctx);   // a rebuild frees the last run's device objects (src/nbody.js:208-245)
     ctx = addon.create(params());
@@ -196,5 +199,5 @@ function createCraterEngine(opts) {
            PRESETS, DEFAULTS };
 }
 
-module.exports = { createCraterEngine, buildCraterRun, latticeFill, transientCrater, finalCrater, seismicMagnitude,
+module.exports = { createCraterEngine, buildCraterRun, latticeFill, latticeLay, transientCrater, finalCrater, seismicMagnitude,
                    MAT_FROZEN, MAT_MELT, MAT_IMPACTOR, HH_C1, HH_MU, HH_P, Z_MODEL, E_KG, CP_ROCK, G_EARTH, T_UNIT, L_UNIT };

@@ -85,7 +85,19 @@ function pairTable(bins, cut) {
 // Batched sweeps (no reference counterpart) add two options: opt.N, a
 // particle count outside N_CHOICES (a multiple of 512, at most 262144: 512
 // wide), and opt.seed, the bodies' own seed (the reference fixes 1234567).
-function buildRun(opt) {
+// native: the addon, whose relaxPiles relaxes the bodies side by side (the
+// same bits as relaxing them here in turn).  buildRuns lays out every run's
+// bodies first and relaxes them all in one call (the relaxation draws no
+// random numbers, so the runs are those buildRun makes one at a time).
+function buildRun(opt, native) {
+  return buildRuns([opt], native)[0];
+}
+function buildRuns(opts, native) {
+  const laid = opts.map(layRun);
+  const reports = bodies.relaxPiles([].concat(...laid.map((l) => l.piles)), native);
+  return laid.map((l, k) => l.finish(reports.slice(2 * k, 2 * k + 2)));
+}
+function layRun(opt) {
   const N = opt.N > 0 ? opt.N : N_CHOICES[opt.nIdx];
   if (opt.N > 0 && (N % 512 !== 0 || N > 262144 || N < 1024)) throw new Error('N must be a multiple of 512 in [1024, 262144]');
   const W = opt.N > 0 ? 512 : N >= 131072 ? 512 : N >= 32768 ? 256 : 128, H = N / W;
@@ -104,63 +116,66 @@ function buildRun(opt) {
   const e1 = bodies.spheroidEcc(Math.abs(opt.spin0)), e2 = bodies.spheroidEcc(Math.abs(opt.spin1));
   const Re1 = R1 / Math.pow(1 - e1 * e1, 1 / 6), Re2 = R2 / Math.pow(1 - e2 * e2, 1 / 6);
   const rand = bodies.seeded(opt.seed > 0 ? opt.seed : 1234567);
-  const b1 = bodies.onion(n1, s, a, rand, e1), b2 = bodies.onion(n2, s, a, rand, e2);
-  const pos = new Float32Array(N * 4), vel = new Float32Array(N * 4), matR = new Float32Array(N), lay = new Uint8Array(N);
-  // layers by whole shells: the core the shells nearest its volume share, the crust the outer shell
-  const layer = (off, n, firsts, scale, cv) => {
-    const nCore = firsts.reduce((best, c) => (Math.abs(c - cv * n) < Math.abs(best - cv * n) ? c : best), 0);
-    const nCrust = n - firsts[firsts.length - 2];
-    let sum = 0;
-    for (let j = 0; j < n; j++) {
-      const L = j < nCore ? 0 : j >= n - nCrust ? 2 : 1;
-      lay[off + j] = L;
-      matR[off + j] = scale * (L === 0 ? RHO_R : 1);
-      sum += matR[off + j];
+  const b1 = bodies.onionLay(n1, s, a, rand, e1), b2 = bodies.onionLay(n2, s, a, rand, e2);
+  return { piles: [b1.pile, b2.pile], finish: (reports) => {
+    [b1.relax, b2.relax] = reports;
+    const pos = new Float32Array(N * 4), vel = new Float32Array(N * 4), matR = new Float32Array(N), lay = new Uint8Array(N);
+    // layers by whole shells: the core the shells nearest its volume share, the crust the outer shell
+    const layer = (off, n, firsts, scale, cv) => {
+      const nCore = firsts.reduce((best, c) => (Math.abs(c - cv * n) < Math.abs(best - cv * n) ? c : best), 0);
+      const nCrust = n - firsts[firsts.length - 2];
+      let sum = 0;
+      for (let j = 0; j < n; j++) {
+        const L = j < nCore ? 0 : j >= n - nCrust ? 2 : 1;
+        lay[off + j] = L;
+        matR[off + j] = scale * (L === 0 ? RHO_R : 1);
+        sum += matR[off + j];
+      }
+      return sum;
+    };
+    const sum1 = layer(0, n1, b1.bounds, 1, coreV);
+    let sum2 = layer(n1, n2, b2.bounds, dens, coreV2);
+    const mref = 1 / sum1;
+    if (opt.mImp > 0) {
+      const k2 = opt.mImp / (mref * sum2);
+      for (let j = n1; j < N; j++) matR[j] *= k2;
+      sum2 *= k2;
     }
-    return sum;
-  };
-  const sum1 = layer(0, n1, b1.bounds, 1, coreV);
-  let sum2 = layer(n1, n2, b2.bounds, dens, coreV2);
-  const mref = 1 / sum1;
-  if (opt.mImp > 0) {
-    const k2 = opt.mImp / (mref * sum2);
-    for (let j = n1; j < N; j++) matR[j] *= k2;
-    sum2 *= k2;
-  }
-  const M1 = 1, M2 = mref * sum2, M = M1 + M2;
-  // the encounter, stated at contact and walked back to the start line 3.7 radii off
-  const vesc = Math.sqrt(2 * M / Rsum), vc = opt.vfac * vesc;
-  const bc = Rsum * Math.sin(opt.angle * Math.PI / 180);
-  const D = Rsum + 3.7;
-  const vD = Math.sqrt(Math.max(vc * vc - 2 * M * (1 / Rsum - 1 / D), 0.04));
-  const bD = Math.min(bc * vc / vD, D * 0.95);
-  const xD = -Math.sqrt(D * D - bD * bD);
-  const inc = (opt.incl || 0) * Math.PI / 180, bY = bD * Math.sin(inc), bZ = bD * Math.cos(inc);
-  const c1 = [-xD * M2 / M, -bY * M2 / M, -bZ * M2 / M];
-  const c2 = [xD + c1[0], bY + c1[1], bZ + c1[2]];
-  const rel = { r: [xD, bY, bZ], v: [vD, 0, 0] };
-  for (let i = 0; i < n1; i++) { const o = i * 4; pos[o] = b1[i * 3] + c1[0]; pos[o + 1] = b1[i * 3 + 1] + c1[1]; pos[o + 2] = b1[i * 3 + 2] + c1[2]; pos[o + 3] = lay[i]; }
-  for (let i = 0; i < n2; i++) { const o = (n1 + i) * 4; pos[o] = b2[i * 3] + c2[0]; pos[o + 1] = b2[i * 3 + 1] + c2[1]; pos[o + 2] = b2[i * 3 + 2] + c2[2]; pos[o + 3] = lay[n1 + i] + 4; }
-  const home = new Float32Array(N * 4), hrand = bodies.seeded(99);
-  for (let i = 0; i < n1; i++) { const o = i * 4; home[o] = b1[i * 3]; home[o + 1] = b1[i * 3 + 1]; home[o + 2] = b1[i * 3 + 2]; home[o + 3] = hrand(); }
-  for (let i = 0; i < n2; i++) { const o = (n1 + i) * 4; home[o] = b2[i * 3] + 1.5; home[o + 1] = b2[i * 3 + 1] + 0.7; home[o + 2] = b2[i * 3 + 2] - 1.1; home[o + 3] = hrand(); }
-  // contact spring: half a radius of overlap at the reference speed; the step from its period
-  const vref = Math.max(1.5, vc);
-  const k = 4 * mref * vref * vref / (a * a);
-  const dt = 0.35 / Math.sqrt(2 * k / (mref * Math.min(1, dens)));
-  const lnE = Math.log(opt.rest), zeta = -lnE / Math.sqrt(Math.PI * Math.PI + lnE * lnE);
-  const cd = 2 * zeta * Math.sqrt(k * mref / 2);
-  const settle = Math.ceil(0.6 / dt);
-  const sim = {
-    N, W, H, G: N >= 65536 ? 128 : 64, n1, n2, m: mref, a, cell: 2.6 * a, dt, k, c: cd, creep: 0.12,
-    R1, R2, Re1, Re2, M1, M2, M, rel, bulk0: [0, 0, 0], bulk1: [0, 0, 0], vc, vesc,
-    spin0: w0, spin1: w1, cen0: c1, cen1: c2, matR, homeArr: home, ref: null,
-    t: -approachTime(rel, M, Re1 + Re2 + 2 * a), phase: 'settle', settleLeft: settle, settleTotal: settle,
-    impactT: -1, com1: [0, 0, 0], stepNo: 0, damp: Math.exp(-dt * 12), soft: false, gen: 0,
-    boxes: [{ on: true, cen: [0, 0, 0], vel: [0, 0, 0], t: 0 }, { on: false, cen: [0, 0, 0], vel: [0, 0, 0], t: 0 }],
-    tK: 0.5 * E_KG / CP_ROCK / mref, kappa: 0.3
-  };
-  return { sim, pos, vel };
+    const M1 = 1, M2 = mref * sum2, M = M1 + M2;
+    // the encounter, stated at contact and walked back to the start line 3.7 radii off
+    const vesc = Math.sqrt(2 * M / Rsum), vc = opt.vfac * vesc;
+    const bc = Rsum * Math.sin(opt.angle * Math.PI / 180);
+    const D = Rsum + 3.7;
+    const vD = Math.sqrt(Math.max(vc * vc - 2 * M * (1 / Rsum - 1 / D), 0.04));
+    const bD = Math.min(bc * vc / vD, D * 0.95);
+    const xD = -Math.sqrt(D * D - bD * bD);
+    const inc = (opt.incl || 0) * Math.PI / 180, bY = bD * Math.sin(inc), bZ = bD * Math.cos(inc);
+    const c1 = [-xD * M2 / M, -bY * M2 / M, -bZ * M2 / M];
+    const c2 = [xD + c1[0], bY + c1[1], bZ + c1[2]];
+    const rel = { r: [xD, bY, bZ], v: [vD, 0, 0] };
+    for (let i = 0; i < n1; i++) { const o = i * 4; pos[o] = b1[i * 3] + c1[0]; pos[o + 1] = b1[i * 3 + 1] + c1[1]; pos[o + 2] = b1[i * 3 + 2] + c1[2]; pos[o + 3] = lay[i]; }
+    for (let i = 0; i < n2; i++) { const o = (n1 + i) * 4; pos[o] = b2[i * 3] + c2[0]; pos[o + 1] = b2[i * 3 + 1] + c2[1]; pos[o + 2] = b2[i * 3 + 2] + c2[2]; pos[o + 3] = lay[n1 + i] + 4; }
+    const home = new Float32Array(N * 4), hrand = bodies.seeded(99);
+    for (let i = 0; i < n1; i++) { const o = i * 4; home[o] = b1[i * 3]; home[o + 1] = b1[i * 3 + 1]; home[o + 2] = b1[i * 3 + 2]; home[o + 3] = hrand(); }
+    for (let i = 0; i < n2; i++) { const o = (n1 + i) * 4; home[o] = b2[i * 3] + 1.5; home[o + 1] = b2[i * 3 + 1] + 0.7; home[o + 2] = b2[i * 3 + 2] - 1.1; home[o + 3] = hrand(); }
+    // contact spring: half a radius of overlap at the reference speed; the step from its period
+    const vref = Math.max(1.5, vc);
+    const k = 4 * mref * vref * vref / (a * a);
+    const dt = 0.35 / Math.sqrt(2 * k / (mref * Math.min(1, dens)));
+    const lnE = Math.log(opt.rest), zeta = -lnE / Math.sqrt(Math.PI * Math.PI + lnE * lnE);
+    const cd = 2 * zeta * Math.sqrt(k * mref / 2);
+    const settle = Math.ceil(0.6 / dt);
+    const sim = {
+      N, W, H, G: N >= 65536 ? 128 : 64, n1, n2, m: mref, a, cell: 2.6 * a, dt, k, c: cd, creep: 0.12,
+      R1, R2, Re1, Re2, M1, M2, M, rel, bulk0: [0, 0, 0], bulk1: [0, 0, 0], vc, vesc,
+      spin0: w0, spin1: w1, cen0: c1, cen1: c2, matR, homeArr: home, ref: null,
+      t: -approachTime(rel, M, Re1 + Re2 + 2 * a), phase: 'settle', settleLeft: settle, settleTotal: settle,
+      impactT: -1, com1: [0, 0, 0], stepNo: 0, damp: Math.exp(-dt * 12), soft: false, gen: 0,
+      boxes: [{ on: true, cen: [0, 0, 0], vel: [0, 0, 0], t: 0 }, { on: false, cen: [0, 0, 0], vel: [0, 0, 0], t: 0 }],
+      tK: 0.5 * E_KG / CP_ROCK / mref, kappa: 0.3
+    };
+    return { sim, pos, vel };
+  } };
 }
 
 function createEngine(opts) {
@@ -182,8 +197,9 @@ function createEngine(opts) {
   }
 
   // build(): the run's bodies and constants (buildRun), uploaded to a fresh context
-  function build(opt) {
-    const run = buildRun(opt);
+  // run: one buildRuns made already for this opt
+  function build(opt, run) {
+    run = run || buildRun(opt, addon);
     sim = run.sim;
     sim.gen = ++gen;
     if (ctx) addon.destroy(ctx);   // a rebuild frees the last run's device objects (src/nbody.js:208-245)
@@ -399,4 +415,4 @@ function approachTime(rel, M, touch) {
   return t;
 }
 
-module.exports = { createEngine, buildRun, pairTable, approachTime, N_CHOICES, FL, FH, PP_CAP, PP_CUT };
+module.exports = { createEngine, buildRun, buildRuns, pairTable, approachTime, N_CHOICES, FL, FH, PP_CAP, PP_CUT };

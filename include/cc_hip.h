@@ -29,6 +29,7 @@ extern "C" {
 #define CC_EINVAL -1     /* bad argument or size */
 #define CC_EDEVICE -2    /* HIP runtime error */
 #define CC_ESTATE -3     /* call out of order */
+#define CC_EHOST -4      /* host resources (memory, threads) */
 
 typedef struct cc_ctx cc_ctx;
 
@@ -239,6 +240,31 @@ const char* cc_batch_last_error(cc_batch* b);
 int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* boxL, const int* nBoxL);
 /* device time of the last cc_batch_step_n, ms */
 int cc_batch_last_ms(cc_batch* b, double* ms);
+
+/* Body setup on the host (no device work): relax() of src/bodies.js:97-146,
+ * the overlap relaxation of a freshly laid pile (Gauss-Seidel over a cell grid
+ * rebuilt every fourth sweep, pairs closer than 0.96 x 2a pushed apart by half
+ * their overlap), float64 in the reference's order on FP32 positions, so the
+ * result is the reference's bit for bit.  The piles of a call are spread over
+ * up to one thread per hardware thread (a pile is sequential in itself).  A
+ * pile is held inside the spheroid of semi-axes A (x, z) and C (y) (the bodies,
+ * src/bodies.js:74), or, CC_PILE_GROUND, inside radius A of the y axis and
+ * between y = -C and y = 0 (the crater's ground patch and its confine
+ * callback, chicxulub.html:543-548).  sweeps <= 0: the reference's 10. */
+#define CC_PILE_SPHEROID 0
+#define CC_PILE_GROUND 1
+typedef struct {
+  float* p;            /* n x 3 positions, relaxed in place */
+  int32_t n, sweeps;
+  double a, A, C;      /* particle radius, the confinement's two sizes */
+  int32_t confine, pad_;
+} cc_pile;
+typedef struct {
+  int32_t sweeps, pad_;
+  double worst;        /* the last sweep's largest overlap over a */
+} cc_pile_report;
+int cc_relax_piles(const cc_pile* piles, int count, cc_pile_report* reports);
+const char* cc_bodies_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -26,18 +26,31 @@ def node(script, tmp_path):
     return r.stdout
 
 
-def test_build_bit_exact(tmp_path):
+ADDON = os.path.join(JS, "cc_addon.node")
+RELAX = [pytest.param(False, id="js-relax"),
+         pytest.param(True, id="native-relax",
+                      marks=pytest.mark.skipif(not os.path.exists(ADDON), reason="the N-API addon is not built"))]
+
+
+def native_arg(native):
+    return "require('%s')" % ADDON if native else "undefined"
+
+
+@pytest.mark.parametrize("native", RELAX)
+def test_build_bit_exact(tmp_path, native):
+    """build() against the reference's own initial state, with the bodies
+    relaxed here in JavaScript or by the library (cc_relax_piles)."""
     fx = gu.fixture("c1_build")
     opt = json.loads(str(fx["opt"]))
     out = node("""
 const nb = require('%s/nbody.js'); const fs = require('fs');
-const r = nb.buildRun(%s);
+const r = nb.buildRun(%s, %s);
 for (const [k, a] of [['pos', r.pos], ['vel', r.vel], ['mat', r.sim.matR], ['home', r.sim.homeArr]])
   fs.writeFileSync('%s/' + k, Buffer.from(a.buffer));
 const s = {}; for (const k of ['N', 'n1', 'n2', 'm', 'a', 'cell', 'dt', 'k', 'c', 't', 'settleLeft', 'tK', 'damp', 'vc', 'Re1', 'Re2', 'M2']) s[k] = r.sim[k];
 s.cen0 = r.sim.cen0; s.cen1 = r.sim.cen1; s.rel = r.sim.rel;
 console.log(JSON.stringify(s));
-""" % (JS, json.dumps(opt), tmp_path), tmp_path)
+""" % (JS, json.dumps(opt), native_arg(native), tmp_path), tmp_path)
     mine = json.loads(out)
     ref = json.loads(str(fx["sim"]))
     for k in ("pos", "vel", "mat", "home"):
@@ -89,20 +102,22 @@ delete r.pp; console.log(JSON.stringify(r));
     assert np.array_equal(into[lst].astype(np.float32), fx["pplist_fof"])
 
 
-def test_crater_build_bit_exact(tmp_path):
+@pytest.mark.parametrize("native", RELAX)
+def test_crater_build_bit_exact(tmp_path, native):
     """A20 the crater page's build (latticeFill, the impactor ball, materials,
-    encounter and contact constants) against the reference page's own."""
+    encounter and contact constants) against the reference page's own, the
+    ground and the impactor relaxed in JavaScript or by the library."""
     fx = gu.fixture("c3_crater")
     opt = json.loads(str(fx["opt"]))
     ref = gu.crater_host(fx, "crater_build")[0]["sim"]
     out = node("""
 const cr = require('%s/crater.js'); const fs = require('fs');
-const r = cr.buildCraterRun(Object.assign({}, %s));
+const r = cr.buildCraterRun(Object.assign({}, %s), %s);
 for (const [k, a] of [['pos', r.pos], ['vel', r.vel], ['mat', r.sim.matR]])
   fs.writeFileSync('%s/' + k, Buffer.from(a.buffer));
 const s = {}; for (const k of Object.keys(r.sim)) { const v = r.sim[k]; if (typeof v === 'number' || typeof v === 'string' || Array.isArray(v)) s[k] = v; }
 console.log(JSON.stringify(s));
-""" % (JS, json.dumps(opt), tmp_path), tmp_path)
+""" % (JS, json.dumps(opt), native_arg(native), tmp_path), tmp_path)
     mine = json.loads(out)
     N = ref["N"]
     assert np.array_equal(np.fromfile(tmp_path / "pos", np.float32), fx["b0_pos"].ravel())
