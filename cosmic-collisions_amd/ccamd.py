@@ -41,7 +41,12 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
-           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error"]
+           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning"]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("cellBlocks", C.c_int32), ("simOrder", C.c_int32), ("cellMs", C.c_double * 4),
+                ("simMs", C.c_double * 2)]
 
 
 class Pile(C.Structure):
@@ -166,6 +171,8 @@ def lib():
             "cc_ff_gradient": ([P, C.c_int, C.c_int, fp, dp], C.c_int),
             "cc_set_far_field": ([P, P, C.c_int, C.c_double], C.c_int), "cc_ff_stats": ([P, dp], C.c_int),
             "cc_relax_piles": ([C.POINTER(Pile), C.c_int, C.POINTER(PileReport)], C.c_int),
+            "cc_autotune": ([P, C.c_int, C.POINTER(TuneReport)], C.c_int),
+            "cc_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
             "cc_bodies_last_error": ([], C.c_char_p),
         }
         for name, (args, res) in sig.items():
@@ -402,6 +409,16 @@ class Context:
         lst = np.zeros(max(int(cap), 1), np.int32)
         self._chk(lib().cc_analyse(self.h, C.byref(args), C.byref(rep), _ptr(lst, C.c_int32)), "cc_analyse")
         return report_dict(rep, lst)
+
+    def autotune(self, reps=20):
+        """cc_autotune at a mesh step: the chosen launch shapes and each candidate's time (ms)."""
+        rep = TuneReport()
+        self._chk(lib().cc_autotune(self.h, int(reps), C.byref(rep)), "cc_autotune")
+        return {"cellBlocks": rep.cellBlocks, "simOrder": rep.simOrder, "cellMs": list(rep.cellMs),
+                "simMs": list(rep.simMs)}
+
+    def set_tuning(self, cell_blocks, sim_order):
+        self._chk(lib().cc_set_tuning(self.h, int(cell_blocks), int(sim_order)), "cc_set_tuning")
 
     BOOKS = ("M", "px", "py", "pz", "Jx", "Jy", "Jz", "KE", "PE", "Q", "iron", "unseated", "full")
 

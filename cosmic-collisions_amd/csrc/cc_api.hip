@@ -55,6 +55,10 @@ struct cc_ctx {
   CycleGraph graphs[4];    // captured step cycles, by what their launches bake in
   int nextGraph = 0;
   int useGraphs = 1;
+  // launch shapes that cannot change a result (cc_autotune, cc_set_tuning): the cell pass's
+  // work-groups, the step pass's work-group order
+  int cellBlocks = 2048;
+  int simOrder = 0;
   // the learned far field, when attached: network, workspace, judge cadence and bound, and per box the gate
   // (1: the network stands in), stats {last error, judged, fallbacks, learned mesh steps}, error sums
   FFNet* ff = nullptr;
@@ -160,6 +164,7 @@ static SimU sim_uniforms(const cc_ctx* c, const cc_sim_args* a) {
   u.cpIron = (float)p.cpIron;
   for (int k = 0; k < 3; k++) { u.spin0[k] = (float)p.spin0[k]; u.spin1[k] = (float)p.spin1[k]; }
   u.grid = c->grid;
+  u.order = c->simOrder;
   u.gm = (float)p.m;
   u.eps2 = (float)(FH * FH);
   u.field = p.field;
@@ -417,7 +422,7 @@ int cc_coarse(cc_ctx* c, int box) {
 }
 int cc_cell(cc_ctx* c, int box) {
   if (!c || box < 0 || box > 1) return CC_EINVAL;
-  launch_cell(c->mesh[box], (float)(0.5 * FH * FH), (float)c->p.m, c->s);
+  launch_cell(c->mesh[box], (float)(0.5 * FH * FH), (float)c->p.m, c->cellBlocks, c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -434,19 +439,19 @@ static int mesh_all(cc_ctx* c, int stepIdx = -1) {
     MeshBuf& m = c->mesh[b];
     float eps2 = (float)(0.5 * FH * FH), gm = (float)c->p.m;
     if (!c->ff || stepIdx < 0) {
-      launch_cell(m, eps2, gm, c->s);
+      launch_cell(m, eps2, gm, c->cellBlocks, c->s);
     } else if (stepIdx % c->ffJudge == 0) {
       float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
       launch_far(m, m.farA, m.farT0, m.farT1, nullptr, c->s);
-      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->s);
+      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, c->s);
       ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, A, T0, T1, nullptr, c->s);
-      launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->s);
+      launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->cellBlocks, c->s);
       launch_judge(c->ffForce, m.force, c->ffSums + 2 * b, c->ffTol, c->ffStats + 4 * b, c->ffGate + b, c->s);
     } else {
       launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
       ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
       launch_count_learned(c->ffGate + b, c->ffStats + 4 * b, c->s);
-      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->s);
+      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, c->s);
     }
     LAUNCHCHK(c);
   }
@@ -570,6 +575,8 @@ static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
   k.push_back((double)c->gb.cur);
   k.push_back((double)(uintptr_t)c->mat);
   k.push_back((double)(uintptr_t)c->ff);
+  k.push_back((double)c->cellBlocks);
+  k.push_back((double)c->simOrder);
   if (c->ff) {
     k.push_back((double)c->ffJudge);
     k.push_back((double)c->ffTol);
@@ -689,6 +696,81 @@ int cc_time_pass(cc_ctx* c, int pass, int reps, double* ms) {
   float t = 0;
   HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
   *ms = t / reps;
+  return 0;
+}
+
+static const int CELL_BLOCKS[4] = { 512, 1024, 2048, 4096 };
+
+int cc_set_tuning(cc_ctx* c, int cellBlocks, int simOrder) {
+  if (!c) return CC_EINVAL;
+  bool known = false;
+  for (int b : CELL_BLOCKS) known = known || b == cellBlocks;
+  if (!known || (simOrder != 0 && simOrder != 1)) FAIL(c, CC_EINVAL, "unknown tuning (%d, %d)", cellBlocks, simOrder);
+  c->cellBlocks = cellBlocks;
+  c->simOrder = simOrder;
+  return 0;
+}
+
+// Each knob timed on the present state in isolation (the mesh pass for the
+// cell pass's grid, the step pass for its order), the others held; a
+// candidate replaces the current choice only when 2 % faster, so timing noise
+// does not move it.  The state is left as it was: the step pass is timed
+// writing the spare buffers without taking them as the state, and the mesh
+// pass only where the next step is a mesh step, which rebuilds the mesh
+// buffers before anything reads them -- all but the monopoles, which the next
+// deposit reads (the strays' pull), so they are put back -- and with no
+// learned far field attached, whose judge keeps statistics across passes.
+int cc_autotune(cc_ctx* c, int reps, cc_tune_report* out) {
+  if (!c || !out || reps <= 0) return CC_EINVAL;
+  if (c->p.field || c->ff || c->stepNo % c->p.meshEvery != 0)
+    FAIL(c, CC_ESTATE, "tune at a mesh step, without constant gravity or a learned far field attached");
+  memset(out, 0, sizeof *out);
+  auto timed = [&](auto&& body, double* ms) -> int {
+    HIPCHK(c, hipEventRecord(c->ev0, c->s));
+    for (int i = 0; i < reps; i++) {
+      int r = body();
+      if (r) return r;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t / reps;
+    return 0;
+  };
+  int r;
+  float4 mono[2];
+  for (int b = 0; b < 2; b++) HIPCHK(c, hipMemcpyAsync(&mono[b], c->mesh[b].mono, 16, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  r = 0;
+  for (int k = 0; k < 4 && !r; k++) {
+    c->cellBlocks = CELL_BLOCKS[k];
+    r = timed([&]() { return mesh_all(c); }, &out->cellMs[k]);
+  }
+  for (int b = 0; b < 2; b++) HIPCHK(c, hipMemcpyAsync(c->mesh[b].mono, &mono[b], 16, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  if (r) return r;
+  int best = 2;
+  for (int k = 0; k < 4; k++)
+    if (out->cellMs[k] < 0.98 * out->cellMs[best]) best = k;
+  c->cellBlocks = CELL_BLOCKS[best];
+  cc_sim_args a;
+  memset(&a, 0, sizeof a);
+  for (int k = 0; k < 2; k++) {
+    c->simOrder = k;
+    auto once = [&]() -> int {
+      SimU u = sim_uniforms(c, &a);
+      int s0 = c->cur, d = c->cur ^ 1;
+      launch_sim(u, c->boxDev, c->pos[s0], c->vel[s0], c->mat, c->q[s0], c->gb.seats, c->mesh[0], c->mesh[1],
+                 c->ppSlot, c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
+      LAUNCHCHK(c);
+      return 0;
+    };
+    if ((r = timed(once, &out->simMs[k]))) return r;
+  }
+  c->simOrder = out->simMs[1] < 0.98 * out->simMs[0] ? 1 : 0;
+  out->cellBlocks = c->cellBlocks;
+  out->simOrder = c->simOrder;
   return 0;
 }
 

@@ -70,6 +70,8 @@ struct SimU {           // the step pass's uniforms, already FP32
   int frozenOn;
   float frozen;
   int reachQ2;          // the seat prefilter's bound in 1/480-cell units, squared (see k_sim)
+  int order;            // 0: work-group b takes particles (b % 8) * (grid / 8) + b / 8 (consecutive
+                        // work-groups' neighbourhoods spread over the XCDs' L2s), 1: particles b
 };
 
 struct CoupleU {        // COUPLE_FS's uniforms (chicxulub.html:260-276), FP32
@@ -114,13 +116,14 @@ void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s);
 void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2,
                     MeshBuf& m, hipStream_t s);
 void launch_coarse(MeshBuf& m, hipStream_t s);
-void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s);
+// the cell pass over the occupied fine texels, `blocks` work-groups of four wavefronts striding over them
+void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, hipStream_t s);
 // the exact far field (skipped while *gate is nonzero) and the per-cell pass from given far-field buffers
 void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
 // the centres of mass of a mesh written from outside: com[t] for mesh[t], t < n
 void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
-                      float eps2, float gm, hipStream_t s);
+                      float eps2, float gm, int blocks, hipStream_t s);
 // the pairwise pass's workspace: each block pair's sums for both of its blocks
 size_t pp_partial_bytes();
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm,

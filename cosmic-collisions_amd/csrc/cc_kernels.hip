@@ -566,14 +566,14 @@ void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, co
   hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, m.ccom, farA, farT0, farT1, gate);
 }
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
-                      float eps2, float gm, hipStream_t s) {
+                      float eps2, float gm, int blocks, hipStream_t s) {
   launch_fill_f32(reinterpret_cast<float*>(force), (size_t)FINE_TEXELS * 4, 0.0f, s);
-  hipLaunchKernelGGL(k_cell, dim3(2048), dim3(256), 0, s, m.fine, m.com, m.coarse, m.ccom, farA, farT0, farT1, m.occ,
+  hipLaunchKernelGGL(k_cell, dim3(blocks), dim3(256), 0, s, m.fine, m.com, m.coarse, m.ccom, farA, farT0, farT1, m.occ,
                      m.occN, eps2, gm, force);
 }
-void launch_cell(MeshBuf& m, float eps2, float gm, hipStream_t s) {
+void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, hipStream_t s) {
   launch_far(m, m.farA, m.farT0, m.farT1, nullptr, s);
-  launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, s);
+  launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, blocks, s);
 }
 
 // ------------------------------------------------------ pairwise P3M (A11) --
@@ -898,7 +898,7 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   __shared__ int cand[CAND_CAP + 1][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts;
                                                   // the last row takes the writes past the list's end
   int lane = threadIdx.x;
-  int me = xcd_block() * blockDim.x + lane;
+  int me = (u.order ? (int)blockIdx.x : xcd_block()) * blockDim.x + lane;
   if (me >= u.N) return;
   float4 P = pos[me];
   float px = P.x, py = P.y, pz = P.z;
