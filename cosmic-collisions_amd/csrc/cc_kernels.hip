@@ -41,14 +41,16 @@ __device__ __forceinline__ float qdiv(float a, float b) {
   return fmaf(r, y, q);
 }
 // a / b correctly rounded for a divisor whose correctly rounded reciprocal y
-// is at hand (a uniform divisor): one residual correction of a * y, exact by
-// Markstein's theorem (y within half an ulp of 1/b and a * y within one ulp of
-// a / b make q + (a - b q) y round to a / b) while the residual a - b q does
-// not underflow, i.e. for dividends above about 2^-100.  Checked exhaustively
-// on the CPU (tests/markstein_check.c) for the pairwise pass's two divisors
-// and every dividend in [2^-80, 1] and 0; its dividends are a distance (0 or
-// above 2^-75: the root of a sum of squares) and the cut minus a distance (0
-// or above 2^-24).
+// is at hand (a uniform divisor, or one shared by several quotients): one
+// residual correction of a * y (Markstein's: y within half an ulp of 1/b makes
+// q + (a - b q) y round to a / b), valid while the residual a - b q does not
+// underflow, i.e. for dividends above about 2^-100 (or 0) and normal
+// quotients.  Checked on the CPU (tests/markstein_check.c): exhaustively for
+// the pairwise pass's two fixed divisors over every dividend in [2^-80, 1],
+// and on random pairs with exponents in [-60, 60] (1.6e10 of them with no
+// difference when this was written).  The pass's dividends are distances and
+// coordinate differences (0, or far above 2^-75) and the interpolated pair
+// table (0, or of the table's size: its potential column is at least 0.039).
 __device__ __forceinline__ float qdiv_by(float a, float b, float y) {
   float q = a * y;
   float r = fmaf(-b, q, a);
@@ -632,7 +634,8 @@ __device__ __forceinline__ void pp_range(int g, int count, int& lo, int& hi) {
 // for the seat at P (mass mi, box bi) from the entry at Q (mass Q.w, box bj),
 // and for Q's seat from P; zero when the pair is beyond the cut.
 __device__ __forceinline__ void pp_terms(float4 P, float bi, float4 Q, float bj, float cut, float cut2, float a2, float a,
-                                         float yCut, float yTap, float tapD, const float2* st, float4& ti, float4& tj) {
+                                         float ya, float yCut, float yTap, float tapD, const float2* st, float4& ti,
+                                         float4& tj) {
   ti = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   tj = ti;
   float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
@@ -650,9 +653,10 @@ __device__ __forceinline__ void pp_terms(float4 P, float bi, float4 Q, float bj,
   float2 tk = st[k], tk1 = st[k1];
   float Tx = gmix(tk.x, tk1.x, t) * same, Ty = gmix(tk.y, tk1.y, t) * same;
   float fn = invN * invN * invN * r - qdiv(Tx, gmax(r2, a2));
-  float rr = gmax(r, 1e-6f);
-  float n0 = qdiv(d0, rr), n1 = qdiv(d1, rr), n2 = qdiv(d2, rr);
-  float ph = -invN - qdiv(Ty, gmax(r, a));
+  float rr = gmax(r, 1e-6f), yr = qdiv(1.0f, rr);   // 1/rr correctly rounded, shared by four quotients
+  float n0 = qdiv_by(d0, rr, yr), n1 = qdiv_by(d1, rr, yr), n2 = qdiv_by(d2, rr, yr);
+  // max(r, a) is rr where r > a (a > 1e-6), else the uniform a with its reciprocal ya
+  float ph = -invN - qdiv_by(Ty, gmax(r, a), r > a ? yr : ya);
   float si = Q.w * fn * taper, sj = P.w * fn * taper;
   ti = make_float4(n0 * si, n1 * si, n2 * si, Q.w * ph * taper);
   tj = make_float4(-(n0 * sj), -(n1 * sj), -(n2 * sj), P.w * ph * taper);
@@ -689,7 +693,7 @@ __global__ void __launch_bounds__(256) k_pp_pairs(const float4* __restrict__ lis
   for (int k = t; k < PP_CHUNK * 4; k += 256) colSum[k] = 0.0f;
   __syncthreads();
   const float cut2 = cut * cut, a = sqrtf(a2), tapD = cut - 0.8f * cut;
-  const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD);
+  const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD), ya = qdiv(1.0f, a);
   // evaluation: lane il of a tile's rows against four consecutive columns
   const int il = t & 31, jl0 = (t >> 6) * 8 + ((t >> 5) & 1) * 4;
   // sums: lanes 0..127 a row's component, lanes 128..255 a column's
@@ -708,7 +712,7 @@ __global__ void __launch_bounds__(256) k_pp_pairs(const float4* __restrict__ lis
         const int jl = jl0 + k, j = tb * PP_TILE + jl;
         float4 ti, tj;
         if (i < nA && j < nB && (both || i != j)) {
-          pp_terms(P, bi, lB[j], bB[j], cut, cut2, a2, a, yCut, yTap, tapD, st, ti, tj);
+          pp_terms(P, bi, lB[j], bB[j], cut, cut2, a2, a, ya, yCut, yTap, tapD, st, ti, tj);
         } else {
           ti = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           tj = ti;

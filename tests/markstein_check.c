@@ -23,7 +23,29 @@ static uint32_t u_of(float f) {
   return u;
 }
 
+/* random mode: n pairs (a, b) with exponents in [-e, e] and random
+ * significands, b any divisor, y its correctly rounded reciprocal */
+static int random_pairs(long n, int e) {
+  uint64_t s = 0x9E3779B97F4A7C15ull;
+  long bad = 0;
+  for (long i = 0; i < n; i++) {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    uint64_t r = s;
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    int ea = (int)((r >> 40) % (uint64_t)(2 * e + 1)) - e, eb = (int)((r >> 24) % (uint64_t)(2 * e + 1)) - e;
+    float a = f_of(((uint32_t)(ea + 127) << 23) | (uint32_t)(r & 0x7FFFFF));
+    float b = f_of(((uint32_t)(eb + 127) << 23) | (uint32_t)(s & 0x7FFFFF));
+    if (r >> 63) a = -a;
+    float y = (float)(1.0 / (double)b);
+    float q = a * y, m = fmaf(fmaf(-b, q, a), y, q);
+    if (m != (float)((double)a / (double)b)) bad++;
+  }
+  printf("random divisors: %ld pairs, %ld differences\n", n, bad);
+  return bad ? 1 : 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && strcmp(argv[1], "random") == 0) return random_pairs(argc > 2 ? atol(argv[2]) : 100000000L, 60);
   float lo = argc > 1 ? strtof(argv[1], NULL) : 0x1p-80f;
   float hi = argc > 2 ? strtof(argv[2], NULL) : 1.0f;
   volatile float cut = (float)(5.0 * (2.0 * 6.0 / 64.0));
