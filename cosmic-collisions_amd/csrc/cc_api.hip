@@ -221,7 +221,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       ALLOC(m.com, (size_t)FINE_TEXELS * 16); ALLOC(m.ccom, 4096 * 16);
       ALLOC(m.coarse, 4096 * 16); ALLOC(m.farA, 4096 * 16); ALLOC(m.farT0, 4096 * 16); ALLOC(m.farT1, 4096 * 16);
       ALLOC(m.mono, 16);
-      ALLOC(m.occ, (size_t)SCRATCH * 4); ALLOC(m.occN, 4);
+      ALLOC(m.occ, (size_t)SCRATCH * 4); ALLOC(m.occN, 4); ALLOC(m.coarseDone, 4);
     }
     ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
@@ -248,6 +248,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       for (float4* t : { m.coarse, m.farA, m.farT0, m.farT1 }) HIPCHK(c, hipMemsetAsync(t, 0, 4096 * 16, c->s));
       HIPCHK(c, hipMemsetAsync(m.mono, 0, 16, c->s));
       HIPCHK(c, hipMemsetAsync(m.occN, 0, 4, c->s));
+      HIPCHK(c, hipMemsetAsync(m.coarseDone, 0, 4, c->s));
     }
     for (int b = 0; b < 2; b++) {
       HIPCHK(c, hipMemsetAsync(c->aux[b], 0, N * 16, c->s));
@@ -410,7 +411,7 @@ int cc_read_seats(cc_ctx* c, uint32_t* seatA, uint32_t* seatB) {
 
 int cc_deposit(cc_ctx* c, int box) {
   if (!c || box < 0 || box > 1) return CC_EINVAL;
-  launch_deposit(c->pos[c->cur], c->mat, c->N, c->boxDev, box, (float)(FH * FH), c->mesh[box], c->s);
+  launch_deposit(c->pos[c->cur], c->mat, c->N, c->boxDev, box, (float)(FH * FH), c->mesh[box], false, c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -422,7 +423,7 @@ int cc_coarse(cc_ctx* c, int box) {
 }
 int cc_cell(cc_ctx* c, int box) {
   if (!c || box < 0 || box > 1) return CC_EINVAL;
-  launch_cell(c->mesh[box], (float)(0.5 * FH * FH), (float)c->p.m, c->cellBlocks, c->s);
+  launch_cell(c->mesh[box], (float)(0.5 * FH * FH), (float)c->p.m, c->cellBlocks, false, c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -435,23 +436,26 @@ static int mesh_all(cc_ctx* c, int stepIdx = -1) {
   int r;
   for (int b = 0; b < 2; b++) {
     if (b == 1 && !(c->box.boxB > 0.5f)) break;
-    if ((r = cc_deposit(c, b)) || (r = cc_coarse(c, b))) return r;
     MeshBuf& m = c->mesh[b];
+    // the deposit also zeroes the force mesh's empty texels, which the cell pass below leaves
+    launch_deposit(c->pos[c->cur], c->mat, c->N, c->boxDev, b, (float)(FH * FH), m, true, c->s);
+    LAUNCHCHK(c);
+    if ((r = cc_coarse(c, b))) return r;
     float eps2 = (float)(0.5 * FH * FH), gm = (float)c->p.m;
     if (!c->ff || stepIdx < 0) {
-      launch_cell(m, eps2, gm, c->cellBlocks, c->s);
+      launch_cell(m, eps2, gm, c->cellBlocks, true, c->s);
     } else if (stepIdx % c->ffJudge == 0) {
       float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
       launch_far(m, m.farA, m.farT0, m.farT1, nullptr, c->s);
-      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, c->s);
+      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
       ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, A, T0, T1, nullptr, c->s);
-      launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->cellBlocks, c->s);
+      launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->cellBlocks, false, c->s);
       launch_judge(c->ffForce, m.force, c->ffSums + 2 * b, c->ffTol, c->ffStats + 4 * b, c->ffGate + b, c->s);
     } else {
       launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
       ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
       launch_count_learned(c->ffGate + b, c->ffStats + 4 * b, c->s);
-      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, c->s);
+      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
     }
     LAUNCHCHK(c);
   }

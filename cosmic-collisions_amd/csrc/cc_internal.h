@@ -92,6 +92,7 @@ struct MeshBuf {
   float4* mono;
   int* occ;                  // occupied fine texels of the last deposit
   int* occN;
+  int* coarseDone;           // the coarse pass's count of finished work-groups (the last sums the monopole, resets it)
 };
 
 struct GridBuf {
@@ -113,17 +114,19 @@ void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBu
 void launch_copy4(const float4* src, float4* dst, int n, hipStream_t s);
 void launch_put_box(const Box& b, Box* dst, hipStream_t s);
 void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s);
+// zeroForce: also zero the force mesh's empty texels (a cell pass follows that writes the rest)
 void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2,
-                    MeshBuf& m, hipStream_t s);
+                    MeshBuf& m, bool zeroForce, hipStream_t s);
 void launch_coarse(MeshBuf& m, hipStream_t s);
-// the cell pass over the occupied fine texels, `blocks` work-groups of four wavefronts striding over them
-void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, hipStream_t s);
+// the cell pass over the occupied fine texels, `blocks` work-groups of four wavefronts striding over them;
+// zeroed: the force mesh's empty texels already zero (the deposit's finish leaves them so)
+void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
 // the exact far field (skipped while *gate is nonzero) and the per-cell pass from given far-field buffers
 void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
 // the centres of mass of a mesh written from outside: com[t] for mesh[t], t < n
 void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
-                      float eps2, float gm, int blocks, hipStream_t s);
+                      float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
 // the pairwise pass's workspace: each block pair's sums for both of its blocks
 size_t pp_partial_bytes();
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm,

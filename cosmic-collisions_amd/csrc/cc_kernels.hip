@@ -355,7 +355,7 @@ __device__ __forceinline__ float4 centre_of(float4 v) {
 // The accumulator to the FP32 mesh (and back to zero), each texel's centre of
 // mass; the occupied texels listed.
 __global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc, float4* fine, float4* com, int* occ,
-                                                        int* occN) {
+                                                        int* occN, float4* force) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= FINE_TEXELS) return;
   unsigned long long* a = acc + (size_t)t * 4;
@@ -366,6 +366,8 @@ __global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc,
   fine[t] = v;
   com[t] = centre_of(v);
   if (t < SCRATCH && v.w > 0.0f) occ[atomicAdd(occN, 1)] = t;
+  // given a force mesh: its empty texels, which the cell pass (it writes the occupied ones and the scratch row) leaves
+  else if (force && t < SCRATCH) force[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 // the centres of a mesh written from outside (fine or coarse)
 __global__ void __launch_bounds__(256) k_centres(const float4* __restrict__ mesh, float4* com, int n) {
@@ -378,11 +380,11 @@ void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s) {
 
 
 void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2, MeshBuf& m,
-                    hipStream_t s) {
+                    bool zeroForce, hipStream_t s) {
   int blocks = (N + 255) / 256;
   hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, b, box, m.mono, eps2, m.acc, m.occN);
   hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.com, m.occ,
-                     m.occN);
+                     m.occN, zeroForce ? m.force : nullptr);
 }
 
 // A plain copy of n float4 (read once, written once): the calibration of the
@@ -410,9 +412,16 @@ void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s) {
 }
 
 // ------------------------------------------------------ coarse + mono (A5) --
-__global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine, float4* coarse, float4* ccom) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4096) return;
+// The coarse mesh (each 4^3 block of fine texels summed in order) and, by the
+// last of the 16 work-groups to finish, the monopole: the coarse texels summed
+// sixteen a thread, then across the wavefronts and the work-group -- the
+// monopole pass of its own before, now without a launch of its own (the
+// counter that finds the last work-group is put back to zero by it).
+__global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine, float4* coarse, float4* ccom,
+                                                float4* mono, int* done) {
+  __shared__ float4 part[4];
+  __shared__ int last;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;   // the grid is 16 x 256: one thread per coarse texel
   int mx = t & 63, my = t >> 6;
   int c0 = mx & 15, c1 = my & 15, c2 = (my >> 4) * 4 + (mx >> 4);
   float4 s = make_float4(0, 0, 0, 0);
@@ -422,22 +431,25 @@ __global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine,
   }
   coarse[t] = s;
   ccom[t] = centre_of(s);
-}
-
-__global__ void __launch_bounds__(256) k_mono(const float4* __restrict__ coarse, float4* mono) {
-  __shared__ float4 part[4];
-  float4 s = make_float4(0, 0, 0, 0);
-  for (int t = threadIdx.x; t < 4096; t += 256) {
-    float4 c = coarse[t];
-    s.x += c.x; s.y += c.y; s.z += c.z; s.w += c.w;
+  __threadfence();   // this work-group's texels visible before it is counted
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();   // every other work-group's texels seen (the L1 is invalidated)
+  float4 m = make_float4(0, 0, 0, 0);
+  for (int u = threadIdx.x; u < 4096; u += 256) {
+    float4 c = coarse[u];
+    m.x += c.x; m.y += c.y; m.z += c.z; m.w += c.w;
   }
-  s.x = wave_sum(s.x); s.y = wave_sum(s.y); s.z = wave_sum(s.z); s.w = wave_sum(s.w);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  m.x = wave_sum(m.x); m.y = wave_sum(m.y); m.z = wave_sum(m.z); m.w = wave_sum(m.w);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     float4 r = part[0];
     for (int w = 1; w < 4; w++) { r.x += part[w].x; r.y += part[w].y; r.z += part[w].z; r.w += part[w].w; }
     mono[0] = r;
+    *done = 0;
   }
 }
 
@@ -561,21 +573,20 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
 }
 
 void launch_coarse(MeshBuf& m, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarse, dim3(16), dim3(256), 0, s, m.fine, m.coarse, m.ccom);
-  hipLaunchKernelGGL(k_mono, dim3(1), dim3(256), 0, s, m.coarse, m.mono);
+  hipLaunchKernelGGL(k_coarse, dim3(16), dim3(256), 0, s, m.fine, m.coarse, m.ccom, m.mono, m.coarseDone);
 }
 void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
   hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, m.ccom, farA, farT0, farT1, gate);
 }
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
-                      float eps2, float gm, int blocks, hipStream_t s) {
-  launch_fill_f32(reinterpret_cast<float*>(force), (size_t)FINE_TEXELS * 4, 0.0f, s);
+                      float eps2, float gm, int blocks, bool zeroed, hipStream_t s) {
+  if (!zeroed) launch_fill_f32(reinterpret_cast<float*>(force), (size_t)FINE_TEXELS * 4, 0.0f, s);
   hipLaunchKernelGGL(k_cell, dim3(blocks), dim3(256), 0, s, m.fine, m.com, m.coarse, m.ccom, farA, farT0, farT1, m.occ,
                      m.occN, eps2, gm, force);
 }
-void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, hipStream_t s) {
+void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, bool zeroed, hipStream_t s) {
   launch_far(m, m.farA, m.farT0, m.farT1, nullptr, s);
-  launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, blocks, s);
+  launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, blocks, zeroed, s);
 }
 
 // ------------------------------------------------------ pairwise P3M (A11) --
