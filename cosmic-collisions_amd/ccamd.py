@@ -172,7 +172,7 @@ def lib():
             "cc_set_far_field": ([P, P, C.c_int, C.c_double], C.c_int), "cc_ff_stats": ([P, dp], C.c_int),
             "cc_relax_piles": ([C.POINTER(Pile), C.c_int, C.POINTER(PileReport)], C.c_int),
             "cc_autotune": ([P, C.c_int, C.POINTER(TuneReport)], C.c_int),
-            "cc_set_tuning": ([P, C.c_int, C.c_int], C.c_int),
+            "cc_set_tuning": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
             "cc_bodies_last_error": ([], C.c_char_p),
         }
         for name, (args, res) in sig.items():
@@ -417,8 +417,8 @@ class Context:
         return {"cellBlocks": rep.cellBlocks, "simOrder": rep.simOrder, "cellMs": list(rep.cellMs),
                 "simMs": list(rep.simMs)}
 
-    def set_tuning(self, cell_blocks, sim_order):
-        self._chk(lib().cc_set_tuning(self.h, int(cell_blocks), int(sim_order)), "cc_set_tuning")
+    def set_tuning(self, cell_blocks, sim_order, fuse_count=1):
+        self._chk(lib().cc_set_tuning(self.h, int(cell_blocks), int(sim_order), int(fuse_count)), "cc_set_tuning")
 
     BOOKS = ("M", "px", "py", "pz", "Jx", "Jy", "Jz", "KE", "PE", "Q", "iron", "unseated", "full")
 

@@ -16,7 +16,7 @@ using namespace cc;
 struct CycleGraph {
   std::vector<double> key;
   hipGraphExec_t exec = nullptr;
-  int curAfter = 0, gridCurAfter = 0;
+  int curAfter = 0, gridCurAfter = 0, countedAfter = 0;
 };
 
 constexpr int SCHED_CAP = 4096;   // step cycles per schedule upload
@@ -59,6 +59,10 @@ struct cc_ctx {
   // work-groups, the step pass's work-group order
   int cellBlocks = 2048;
   int simOrder = 0;
+  // the step pass before a rebuild takes the rebuild's count (fuseCount), and whether the
+  // counters of parity gb.cur ^ 1 now hold a count of the present positions (counted)
+  int fuseCount = 1;
+  int counted = 0;
   // the learned far field, when attached: network, workspace, judge cadence and bound, and per box the gate
   // (1: the network stands in), stats {last error, judged, fallbacks, learned mesh steps}, error sums
   FFNet* ff = nullptr;
@@ -183,6 +187,16 @@ static SimU sim_uniforms(const cc_ctx* c, const cc_sim_args* a) {
   return u;
 }
 
+// A count taken for the next rebuild goes stale when the particles move (or
+// the cell size changes) before it is seated: its counters back to zero.
+static int discard_count(cc_ctx* c) {
+  if (!c->counted) return 0;
+  launch_slots_discard(c->N, c->gb, c->s);
+  LAUNCHCHK(c);
+  c->counted = 0;
+  return 0;
+}
+
 extern "C" {
 
 const char* cc_last_error(cc_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
@@ -208,10 +222,10 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     }
     ALLOC(c->mat, N * 4); ALLOC(c->matTmp, N * 4);
     ALLOC(c->gb.seats, (c->cells + 1) * SLOTS * 4);
-    ALLOC(c->gb.cnt, c->cells * 4);
+    ALLOC(c->gb.cnt[0], c->cells * 4); ALLOC(c->gb.cnt[1], c->cells * 4);
     ALLOC(c->gb.bucket, c->cells * BUCKET_CAP * 4);
     ALLOC(c->gb.occ[0], N * 4); ALLOC(c->gb.occ[1], N * 4);
-    ALLOC(c->gb.segN, 2 * ((N + 255) / 256) * 4);
+    ALLOC(c->gb.segN, 2 * ((N + 63) / 64) * 4);
     ALLOC(c->gb.ovf, N * 8); ALLOC(c->gb.ovfN, 8);
     ALLOC(c->gb.word, N * 4);
     for (int b = 0; b < 2; b++) {
@@ -235,8 +249,8 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     HIPCHK(c, hipHostMalloc((void**)&c->boxHost, (size_t)SCHED_CAP * sizeof(Box), hipHostMallocDefault));
     // the empty state: seats empty, counters and accumulators zero, meshes zero
     launch_fill_u32(c->gb.seats, (c->cells + 1) * SLOTS, EMPTY_SEAT, c->s);
-    HIPCHK(c, hipMemsetAsync(c->gb.cnt, 0, c->cells * 4, c->s));
-    HIPCHK(c, hipMemsetAsync(c->gb.segN, 0, 2 * ((N + 255) / 256) * 4, c->s));
+    for (int b = 0; b < 2; b++) HIPCHK(c, hipMemsetAsync(c->gb.cnt[b], 0, c->cells * 4, c->s));
+    HIPCHK(c, hipMemsetAsync(c->gb.segN, 0, 2 * ((N + 63) / 64) * 4, c->s));
     HIPCHK(c, hipMemsetAsync(c->gb.ovfN, 0, 8, c->s));
     for (int b = 0; b < 2; b++) {
       MeshBuf& m = c->mesh[b];
@@ -303,12 +317,15 @@ int cc_set_params(cc_ctx* c, const cc_params* p) {
   int r = check_params(c, p);
   if (r) return r;
   if (p->N != c->p.N || p->W != c->p.W || p->G != c->p.G) FAIL(c, CC_EINVAL, "N, W and G are fixed at create");
+  if (p->cell != c->p.cell && (r = discard_count(c))) return r;
   c->p = *p;
   return 0;
 }
 
 int cc_upload_state(cc_ctx* c, const float* pos, const float* vel, const float* mat, const float* q) {
   if (!c) return CC_EINVAL;
+  int r;
+  if (pos && (r = discard_count(c))) return r;
   size_t N = (size_t)c->N;
   if (pos) HIPCHK(c, hipMemcpyAsync(c->pos[c->cur], pos, N * 16, hipMemcpyHostToDevice, c->s));
   if (vel) HIPCHK(c, hipMemcpyAsync(c->vel[c->cur], vel, N * 16, hipMemcpyHostToDevice, c->s));
@@ -393,6 +410,8 @@ int cc_write_mesh(cc_ctx* c, int box, int which, const float* in) {
 
 int cc_slots(cc_ctx* c) {
   if (!c) return CC_EINVAL;
+  int r;
+  if ((r = discard_count(c))) return r;
   launch_slots(c->pos[c->cur], c->N, (float)(1.0 / c->p.cell), c->grid, c->gb, c->s);
   LAUNCHCHK(c);
   return 0;
@@ -506,13 +525,19 @@ int cc_read_pp(cc_ctx* c, float* list, float* box, float* force) {
   return 0;
 }
 
-static int sim_pass(cc_ctx* c, const cc_sim_args* a) {
+// The step pass; count: it also takes the next rebuild's count on the
+// positions it writes (the counters of parity gb.cur ^ 1, zero before it).
+static int sim_pass(cc_ctx* c, const cc_sim_args* a, bool count = false) {
+  int r;
+  if ((r = discard_count(c))) return r;   // the particles move: a count of theirs is stale
   SimU u = sim_uniforms(c, a);
   int s = c->cur, d = c->cur ^ 1;
+  SlotsOut o = slots_out(c->gb, c->N, c->gb.cur ^ 1);
   launch_sim(u, c->boxDev, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot,
-             c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
+             c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], count ? &o : nullptr, c->s);
   LAUNCHCHK(c);
   c->cur = d;
+  c->counted = count ? 1 : 0;
   return 0;
 }
 
@@ -538,13 +563,22 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
   }
-  if (n % 2 == 0 && (r = cc_slots(c))) return r;
+  if (n % 2 == 0) {   // a rebuild: seated from the count the last step pass took, or counted and seated here
+    if (c->counted) {
+      launch_slots_seat(c->N, c->gb, c->s);
+      LAUNCHCHK(c);
+      c->counted = 0;
+    } else if ((r = cc_slots(c))) {
+      return r;
+    }
+  }
+  const bool count = c->fuseCount && n % 2 == 1;   // the next step rebuilds: this step pass takes its count
   if (c->p.field) {   // constant gravity: no mesh, and the field acts on every step
     cc_sim_args b = *a;
     b.settle = settle;
     b.kick = kick;
     b.gdt = c->p.dt;
-    return sim_pass(c, &b);
+    return sim_pass(c, &b, count);
   }
   if (mesh && (r = mesh_all(c, n))) return r;
   if (side) HIPCHK(c, hipStreamWaitEvent(c->s, c->ppJoin, 0));
@@ -552,7 +586,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   b.settle = settle;
   b.kick = kick;
   b.gdt = mesh ? c->p.meshEvery * c->p.dt : 0.0;
-  return sim_pass(c, &b);
+  return sim_pass(c, &b, count);
 }
 
 int cc_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
@@ -581,6 +615,8 @@ static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
   k.push_back((double)(uintptr_t)c->ff);
   k.push_back((double)c->cellBlocks);
   k.push_back((double)c->simOrder);
+  k.push_back((double)c->fuseCount);
+  k.push_back((double)c->counted);
   if (c->ff) {
     k.push_back((double)c->ffJudge);
     k.push_back((double)c->ffTol);
@@ -604,7 +640,7 @@ static int cycle_graph(cc_ctx* c, const cc_sim_args* a, CycleGraph** out) {
     hipGraphExecDestroy(g.exec);
     g.exec = nullptr;
   }
-  int stepNo = c->stepNo, cur = c->cur, gcur = c->gb.cur;
+  int stepNo = c->stepNo, cur = c->cur, gcur = c->gb.cur, counted = c->counted;
   HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
   launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
   int r = 0;
@@ -613,7 +649,8 @@ static int cycle_graph(cc_ctx* c, const cc_sim_args* a, CycleGraph** out) {
   hipError_t e = hipStreamEndCapture(c->s, &graph);
   g.curAfter = c->cur;
   g.gridCurAfter = c->gb.cur;
-  c->stepNo = stepNo; c->cur = cur; c->gb.cur = gcur;
+  g.countedAfter = c->counted;
+  c->stepNo = stepNo; c->cur = cur; c->gb.cur = gcur; c->counted = counted;
   if (r) { if (graph) hipGraphDestroy(graph); return r; }
   if (e != hipSuccess) FAIL(c, CC_EDEVICE, "step cycle capture: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
@@ -665,6 +702,7 @@ int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nB
       HIPCHK(c, hipGraphLaunch(g->exec, c->s));
       c->cur = g->curAfter;
       c->gb.cur = g->gridCurAfter;
+      c->counted = g->countedAfter;
       c->stepNo += M;
     }
     i += chunk * M;
@@ -705,13 +743,15 @@ int cc_time_pass(cc_ctx* c, int pass, int reps, double* ms) {
 
 static const int CELL_BLOCKS[4] = { 512, 1024, 2048, 4096 };
 
-int cc_set_tuning(cc_ctx* c, int cellBlocks, int simOrder) {
+int cc_set_tuning(cc_ctx* c, int cellBlocks, int simOrder, int fuseCount) {
   if (!c) return CC_EINVAL;
   bool known = false;
   for (int b : CELL_BLOCKS) known = known || b == cellBlocks;
-  if (!known || (simOrder != 0 && simOrder != 1)) FAIL(c, CC_EINVAL, "unknown tuning (%d, %d)", cellBlocks, simOrder);
+  if (!known || (simOrder != 0 && simOrder != 1) || (fuseCount != 0 && fuseCount != 1))
+    FAIL(c, CC_EINVAL, "unknown tuning (%d, %d, %d)", cellBlocks, simOrder, fuseCount);
   c->cellBlocks = cellBlocks;
   c->simOrder = simOrder;
+  c->fuseCount = fuseCount;
   return 0;
 }
 
@@ -766,7 +806,7 @@ int cc_autotune(cc_ctx* c, int reps, cc_tune_report* out) {
       SimU u = sim_uniforms(c, &a);
       int s0 = c->cur, d = c->cur ^ 1;
       launch_sim(u, c->boxDev, c->pos[s0], c->vel[s0], c->mat, c->q[s0], c->gb.seats, c->mesh[0], c->mesh[1],
-                 c->ppSlot, c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
+                 c->ppSlot, c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], nullptr, c->s);
       LAUNCHCHK(c);
       return 0;
     };
@@ -848,6 +888,8 @@ int cc_rigid(cc_ctx* c, const double bulk0[3], const double bulk1[3], const doub
     s0[k] = (float)c->p.spin0[k]; s1[k] = (float)c->p.spin1[k];
   }
   for (int k = 0; k < 9; k++) { r0[k] = (float)rot0[k]; r1[k] = (float)rot1[k]; }
+  int r;
+  if ((r = discard_count(c))) return r;
   int s = c->cur, d = c->cur ^ 1;
   launch_rigid(c->N, c->pos[s], c->vel[s], c->aux[s], c->q[s], b0, b1, c0, c1, s0, s1, r0, r1, (float)dtF, c->pos[d],
                c->vel[d], c->aux[d], c->q[d], c->s);
@@ -863,6 +905,8 @@ int cc_couple(cc_ctx* c, const cc_couple_args* a) {
   u.vimp = (float)a->vimp; u.aimp = (float)a->aimp; u.c1 = (float)a->c1; u.invMu = (float)a->invMu;
   u.pexp = (float)a->pexp; u.stall = (float)a->stall; u.z = (float)a->z; u.heat = (float)a->heat;
   u.tK = (float)a->tK; u.frozen = (float)a->frozen; u.ambient = (float)a->ambient; u.melt = (float)a->melt;
+  int r;
+  if ((r = discard_count(c))) return r;
   int s = c->cur, d = c->cur ^ 1;
   launch_couple(c->N, u, c->pos[s], c->vel[s], c->aux[s], c->mat, c->pos[d], c->vel[d], c->aux[d], c->q[d], c->s);
   LAUNCHCHK(c);
@@ -885,6 +929,8 @@ int cc_diag(cc_ctx* c, float* out) {
 
 int cc_morton(cc_ctx* c, const double core[3], int32_t* perm, int32_t* into) {
   if (!c || !core) return CC_EINVAL;
+  int r0;
+  if ((r0 = discard_count(c))) return r0;
   int s = c->cur, d = c->cur ^ 1;
   launch_morton(c->pos[s], c->N, core[0], core[1], core[2], c->mkeys, c->mslot, c->mhist, c->mblock, c->mperm,
                 c->minto, c->s);
@@ -949,7 +995,7 @@ struct cc_batch {
   std::vector<hipEvent_t> join;
   std::vector<double> key;
   hipGraphExec_t exec = nullptr;
-  std::vector<int> curAfter, gridCurAfter;
+  std::vector<int> curAfter, gridCurAfter, countedAfter;
   double lastMs = 0.0;
   std::string err;
 };
@@ -980,8 +1026,10 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args) {
     b->exec = nullptr;
   }
   size_t n = b->ctx.size();
-  std::vector<int> stepNo(n), cur(n), gcur(n);
-  for (size_t i = 0; i < n; i++) { stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; }
+  std::vector<int> stepNo(n), cur(n), gcur(n), counted(n);
+  for (size_t i = 0; i < n; i++) {
+    stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; counted[i] = b->ctx[i]->counted;
+  }
   BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipEventRecord(b->fork, b->s);
   int r = 0;
@@ -999,10 +1047,11 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args) {
   hipError_t ee = hipStreamEndCapture(b->s, &graph);
   b->curAfter.assign(n, 0);
   b->gridCurAfter.assign(n, 0);
+  b->countedAfter.assign(n, 0);
   for (size_t i = 0; i < n; i++) {
     cc_ctx* c = b->ctx[i];
-    b->curAfter[i] = c->cur; b->gridCurAfter[i] = c->gb.cur;
-    c->stepNo = stepNo[i]; c->cur = cur[i]; c->gb.cur = gcur[i];
+    b->curAfter[i] = c->cur; b->gridCurAfter[i] = c->gb.cur; b->countedAfter[i] = c->counted;
+    c->stepNo = stepNo[i]; c->cur = cur[i]; c->gb.cur = gcur[i]; c->counted = counted[i];
   }
   if (r || e != hipSuccess || ee != hipSuccess) {
     if (graph) hipGraphDestroy(graph);
@@ -1125,7 +1174,7 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
       BCHK(b, hipGraphLaunch(b->exec, b->s));
       for (size_t i = 0; i < m; i++) {
         cc_ctx* c = b->ctx[i];
-        c->cur = b->curAfter[i]; c->gb.cur = b->gridCurAfter[i]; c->stepNo += M;
+        c->cur = b->curAfter[i]; c->gb.cur = b->gridCurAfter[i]; c->counted = b->countedAfter[i]; c->stepNo += M;
       }
     }
     i0 += chunk * M;

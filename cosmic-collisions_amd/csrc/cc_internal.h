@@ -97,20 +97,38 @@ struct MeshBuf {
 
 struct GridBuf {
   uint32_t* seats;           // [(cells + 1) * 8]: the last cell stays empty (a safe target for skipped loads)
-  int* cnt;                  // [cells]
+  int* cnt[2];               // [cells] each: arrivals per cell, one set per rebuild parity
   int* bucket;               // [cells*BUCKET_CAP]
-  int* occ[2];               // occupied cells, two lists used in turn, in 256-entry segments (one per work-group)
-  int* segN;                 // [2][blocks]: entries of each segment
+  int* occ[2];               // occupied cells, one list per parity, in 64-entry segments (one per wavefront)
+  int* segN;                 // [2][ceil(N / 64)]: entries of each segment
   int2* ovf;                 // (cell, particle) past a cell's bucket
   int* ovfN;                 // [2]: the overflow count of each rebuild parity
   uint32_t* word;            // [N] seat word of each particle
-  int cur;                   // which occupied list is the current one
+  int cur;                   // the parity of the last rebuild (whose seats are in place)
+};
+
+// where a count writes: the arrays of one parity
+struct SlotsOut {
+  int* cnt;
+  int* bucket;
+  int* occ;
+  int* segN;
+  int2* ovf;
+  int* ovfN;
+  uint32_t* word;
 };
 
 // ---- launches (cc_kernels.hip) ----
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+// a rebuild: the count (into parity cur ^ 1) and the seat pass (which makes it cur)
 void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
+void launch_slots_count(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
+void launch_slots_seat(int N, GridBuf& gb, hipStream_t s);
+// a count of parity cur ^ 1 that will not be seated: its counters back to zero
+void launch_slots_discard(int N, GridBuf& gb, hipStream_t s);
+// the arrays a count of that parity writes
+SlotsOut slots_out(GridBuf& gb, int N, int parity);
 void launch_copy4(const float4* src, float4* dst, int n, hipStream_t s);
 void launch_put_box(const Box& b, Box* dst, hipStream_t s);
 void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s);
@@ -132,9 +150,11 @@ size_t pp_partial_bytes();
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm,
                float cut, float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force,
                hipStream_t s);
+// count: the next rebuild's count taken on the positions this pass writes (null: none)
 void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
-                const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s);
+                const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut* count,
+                hipStream_t s);
 void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux, const float* q,
                   const float* bulk0, const float* bulk1, const float* cen0, const float* cen1,
                   const float* spin0, const float* spin1, const float* rot0, const float* rot1, float dt,

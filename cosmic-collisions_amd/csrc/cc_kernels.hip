@@ -153,66 +153,81 @@ void launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
 }
 
 // ----------------------------------------------------- contact grid (A1-A3) --
-// One pass of a rebuild, two independent jobs (they touch disjoint arrays):
-//  - the seats of the cells the last rebuild listed back to empty (its lists
-//    are per work-group segments: segment b is occ[b*256 ...], segN[b]
-//    entries), so every cell's seats are empty but the new ones';
-//  - per particle: its hashed cell, its seat word (index, place in the cell
-//    to a fifteenth, bit 30; src/glsl-sim.js:133-147), an arrival in the
-//    cell's bucket, the cell listed in the work-group's segment on its first
-//    arrival.  Arrivals past a bucket go to the overflow list, whose counter
-//    (one of two used in turn) the previous rebuild's seat pass set to zero.
-__global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ pos, int N, float invCell, Grid g,
-                                                     int* cnt, int* bucket, int* occ, int* segN, int2* ovf, int* ovfN,
-                                                     uint32_t* word, uint32_t* seats,
-                                                     const int* __restrict__ occPrev,
-                                                     const int* __restrict__ segPrev) {
-  __shared__ int listed;
-  if (threadIdx.x == 0) listed = 0;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < N && (t & 255) < segPrev[t >> 8]) {
-    uint4 e = make_uint4(EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT);
-    uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)occPrev[t] * SLOTS);
-    sv[0] = e; sv[1] = e;
-  }
-  __syncthreads();
-  int b = xcd_block();
-  int i = b * blockDim.x + threadIdx.x;
-  bool live = i < N;
-  int old = -1, h = 0;
-  if (live) {
-    float4 p = pos[i];
-    float gx = p.x * invCell, gy = p.y * invCell, gz = p.z * invCell;
-    float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
-    h = hash_cell(g, (int)fx, (int)fy, (int)fz);
-    uint32_t qx = (uint32_t)gclamp((gx - fx) * 15.0f, 0.0f, 14.0f);
-    uint32_t qy = (uint32_t)gclamp((gy - fy) * 15.0f, 0.0f, 14.0f);
-    uint32_t qz = (uint32_t)gclamp((gz - fz) * 15.0f, 0.0f, 14.0f);
-    int B = g.IB;
-    word[i] = (uint32_t)i | (qx << B) | (qy << (B + 4)) | (qz << (B + 8)) | (B == 18 ? 0x40000000u : 0u);
-    old = atomicAdd(&cnt[h], 1);
-  }
-  if (old == 0) occ[(size_t)b * 256 + atomicAdd(&listed, 1)] = h;   // a cell's first arrival lists it
-  __syncthreads();
-  if (threadIdx.x == 0) segN[b] = listed;
-  if (!live) return;
-  if (old < BUCKET_CAP) bucket[(size_t)h * BUCKET_CAP + old] = i;
-  else ovf[atomicAdd(ovfN, 1)] = make_int2(h, i);
+// A rebuild (src/glsl-sim.js:133-147, the depth-peeled seat passes) in two
+// steps: a count and a seat pass.  The count takes each particle's hashed
+// cell and seat word (index, place in the cell to a fifteenth, bit 30), an
+// arrival in the cell's counter and bucket (arrivals past the bucket to the
+// overflow list), and lists the cell on its first arrival in the wavefront's
+// 64-entry segment of the occupied list.  The count runs as a pass of its own
+// or at the end of the step pass before a rebuild, on the positions that step
+// writes (k_sim<true>): the same positions, the same counts.  Counters, lists
+// and overflow counters come in two sets used by alternate rebuilds.
+
+// the particle at (px, py, pz) with index i: its hashed cell h and seat word
+__device__ __forceinline__ uint32_t seat_word(float px, float py, float pz, float invCell, const Grid& g, int i,
+                                              int& h) {
+  float gx = px * invCell, gy = py * invCell, gz = pz * invCell;
+  float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+  h = hash_cell(g, (int)fx, (int)fy, (int)fz);
+  uint32_t qx = (uint32_t)gclamp((gx - fx) * 15.0f, 0.0f, 14.0f);
+  uint32_t qy = (uint32_t)gclamp((gy - fy) * 15.0f, 0.0f, 14.0f);
+  uint32_t qz = (uint32_t)gclamp((gz - fz) * 15.0f, 0.0f, 14.0f);
+  int B = g.IB;
+  return (uint32_t)i | (qx << B) | (qy << (B + 4)) | (qz << (B + 8)) | (B == 18 ? 0x40000000u : 0u);
 }
 
-// Per occupied cell: the eight lowest arrival indices, ascending -- what the
-// reference's depth peel leaves in seats 0..7.
-__global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN, int N,
-                                                    int* cnt,
+// One particle's arrival, by a lane of a wavefront whose lanes hold particles
+// i = 64 s .. 64 s + 63 (segment s); lanes past N must have left.
+__device__ __forceinline__ void count_arrival(int i, uint32_t w, int h, const SlotsOut& o) {
+  o.word[i] = w;
+  int old = atomicAdd(&o.cnt[h], 1);
+  const unsigned long long first = __ballot(old == 0);
+  const int seg = i >> 6;
+  if (old == 0)   // a cell's first arrival lists it, in lane order within the segment
+    o.occ[(seg << 6) + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(first >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)first, 0u))] = h;
+  if ((i & 63) == 0) o.segN[seg] = __popcll(first);
+  if (old < BUCKET_CAP) o.bucket[(size_t)h * BUCKET_CAP + old] = i;
+  else o.ovf[atomicAdd(o.ovfN, 1)] = make_int2(h, i);
+}
+
+__global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ pos, int N, float invCell, Grid g,
+                                                     SlotsOut o) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float4 p = pos[i];
+  int h;
+  uint32_t w = seat_word(p.x, p.y, p.z, invCell, g, i, h);
+  count_arrival(i, w, h, o);
+}
+
+// The seat pass.  Per cell of this rebuild: the eight lowest arrival indices,
+// ascending -- what the reference's depth peel leaves in seats 0..7.  Per cell
+// of the previous rebuild: its seats back to empty where nothing arrived this
+// time (a cell with arrivals has all eight seats written), and its counter of
+// that rebuild back to zero for the rebuild after this one.  Neither job
+// writes what the other reads.
+__global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN,
+                                                    const int* __restrict__ occPrev, const int* __restrict__ segPrev,
+                                                    int N, const int* __restrict__ cnt, int* cntPrev,
                                                     const int* __restrict__ bucket, const int2* __restrict__ ovf,
                                                     const int* ovfN, int* ovfNOther,
                                                     const uint32_t* __restrict__ word, uint32_t* seats) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) *ovfNOther = 0;   // the overflow counter of the next rebuild
-  if (t >= N || (t & 255) >= segN[t >> 8]) return;
+  if (t >= N) return;
+  if ((t & 63) < segPrev[t >> 6]) {
+    int h = occPrev[t];
+    if (cnt[h] == 0) {
+      uint4 e = make_uint4(EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT);
+      uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)h * SLOTS);
+      sv[0] = e; sv[1] = e;
+    }
+    cntPrev[h] = 0;
+  }
+  if ((t & 63) >= segN[t >> 6]) return;
   int h = occ[t];
   int n = cnt[h];
-  cnt[h] = 0;
   int m = n < BUCKET_CAP ? n : BUCKET_CAP;
   int idx[BUCKET_CAP];
   const int4* b4 = reinterpret_cast<const int4*>(bucket + (size_t)h * BUCKET_CAP);
@@ -242,20 +257,54 @@ __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ,
     out[k] = best == 0x7fffffff ? EMPTY_SEAT : word[best];
     if (best != 0x7fffffff) last = best;
   }
-  uint4* s = reinterpret_cast<uint4*>(seats + (size_t)h * SLOTS);
-  s[0] = make_uint4(out[0], out[2], out[4], out[6]);
-  s[1] = make_uint4(out[1], out[3], out[5], out[7]);
+  uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)h * SLOTS);
+  sv[0] = make_uint4(out[0], out[2], out[4], out[6]);
+  sv[1] = make_uint4(out[1], out[3], out[5], out[7]);
+}
+
+// A count that will not be seated (the particles moved since): its counters
+// and overflow counter back to zero.
+__global__ void __launch_bounds__(256) k_slots_discard(const int* __restrict__ occ, const int* __restrict__ segN, int N,
+                                                       int* cnt, int* ovfN) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *ovfN = 0;
+  if (t < N && (t & 63) < segN[t >> 6]) cnt[occ[t]] = 0;
+}
+
+SlotsOut slots_out(GridBuf& gb, int N, int parity) {
+  SlotsOut o;
+  o.cnt = gb.cnt[parity];
+  o.bucket = gb.bucket;
+  o.occ = gb.occ[parity];
+  o.segN = gb.segN + (size_t)parity * ((N + 63) / 64);
+  o.ovf = gb.ovf;
+  o.ovfN = gb.ovfN + parity;
+  o.word = gb.word;
+  return o;
+}
+
+void launch_slots_count(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s) {
+  int blocks = (N + 255) / 256;
+  blocks = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
+  hipLaunchKernelGGL(k_slots_count, dim3(blocks), dim3(256), 0, s, pos, N, invCell, g, slots_out(gb, N, gb.cur ^ 1));
+}
+
+void launch_slots_seat(int N, GridBuf& gb, hipStream_t s) {
+  int prev = gb.cur, next = gb.cur ^ 1, blocks = (N + 255) / 256;
+  SlotsOut n = slots_out(gb, N, next), p = slots_out(gb, N, prev);
+  hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, n.occ, n.segN, p.occ, p.segN, N, n.cnt, p.cnt,
+                     gb.bucket, gb.ovf, n.ovfN, p.ovfN, gb.word, gb.seats);
+  gb.cur = next;
+}
+
+void launch_slots_discard(int N, GridBuf& gb, hipStream_t s) {
+  SlotsOut n = slots_out(gb, N, gb.cur ^ 1);
+  hipLaunchKernelGGL(k_slots_discard, dim3((N + 255) / 256), dim3(256), 0, s, n.occ, n.segN, N, n.cnt, n.ovfN);
 }
 
 void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s) {
-  int prev = gb.cur, next = gb.cur ^ 1, blocks = (N + 255) / 256;
-  int* segPrev = gb.segN + (size_t)prev * blocks;
-  int* segNext = gb.segN + (size_t)next * blocks;
-  hipLaunchKernelGGL(k_slots_count, dim3(blocks), dim3(256), 0, s, pos, N, invCell, g, gb.cnt, gb.bucket,
-                     gb.occ[next], segNext, gb.ovf, gb.ovfN + next, gb.word, gb.seats, gb.occ[prev], segPrev);
-  hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, gb.occ[next], segNext, N, gb.cnt, gb.bucket,
-                     gb.ovf, gb.ovfN + next, gb.ovfN + prev, gb.word, gb.seats);
-  gb.cur = next;
+  launch_slots_count(pos, N, invCell, g, gb, s);
+  launch_slots_seat(N, gb, s);
 }
 
 // ----------------------------------------------------------- deposit (A4) --
@@ -901,6 +950,7 @@ __device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int b
   return __mul24(ex, ex) + __mul24(ey, ey) + __mul24(ez, ez);
 }
 
+template <bool COUNT>
 __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict__ bxp, float touch2r,
                                                    const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float* __restrict__ mat,
@@ -909,7 +959,7 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
                                                    const float4* __restrict__ force1, const float4* __restrict__ mono1,
                                                    const float* __restrict__ ppSlot,
                                                    const float4* __restrict__ ppForce, float4* oPos, float4* oVel,
-                                                   float4* oAux, float* oQ) {
+                                                   float4* oAux, float* oQ, SlotsOut cnt) {
   __shared__ int cand[CAND_CAP + 1][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts;
                                                   // the last row takes the writes past the list's end
   int lane = threadIdx.x;
@@ -1110,6 +1160,11 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   oVel[me] = make_float4(vx, vy, vz, u.tK * Q / ci);
   oAux[me] = make_float4(gvx, gvy, gvz, nb);
   oQ[me] = Q;
+  if (COUNT) {   // the next rebuild's count, on the position just written (a block is one wavefront)
+    int h;
+    uint32_t w = seat_word(px, py, pz, u.invCell, u.grid, me, h);
+    count_arrival(me, w, h, cnt);
+  }
 }
 
 // the smallest r^2 whose correctly rounded square root is not below the
@@ -1123,11 +1178,16 @@ float touch_r2(float touch) {
 
 void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
-                const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, hipStream_t s) {
+                const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut* count,
+                hipStream_t s) {
   int blocks = (u.N + SIM_BLOCK - 1) / SIM_BLOCK;
   blocks = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
-  hipLaunchKernelGGL(k_sim, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, touch_r2(u.touch), pos, vel, mat, q, seats,
-                     m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ);
+  if (count)
+    hipLaunchKernelGGL(k_sim<true>, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, touch_r2(u.touch), pos, vel, mat, q,
+                       seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, *count);
+  else
+    hipLaunchKernelGGL(k_sim<false>, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, touch_r2(u.touch), pos, vel, mat, q,
+                       seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, SlotsOut{});
 }
 
 // ----------------------------------------------------------- rigid (A14) --

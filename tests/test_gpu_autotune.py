@@ -1,5 +1,7 @@
 """The launch-shape tuning (cc_set_tuning, cc_autotune): every cell-pass grid
-and step-pass work-group order gives bit-identical meshes and states, an
+and step-pass work-group order, and the rebuild's count taken by the step
+pass before it or by a pass of its own, give bit-identical meshes and
+states (and the same seats), an
 autotuned context runs on bit for bit with an untuned one from the same
 state, and autotune refuses where its timing would disturb the run.  Needs
 the GPU."""
@@ -32,22 +34,62 @@ def boxes(fx, n, start=0):
 def test_every_tuning_gives_the_same_bits(name):
     fx = gu.fixture("c1_" + name)
     ref = None
-    for cb in CELL:
-        for order in (0, 1):
-            print(name, "tuning", cb, order, flush=True)
-            ctx = make(fx, True)
-            try:
-                ctx.set_tuning(cb, order)
-                ctx.mesh()
-                force = [ctx.read_mesh(b, "force") for b in (0, 1)]
-                ctx.step_n(20, ccamd.sim_args(), boxes(fx, 20))
-                got = force + list(ctx.download()[:4])
-            finally:
-                ctx.close()
-            if ref is None:
-                ref = got
-            for a, b, what in zip(ref, got, ("force0", "force1", "pos", "vel", "aux", "q")):
-                assert np.array_equal(a, b), (cb, order, what)
+    tunings = [(cb, order, 1) for cb in CELL for order in (0, 1)] + [(2048, 0, 0)]
+    for cb, order, fuse in tunings:
+        print(name, "tuning", cb, order, fuse, flush=True)
+        ctx = make(fx, True)
+        try:
+            ctx.set_tuning(cb, order, fuse)
+            ctx.mesh()
+            force = [ctx.read_mesh(b, "force") for b in (0, 1)]
+            ctx.step_n(20, ccamd.sim_args(), boxes(fx, 20))
+            got = force + list(ctx.download()[:4]) + list(ctx.read_seats())
+        finally:
+            ctx.close()
+        if ref is None:
+            ref = got
+        for a, b, what in zip(ref, got, ("force0", "force1", "pos", "vel", "aux", "q", "seatA", "seatB")):
+            assert np.array_equal(a, b), (cb, order, fuse, what)
+
+
+def test_count_in_step_pass_through_interruptions():
+    """The rebuild's count taken by the step pass before it, against a count of
+    its own, through what can leave a taken count stale: eager steps, a change
+    of step number, an explicit step pass and rebuild, a Morton sort, an
+    upload, and graph cycles starting on odd and even steps."""
+    fx = gu.fixture("c1_impact")
+    fused, alone = make(fx, True), make(fx, True)
+    alone.set_tuning(2048, 0, 0)
+    args = ccamd.sim_args()
+    core = np.zeros(3)
+    try:
+        def both(what, f):
+            print("interruption:", what, flush=True)
+            for ctx in (fused, alone):
+                f(ctx)
+            a, b = fused.download(), alone.download()
+            for x, y, k in zip(a, b, ("pos", "vel", "aux", "q", "mat")):
+                assert np.array_equal(x, y), (what, k)
+            for x, y in zip(fused.read_seats(), alone.read_seats()):
+                assert np.array_equal(x, y), (what, "seats")
+        both("13 steps", lambda c: c.step_n(13, args, boxes(fx, 13)))
+        both("eager step", lambda c: c.step(0, 0, args))
+        both("eager step", lambda c: c.step(0, 0, args))
+        both("19 steps", lambda c: c.step_n(19, args, boxes(fx, 19, 15)))
+        both("step number moved", lambda c: setattr(c, "step_no", 41))
+        both("9 steps", lambda c: c.step_n(9, args, boxes(fx, 9, 41)))
+        both("explicit step pass", lambda c: c.sim(args))
+        both("5 steps", lambda c: c.step_n(5, args, boxes(fx, 5, 50)))
+        both("explicit rebuild", lambda c: c.slots())
+        both("7 steps", lambda c: c.step_n(7, args, boxes(fx, 7, 55)))
+        both("Morton sort", lambda c: c.morton(core))
+        both("11 steps", lambda c: c.step_n(11, args, boxes(fx, 11, 62)))
+        P, V, A, Q, M = fused.download()
+        both("upload", lambda c: c.upload(P, V, M, Q))
+        both("16 steps", lambda c: c.step_n(16, args, boxes(fx, 16, 73)))
+    finally:
+        fused.close()
+        alone.close()
 
 
 def test_autotune_leaves_the_run_unchanged():
@@ -68,7 +110,7 @@ def test_autotune_leaves_the_run_unchanged():
         with pytest.raises(ccamd.CCError):
             tuned.autotune(reps=2)
         with pytest.raises(ccamd.CCError):
-            tuned.set_tuning(3000, 0)
+            tuned.set_tuning(3000, 0, 1)
     finally:
         tuned.close()
         plain.close()
