@@ -249,7 +249,8 @@ int cc_batch_last_ms(cc_batch* b, double* ms);
  * (fuseCount 1: by the step pass before it, on the positions that pass
  * writes; 0: by a pass of its own at the rebuild; the same counts).  cc_autotune times every candidate on the
  * present state and keeps the fastest (a candidate must beat the current
- * choice by 2 %), the state as it was; it is called where the next step is a
+ * choice by 2 %, the launch order by 5 %: it fetches about four times the
+ * HBM bytes), the state as it was; it is called where the next step is a
  * mesh step, without constant gravity or a learned far field attached
  * (CC_ESTATE otherwise); it leaves fuseCount as it is.  cc_set_tuning sets all
  * three outright.  Defaults 2048, 0 and 1.  Captured step cycles are
