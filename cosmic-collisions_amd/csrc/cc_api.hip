@@ -227,7 +227,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     ALLOC(c->gb.occ[0], N * 4); ALLOC(c->gb.occ[1], N * 4);
     ALLOC(c->gb.segN, 2 * ((N + 63) / 64) * 4);
     ALLOC(c->gb.ovf, N * 8); ALLOC(c->gb.ovfN, 8);
-    ALLOC(c->gb.word, N * 4);
+    c->gb.idxMask = (1u << c->grid.IB) - 1u;
     for (int b = 0; b < 2; b++) {
       MeshBuf& m = c->mesh[b];
       ALLOC(m.acc, (size_t)FINE_TEXELS * 4 * 8);

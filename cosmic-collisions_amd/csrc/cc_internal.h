@@ -98,24 +98,23 @@ struct MeshBuf {
 struct GridBuf {
   uint32_t* seats;           // [(cells + 1) * 8]: the last cell stays empty (a safe target for skipped loads)
   int* cnt[2];               // [cells] each: arrivals per cell, one set per rebuild parity
-  int* bucket;               // [cells*BUCKET_CAP]
+  uint32_t* bucket;          // [cells*BUCKET_CAP]: the arrivals' seat words (the index is their low IB bits)
   int* occ[2];               // occupied cells, one list per parity, in 64-entry segments (one per wavefront)
   int* segN;                 // [2][ceil(N / 64)]: entries of each segment
-  int2* ovf;                 // (cell, particle) past a cell's bucket
+  int2* ovf;                 // (cell, seat word) past a cell's bucket
   int* ovfN;                 // [2]: the overflow count of each rebuild parity
-  uint32_t* word;            // [N] seat word of each particle
+  uint32_t idxMask;          // (1 << IB) - 1
   int cur;                   // the parity of the last rebuild (whose seats are in place)
 };
 
 // where a count writes: the arrays of one parity
 struct SlotsOut {
   int* cnt;
-  int* bucket;
+  uint32_t* bucket;
   int* occ;
   int* segN;
   int2* ovf;
   int* ovfN;
-  uint32_t* word;
 };
 
 // ---- launches (cc_kernels.hip) ----

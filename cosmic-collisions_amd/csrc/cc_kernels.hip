@@ -158,7 +158,8 @@ void launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
 // cell and seat word (index, place in the cell to a fifteenth, bit 30), an
 // arrival in the cell's counter and bucket (arrivals past the bucket to the
 // overflow list), and lists the cell on its first arrival in the wavefront's
-// 64-entry segment of the occupied list.  The count runs as a pass of its own
+// 64-entry segment of the occupied list; the bucket holds the seat words
+// themselves (the index is their low bits).  The count runs as a pass of its own
 // or at the end of the step pass before a rebuild, on the positions that step
 // writes (k_sim<true>): the same positions, the same counts.  Counters, lists
 // and overflow counters come in two sets used by alternate rebuilds.
@@ -179,7 +180,6 @@ __device__ __forceinline__ uint32_t seat_word(float px, float py, float pz, floa
 // One particle's arrival, by a lane of a wavefront whose lanes hold particles
 // i = 64 s .. 64 s + 63 (segment s); lanes past N must have left.
 __device__ __forceinline__ void count_arrival(int i, uint32_t w, int h, const SlotsOut& o) {
-  o.word[i] = w;
   int old = atomicAdd(&o.cnt[h], 1);
   const unsigned long long first = __ballot(old == 0);
   const int seg = i >> 6;
@@ -187,8 +187,8 @@ __device__ __forceinline__ void count_arrival(int i, uint32_t w, int h, const Sl
     o.occ[(seg << 6) + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(first >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)first, 0u))] = h;
   if ((i & 63) == 0) o.segN[seg] = __popcll(first);
-  if (old < BUCKET_CAP) o.bucket[(size_t)h * BUCKET_CAP + old] = i;
-  else o.ovf[atomicAdd(o.ovfN, 1)] = make_int2(h, i);
+  if (old < BUCKET_CAP) o.bucket[(size_t)h * BUCKET_CAP + old] = w;
+  else o.ovf[atomicAdd(o.ovfN, 1)] = make_int2(h, (int)w);
 }
 
 __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ pos, int N, float invCell, Grid g,
@@ -210,9 +210,9 @@ __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ 
 __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN,
                                                     const int* __restrict__ occPrev, const int* __restrict__ segPrev,
                                                     int N, const int* __restrict__ cnt, int* cntPrev,
-                                                    const int* __restrict__ bucket, const int2* __restrict__ ovf,
-                                                    const int* ovfN, int* ovfNOther,
-                                                    const uint32_t* __restrict__ word, uint32_t* seats) {
+                                                    const uint32_t* __restrict__ bucket, const int2* __restrict__ ovf,
+                                                    const int* ovfN, int* ovfNOther, uint32_t idxMask,
+                                                    uint32_t* seats) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) *ovfNOther = 0;   // the overflow counter of the next rebuild
   if (t >= N) return;
@@ -229,15 +229,15 @@ __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ,
   int h = occ[t];
   int n = cnt[h];
   int m = n < BUCKET_CAP ? n : BUCKET_CAP;
-  int idx[BUCKET_CAP];
-  const int4* b4 = reinterpret_cast<const int4*>(bucket + (size_t)h * BUCKET_CAP);
+  uint32_t wd[BUCKET_CAP];   // the arrivals' seat words
+  int ix[BUCKET_CAP];        // and their particle indices (past the arrivals: above any index)
+  const uint4* b4 = reinterpret_cast<const uint4*>(bucket + (size_t)h * BUCKET_CAP);
 #pragma unroll
   for (int q = 0; q < BUCKET_CAP / 4; q++) {
-    int4 v = (q * 4 < m) ? b4[q] : make_int4(0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff);
-    idx[q * 4] = q * 4 < m ? v.x : 0x7fffffff;
-    idx[q * 4 + 1] = q * 4 + 1 < m ? v.y : 0x7fffffff;
-    idx[q * 4 + 2] = q * 4 + 2 < m ? v.z : 0x7fffffff;
-    idx[q * 4 + 3] = q * 4 + 3 < m ? v.w : 0x7fffffff;
+    uint4 v = (q * 4 < m) ? b4[q] : make_uint4(0u, 0u, 0u, 0u);
+    wd[q * 4] = v.x; wd[q * 4 + 1] = v.y; wd[q * 4 + 2] = v.z; wd[q * 4 + 3] = v.w;
+#pragma unroll
+    for (int u = 0; u < 4; u++) ix[q * 4 + u] = q * 4 + u < m ? (int)(wd[q * 4 + u] & idxMask) : 0x7fffffff;
   }
   int nov = n > BUCKET_CAP ? *ovfN : 0;
   uint32_t out[SLOTS];
@@ -245,16 +245,19 @@ __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ,
 #pragma unroll
   for (int k = 0; k < SLOTS; k++) {
     int best = 0x7fffffff;
+    uint32_t bw = EMPTY_SEAT;
 #pragma unroll
     for (int q = 0; q < BUCKET_CAP; q++) {
-      int v = idx[q];
-      best = (v > last && v < best) ? v : best;
+      bool take = ix[q] > last && ix[q] < best;
+      best = take ? ix[q] : best;
+      bw = take ? wd[q] : bw;
     }
     for (int o = 0; o < nov; o++) {       // a cell past its bucket: its other members
       int2 e = ovf[o];
-      if (e.x == h && e.y > last && e.y < best) best = e.y;
+      int v = (int)((uint32_t)e.y & idxMask);
+      if (e.x == h && v > last && v < best) { best = v; bw = (uint32_t)e.y; }
     }
-    out[k] = best == 0x7fffffff ? EMPTY_SEAT : word[best];
+    out[k] = bw;
     if (best != 0x7fffffff) last = best;
   }
   uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)h * SLOTS);
@@ -279,7 +282,6 @@ SlotsOut slots_out(GridBuf& gb, int N, int parity) {
   o.segN = gb.segN + (size_t)parity * ((N + 63) / 64);
   o.ovf = gb.ovf;
   o.ovfN = gb.ovfN + parity;
-  o.word = gb.word;
   return o;
 }
 
@@ -293,7 +295,7 @@ void launch_slots_seat(int N, GridBuf& gb, hipStream_t s) {
   int prev = gb.cur, next = gb.cur ^ 1, blocks = (N + 255) / 256;
   SlotsOut n = slots_out(gb, N, next), p = slots_out(gb, N, prev);
   hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, n.occ, n.segN, p.occ, p.segN, N, n.cnt, p.cnt,
-                     gb.bucket, gb.ovf, n.ovfN, p.ovfN, gb.word, gb.seats);
+                     gb.bucket, gb.ovf, n.ovfN, p.ovfN, gb.idxMask, gb.seats);
   gb.cur = next;
 }
 
