@@ -5,9 +5,11 @@
 //                    state textures.
 //   contact grid   : seats uint32[cells*8] -- the eight seat words of hashed
 //                    cell h (h = atlas texel y*AW + x) contiguous, so one cell
-//                    is two 16-byte loads; cnt int[cells], bucket int[cells*CAP]
-//                    (arrival lists of the current rebuild), occupied-cell
-//                    lists (two, alternating), an overflow list.
+//                    is two 16-byte loads (one more cell, always empty, past
+//                    the grid); cnt int[cells] x 2 and occupied-cell lists x 2
+//                    (one of each per rebuild parity), bucket uint32[cells*CAP]
+//                    (the arrivals' seat words of the current rebuild), an
+//                    overflow list.
 //   mesh (per box) : acc int64[513*512*4] fixed-point deposit accumulator,
 //                    fine/force float4[513*512] in the reference atlas layout
 //                    (8x8 slices of 64x64, scratch texel (0,512)),
