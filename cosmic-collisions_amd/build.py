@@ -18,7 +18,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 SOURCES = ["csrc/cc_kernels.hip", "csrc/cc_api.hip", "csrc/cc_ff.hip", "csrc/cc_fof.hip", "csrc/cc_fof_host.cpp",
-           "csrc/cc_bodies.cpp"]
+           "csrc/cc_bodies.cpp", "csrc/cc_crater.cpp"]
 
 
 def run(cmd, cwd):

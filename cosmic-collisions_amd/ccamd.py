@@ -41,7 +41,17 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
-           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning"]
+           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_crater_stats"]
+
+
+class CraterArgs(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("Rbox", "lSI", "vSI", "gSI", "rhoSI", "vOrb", "vEsc")]
+
+
+class CraterReport(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("craterD", "rimR", "rimH", "depth", "excavated", "thrown", "platform",
+                                          "airborne", "fastOrb", "fastEsc", "melt", "vapour", "Tmax", "mass", "KE",
+                                          "PE", "drift", "Tsurf", "R")] + [("com", C.c_double * 3)]
 
 
 class TuneReport(C.Structure):
@@ -173,6 +183,8 @@ def lib():
             "cc_relax_piles": ([C.POINTER(Pile), C.c_int, C.POINTER(PileReport)], C.c_int),
             "cc_autotune": ([P, C.c_int, C.POINTER(TuneReport)], C.c_int),
             "cc_set_tuning": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
+            "cc_crater_stats": ([C.c_int, C.c_double, fp, fp, fp, C.POINTER(CraterArgs), C.POINTER(CraterReport)],
+                                C.c_int),
             "cc_bodies_last_error": ([], C.c_char_p),
         }
         for name, (args, res) in sig.items():
@@ -575,3 +587,20 @@ def relax_piles(piles):
     if r:
         raise CCError("cc_relax_piles failed (%d): %s" % (r, lib().cc_bodies_last_error().decode()))
     return [(reps[k].sweeps, reps[k].worst) for k in range(len(piles))]
+
+
+def crater_stats(pos, vel, R, mref, **args):
+    """craterStats() of chicxulub.html:848-919 (cc_crater_stats) on host arrays: the report as a dict."""
+    pos, vel = (np.ascontiguousarray(x, np.float32).reshape(-1, 4) for x in (pos, vel))
+    R = np.ascontiguousarray(R, np.float32).ravel()
+    n = len(R)
+    if len(pos) != n or len(vel) != n:
+        raise CCError("pos, vel and R must hold the same particles")
+    a = CraterArgs(*[float(args[k]) for k in ("Rbox", "lSI", "vSI", "gSI", "rhoSI", "vOrb", "vEsc")])
+    rep = CraterReport()
+    r = lib().cc_crater_stats(n, float(mref), _ptr(pos), _ptr(vel), _ptr(R), C.byref(a), C.byref(rep))
+    if r:
+        raise CCError("cc_crater_stats failed (%d)" % r)
+    d = _plain(rep)
+    d["vel"] = [0.0, 0.0, 0.0]
+    return d

@@ -22,7 +22,7 @@ namespace cc {
 
 // V8's Math.cbrt: FreeBSD's s_cbrt.c (a 5-bit estimate from the exponent, a
 // polynomial to 23 bits, rounding to 21 bits and one Newton step to 53 bits)
-static double js_cbrt(double x) {
+double js_cbrt(double x) {
   const uint32_t B1 = 715094163u, B2 = 696219795u;
   const double P0 = 1.87595182427177009643, P1 = -1.88497979543377169875, P2 = 1.621429720105354466140,
                P3 = -0.758397934778766047437, P4 = 0.145996192886612446982;

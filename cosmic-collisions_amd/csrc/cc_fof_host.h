@@ -8,6 +8,9 @@
 
 namespace cc {
 
+// V8's Math.cbrt, for the host readouts whose radii must round as the JavaScript page's do
+double js_cbrt(double x);
+
 struct F4 { float x, y, z, w; };
 
 struct FofHostIn {

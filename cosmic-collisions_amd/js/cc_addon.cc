@@ -531,6 +531,46 @@ napi_value RelaxPiles(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// craterStats(pos, vel, R, N, mref, {Rbox, lSI, vSI, gSI, rhoSI, vOrb, vEsc}) -> craterStats()'s report
+// (chicxulub.html:848-919), the same names (cc_crater_stats)
+napi_value CraterStats(napi_env env, napi_callback_info info) {
+  Args a(env, info);
+  if (a.argc < 6) { napi_throw_type_error(env, nullptr, "craterStats(pos, vel, R, N, mref, args)"); return nullptr; }
+  int N = (int)num(env, a.argv[3]);
+  if (N < 0) { napi_throw_range_error(env, nullptr, "N must be >= 0"); return nullptr; }
+  bool ok = true;
+  const float* P = static_cast<const float*>(typed(env, a.argv[0], napi_float32_array, (size_t)N * 4, &ok));
+  if (!ok) return nullptr;
+  const float* V = static_cast<const float*>(typed(env, a.argv[1], napi_float32_array, (size_t)N * 4, &ok));
+  if (!ok) return nullptr;
+  const float* R = static_cast<const float*>(typed(env, a.argv[2], napi_float32_array, (size_t)N, &ok));
+  if (!ok) return nullptr;
+  napi_value o = a.argv[5];
+  cc_crater_args ca;
+  ca.Rbox = num(env, prop(env, o, "Rbox")); ca.lSI = num(env, prop(env, o, "lSI")); ca.vSI = num(env, prop(env, o, "vSI"));
+  ca.gSI = num(env, prop(env, o, "gSI")); ca.rhoSI = num(env, prop(env, o, "rhoSI"));
+  ca.vOrb = num(env, prop(env, o, "vOrb")); ca.vEsc = num(env, prop(env, o, "vEsc"));
+  cc_crater_report r;
+  int code = cc_crater_stats(N, num(env, a.argv[4]), P, V, R, &ca, &r);
+  if (code) {
+    napi_throw_error(env, nullptr, ("cc_crater_stats failed (" + std::to_string(code) + "): bad arguments").c_str());
+    return nullptr;
+  }
+  napi_value out;
+  napi_create_object(env, &out);
+  set_num(env, out, "craterD", r.craterD); set_num(env, out, "rimR", r.rimR); set_num(env, out, "rimH", r.rimH);
+  set_num(env, out, "depth", r.depth); set_num(env, out, "excavated", r.excavated); set_num(env, out, "thrown", r.thrown);
+  set_num(env, out, "platform", r.platform); set_num(env, out, "airborne", r.airborne);
+  set_num(env, out, "fastOrb", r.fastOrb); set_num(env, out, "fastEsc", r.fastEsc); set_num(env, out, "melt", r.melt);
+  set_num(env, out, "vapour", r.vapour); set_num(env, out, "Tmax", r.Tmax); set_num(env, out, "mass", r.mass);
+  set_num(env, out, "KE", r.KE); set_num(env, out, "PE", r.PE); set_num(env, out, "drift", r.drift);
+  set_num(env, out, "Tsurf", r.Tsurf); set_num(env, out, "R", r.R);
+  const double zero[3] = { 0, 0, 0 };
+  set_vec(env, out, "vel", zero, 3);
+  set_vec(env, out, "com", r.com, 3);
+  return out;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char* name; napi_callback cb; } fns[] = {
     { "create", Create }, { "destroy", Destroy }, { "setParams", SetParams }, { "uploadState", Upload }, { "downloadState", Download },
@@ -540,7 +580,7 @@ napi_value Init(napi_env env, napi_value exports) {
     { "sim", Sim }, { "step", Step }, { "stepN", StepN }, { "getStepNo", GetStepNo }, { "setStepNo", SetStepNo },
     { "setGraphs", SetGraphs },
     { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "analyse", Analyse }, { "books", Books }, { "sync", Sync },
-    { "relaxPiles", RelaxPiles },
+    { "relaxPiles", RelaxPiles }, { "craterStats", CraterStats },
   };
   for (auto& f : fns) {
     napi_value fn;

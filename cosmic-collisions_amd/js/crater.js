@@ -194,9 +194,20 @@ function createCraterEngine(opts) {
     addon.downloadState(ctx, st.pos, st.vel, st.aux, st.q, st.mat, N);
     return st;
   }
+  // analyze(): the crater view's readout (analyze()/craterStats of chicxulub.html:848-919, :926-945) on the
+  // state read back, computed by the library on the host; report.t the run's time
+  let report = null;
+  function analyze() {
+    const st = readState();
+    report = addon.craterStats(st.pos, st.vel, sim.matR, sim.N, sim.m, {
+      Rbox: sim.Rbox, lSI: L_UNIT, vSI: V_UNIT, gSI: G_EARTH, rhoSI: sim.rhoT, vOrb: 7900 / V_UNIT, vEsc: 11200 / V_UNIT
+    });
+    report.t = sim.t;
+    return report;
+  }
   function destroy() { if (ctx) { addon.destroy(ctx); ctx = null; } }
-  return { build, step, settleSome, approachStep, couple, readState, destroy, get sim() { return sim; }, get ctx() { return ctx; },
-           PRESETS, DEFAULTS };
+  return { build, step, settleSome, approachStep, couple, readState, analyze, destroy, get sim() { return sim; },
+           get ctx() { return ctx; }, get report() { return report; }, PRESETS, DEFAULTS };
 }
 
 module.exports = { createCraterEngine, buildCraterRun, latticeFill, latticeLay, transientCrater, finalCrater, seismicMagnitude,

@@ -288,6 +288,26 @@ typedef struct {
 int cc_relax_piles(const cc_pile* piles, int count, cc_pile_report* reports);
 const char* cc_bodies_last_error(void);
 
+/* The crater view's readout (craterStats, chicxulub.html:848-919), on the
+ * host from a read-out state (pos and vel N x 4, R the masses in units of
+ * mref): float64 in the reference's order, the same names.  Particles with
+ * pos.w < 0.5 (the frozen shell) are left out; the settled surface is binned
+ * in 96 rings out to Rbox for the rim and the crater's edge. */
+typedef struct {
+  double Rbox;             /* the ground patch's radius */
+  double lSI, vSI, gSI;    /* metres, metres per second per length and speed unit; gravity m/s^2 */
+  double rhoSI;            /* the target's density, kg/m^3 */
+  double vOrb, vEsc;       /* orbital and escape speeds in speed units */
+} cc_crater_args;
+typedef struct {
+  double craterD, rimR, rimH, depth;
+  double excavated, thrown, platform, airborne, fastOrb, fastEsc, melt, vapour, Tmax;
+  double mass, KE, PE, drift, Tsurf, R;
+  double com[3];
+} cc_crater_report;
+int cc_crater_stats(int N, double mref, const float* pos, const float* vel, const float* R, const cc_crater_args* args,
+                    cc_crater_report* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,7 +125,7 @@ def test_python_binding_matches_javascript(tmp_path):
     ground.tofile(tmp_path / "ground.f32")
     node("""
 const b = require('%s/bodies.js'); const fs = require('fs');
-const rd = (f) => new Float32Array(fs.readFileSync('%s/' + f).buffer.slice(0));
+const rd = (f) => { const b = fs.readFileSync('%s/' + f); return new Float32Array(b.buffer.slice(b.byteOffset, b.byteOffset + b.length)); };
 const p1 = rd('body.f32'), p2 = rd('ground.f32');
 const r = b.relaxPiles([{ p: p1, n: 2000, a: 0.02, A: 0.2, C: 0.18, confine: b.SPHEROID, sweeps: 0 },
                         { p: p2, n: 1500, a: 0.02, A: 0.25, C: 0.3, confine: b.GROUND, sweeps: 16 }]);
