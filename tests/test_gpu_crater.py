@@ -131,8 +131,12 @@ for (const k of ['pos', 'vel']) fs.writeFileSync('%s/post_' + k, Buffer.from(st[
 for (let i = 0; i < 4; i++) eng.step(false);
 st = eng.readState();
 fs.writeFileSync('%s/f4_pos', Buffer.from(st.pos.buffer));
-console.log(JSON.stringify({ frames: frames, phase: sim.phase, N: sim.N }));
-""" % (JS, json.dumps(json.loads(str(fx["opt"]))), *([tmp_path] * 4)))
+fs.writeFileSync('%s/f4_vel', Buffer.from(st.vel.buffer));
+fs.writeFileSync('%s/matR', Buffer.from(sim.matR.buffer));
+const rep = eng.analyze();
+console.log(JSON.stringify({ frames: frames, phase: sim.phase, N: sim.N, rep, m: sim.m, Rbox: sim.Rbox, rhoT: sim.rhoT,
+                             t: sim.t, same: rep === eng.report }));
+""" % (JS, json.dumps(json.loads(str(fx["opt"]))), *([tmp_path] * 6)))
     print("crater engine (node)", flush=True)
     r = subprocess.run(["timeout", "-k", "10", "120", "node", str(script)], capture_output=True, text=True, cwd=JS)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -153,6 +157,16 @@ console.log(JSON.stringify({ frames: frames, phase: sim.phase, N: sim.N }));
     assert frozen.any() and np.array_equal(f4[frozen, :3], post[frozen, :3]), "the frozen shell moved"
     assert (post[:, 3] == 3).sum() >= (fx["b0_pos"].reshape(N, 4)[:, 3] == 4).sum(), "the impactor is melt"
     assert np.isfinite(f4).all()
+    # analyze(): the page's readout on the state it reads back, with the page's units
+    f4v = rd("f4_vel").reshape(N, 4)
+    want = ccamd.crater_stats(f4, f4v, rd("matR"), out["m"], Rbox=out["Rbox"], lSI=1e4, vSI=1000.0, gSI=9.81,
+                              rhoSI=out["rhoT"], vOrb=7900 / 1000, vEsc=11200 / 1000)
+    rep = out["rep"]
+    assert out["same"] and rep["t"] == out["t"]
+    flt = lambda v: [float(x) for x in v] if isinstance(v, list) else float(v)  # noqa: E731
+    for k in want:   # JSON writes integral doubles past 2^53 as integers: compare as doubles
+        assert flt(rep[k]) == want[k], k
+    assert rep["mass"] > 0 and rep["Tmax"] > 0
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
