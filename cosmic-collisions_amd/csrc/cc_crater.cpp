@@ -27,9 +27,11 @@ extern "C" int cc_crater_stats(int N, double mref, const float* P, const float* 
   double exc = 0, excPlat = 0, air = 0, fastOrb = 0, fastEsc = 0, melt = 0, vap = 0, Tmax = 0;
   double ke = 0, pe = 0, px = 0, py = 0, pz = 0, mass = 0;
   double hotM = 0, hx = 0, hy = 0, hz = 0, Tw = 0, Tws = 0;
-  for (int i = 0; i < N; i++) {
-    const double x = P[i * 4], y = P[i * 4 + 1], z = P[i * 4 + 2], w = P[i * 4 + 3];
-    const double vx = V[i * 4], vy = V[i * 4 + 1], vz = V[i * 4 + 2], T = V[i * 4 + 3];
+  for (size_t i = 0; i < (size_t)N; i++) {
+    const float* p = P + 4 * i;
+    const float* v = V + 4 * i;
+    const double x = p[0], y = p[1], z = p[2], w = p[3];
+    const double vx = v[0], vy = v[1], vz = v[2], T = v[3];
     if (w < 0.5) continue;   // the frozen shell is not part of the answer
     const double m = mref * R[i];
     const double sp2 = vx * vx + vy * vy + vz * vz;
