@@ -126,7 +126,12 @@ def learned_leg(state_root, steps, warmup, reuse, cycles, iters):
                               "fallbacks": int(st_[0, 2]), "learned_mesh_steps": int(st_[0, 3])})
         R.close()
     ff.close()
-    out["value"] = out["learned"]["value"]
+    # the learned far field's throughput only where the network stood in on some mesh step; otherwise every
+    # mesh step ran the exact far field (and the network beside it), which is no learned figure
+    stood_in = out["learned"]["learned_mesh_steps"] > 0
+    out["value"] = out["learned"]["value"] if stood_in else None
+    if not stood_in:
+        out["learned"]["label"] = "exact far field with the judge: the network fell back on every judged mesh step"
     out["unit"] = "particle-steps/s"
     return out
 
@@ -187,14 +192,18 @@ def main():
     import torch
     R = runs.Runner(run, st)
     N = run["sim"]["N"]
-    tune, pre = None, 0
-    if args.tune:   # warm-up steps to a mesh step, the launch shapes timed there (results unchanged), the rest
-        pre = min((-R.step_no) % 8, args.warmup)
-        R.steps(pre)
-        if R.step_no % 8 == 0:
-            R.ctx.step_no = R.step_no
-            tune = R.ctx.autotune(20)
-    R.steps(args.warmup - pre)
+    # Warm-up: steps to a mesh step, the launch shapes timed there (cc_autotune, results unchanged), then the
+    # rest of the warm-up but at least one whole 8-step cycle, so that the cycle graph the timed steps replay is
+    # captured and instantiated before the clock starts (its key includes the tuning).
+    tune = None
+    pre = (-R.step_no) % 8
+    R.steps(pre)
+    if args.tune:
+        R.ctx.step_no = R.step_no
+        tune = R.ctx.autotune(20)
+    rest = max(args.warmup - pre, 8)
+    R.steps(rest)
+    warmup_run = pre + rest
     torch.cuda.synchronize()
     R.ctx.sync()
     t0 = time.perf_counter()
@@ -208,6 +217,10 @@ def main():
     R.ctx.time_pass("copy", 20)   # a copy of known size, the traffic counters' calibration in a profiled run
     alg_bytes = 92.0 * N     # pos+vel+mat+q read, pos+vel+aux+q written: each particle's own data once
     achieved = alg_bytes / (sim_ms * 1e-3) / 1e9
+    # the whole step: B(N) = 117 + 12.6e6 / (8 N) compulsory bytes per particle-step with one mesh box
+    # (contact grid 22, step pass 92, deposit 2.5, the rest of the mesh and its gather 12.6 MB per mesh step)
+    step_bytes = 117.0 + 12.6e6 / (8.0 * N)
+    step_gbs = value * step_bytes / 1e9
     out = {
         "metric": "particle-steps/s at 262k, plain vs batched/learned far field @<=1e-3 force err",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -219,10 +232,15 @@ def main():
                    "particles": N, "path": "plain", "state": "the engine's own build/settle/approach/kick, "
                                                               "Morton-sorted, timed from first touch"},
         "device_ms_per_step": dev_ms / args.steps,
+        "warmup_run": warmup_run,
         "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_sim (contacts + heat + gravity gather + symplectic Euler)", "kernel_ms": sim_ms},
+        "roofline_step": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS, "bytes_per_particle_step": step_bytes,
+                          "what": "the whole step at the metric's value: compulsory bytes per particle-step "
+                                  "117 + 12.6e6 / (8 N)"},
     }
     if not args.no_cpu:
         P, V, _, Q, M = R.ctx.download()

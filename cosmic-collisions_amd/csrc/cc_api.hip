@@ -40,7 +40,6 @@ struct cc_ctx {
   float4* ppList = nullptr;
   float* ppBox = nullptr;
   float4* ppForce = nullptr;
-  float4* ppPartial = nullptr;
   float* ppSlot = nullptr;
   float2* ppTab = nullptr;
   int ppCount = 0;
@@ -239,7 +238,6 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     }
     ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
-    ALLOC(c->ppPartial, pp_partial_bytes());
     ALLOC(c->diag, N * 16);
     ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
     ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4);
@@ -496,7 +494,7 @@ int cc_pp(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   if (!c->ppCount) return 0;
   launch_pp(c->pos[c->cur], c->mat, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
-            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppPartial, c->ppForce, c->s);
+            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -559,7 +557,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
     HIPCHK(c, hipEventRecord(c->ppFork, c->s));
     HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
     launch_pp(c->pos[c->cur], c->mat, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
-              (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppPartial, c->ppForce, c->s2);
+              (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s2);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
   }
@@ -597,7 +595,10 @@ int cc_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
 // What a captured cycle's launches bake in: the uniforms (params and the
 // call's settle/kick/bulk/centre values), the loose-list count (its grid), box
 // B (whether its mesh is built), the parities of the state and seat-list
-// double buffers and the mass array (swapped by the Morton permutation).
+// double buffers and the mass array (swapped by the Morton permutation), and
+// the step number's parity (which steps of the cycle rebuild the seats and
+// which take the next rebuild's count: with an odd meshEvery a cycle can
+// start on either).
 static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
   const cc_params& p = c->p;
   std::vector<double> k = { (double)p.N, (double)p.G, p.m, p.a, p.cell, p.dt, p.k, p.c, p.creep, p.damp, p.tK,
@@ -617,6 +618,7 @@ static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
   k.push_back((double)c->simOrder);
   k.push_back((double)c->fuseCount);
   k.push_back((double)c->counted);
+  k.push_back((double)(c->stepNo & 1));
   if (c->ff) {
     k.push_back((double)c->ffJudge);
     k.push_back((double)c->ffTol);

@@ -20,6 +20,19 @@
 
 namespace cc {
 
+// js_cbrt and js_acos below restate two routines of fdlibm, the libm that V8
+// carries (FreeBSD's s_cbrt.c, with Bruce D. Evans's optimisation, and
+// fdlibm's e_acos.c), whose notice is kept here as their licence asks:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
+//
+// (s_cbrt.c additionally: "Optimized by Bruce D. Evans.")
+
 // V8's Math.cbrt: FreeBSD's s_cbrt.c (a 5-bit estimate from the exponent, a
 // polynomial to 23 bits, rounding to 21 bits and one Newton step to 53 bits)
 double js_cbrt(double x) {

@@ -655,54 +655,31 @@ __global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ po
   list[s] = make_float4(p.x, p.y, p.z, mat[i]);
 }
 
-__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
-  float t = gclamp((x - e0) / (e1 - e0), 0.0f, 1.0f);
-  return t * t * (3.0f - 2.0f * t);
-}
+// The pairwise pass (src/glsl-sim.js:712-754): each seat's four slices, slice
+// s the sum over the s-th quarter of the list in the reference's order -- one
+// sequential FP32 chain per seat, slice and component, as the fragment's loop
+// adds it, whatever the list's length.  A wavefront (a work-group of its own)
+// owns PP_ROWS seats and one slice and walks the slice in tiles of PP_COLS
+// entries:
+//   1. its lanes take the tile's distances (a seat against every fourth
+//      entry) and list the pairs within the cut, by ballot -- the rest of the
+//      tile's terms are zero, as the reference skips those pairs;
+//   2. the listed pairs' terms, 64 at a time with no lane idle on a pair
+//      beyond the cut, into an LDS tile;
+//   3. lane (seat, component) adds the tile's terms in entry order to its
+//      running sum.
+// Every pair is evaluated once for each end, as the reference evaluates it.
+constexpr int PP_ROWS = 16;
+constexpr int PP_COLS = 32;
+constexpr int PP_TPAD = PP_COLS + 1;   // float4 row stride of the term tile (no bank conflicts in step 3)
 
-// Each reference slice (a quarter of the list, src/glsl-sim.js:712-754) is cut
-// into ranges of PP_CHUNK entries: a seat's sum over one range, in the
-// reference's order, is one partial, and k_pp_sum adds a slice's partials in
-// order.  The 32 ranges (4 slices x 8) pair up into 496 work-groups, one per
-// pair of ranges A < B, and 128 for the ranges with themselves, each taking a
-// quarter of the range's rows (these have no column sums, so their rows part
-// freely; 624 work-groups of about 2.1 pairs' work per CU where 528 equal ones
-// left some CUs three).  A pair's terms for its two ends differ only in the
-// mass they are scaled by and the sign of the direction (the distance, the
-// kernel, the taper and the box test are the same numbers), so the work-group
-// for A < B evaluates each pair (i in A, j in B) once and sums it into both
-// i's partial over B and j's partial over A.  Both sums keep the reference's
-// order: the pairs are taken in 32 x 32 tiles, A's tile outer and B's inner,
-// each tile's terms set out in LDS, and a lane per (particle, component) adds a
-// tile's 32 terms in order to its running sum -- i's over B's tiles in turn,
-// j's (kept in LDS across the outer loop) over A's tiles in turn.  A work-group
-// with A = B evaluates its pairs in both directions, as the reference does.
-constexpr int PP_CHUNK = 256;
-constexpr int PP_MAXCHUNK = PP_CAP / 4 / PP_CHUNK;   // 8
-constexpr int PP_RANGES = 4 * PP_MAXCHUNK;           // 32
-constexpr int PP_TILE = 32;
-constexpr int PP_PAD = PP_TILE + 1;                  // float4 row stride of a term tile (no bank conflicts)
-constexpr int PP_PAIRS = PP_RANGES * (PP_RANGES - 1) / 2;   // 496 range pairs A < B
-
-// range g = slice * PP_MAXCHUNK + chunk of a list of count seats: [lo, hi), empty when lo >= hi
-__device__ __forceinline__ void pp_range(int g, int count, int& lo, int& hi) {
-  int share = (count + 3) >> 2, s = g / PP_MAXCHUNK, c = g % PP_MAXCHUNK;
-  int j0 = s * share, j1 = min(j0 + share, count);
-  lo = j0 + c * PP_CHUNK;
-  hi = min(lo + PP_CHUNK, j1);
-}
-
-// One pair's terms, each operation the reference's (src/glsl-sim.js:712-754):
-// for the seat at P (mass mi, box bi) from the entry at Q (mass Q.w, box bj),
-// and for Q's seat from P; zero when the pair is beyond the cut.
-__device__ __forceinline__ void pp_terms(float4 P, float bi, float4 Q, float bj, float cut, float cut2, float a2, float a,
-                                         float ya, float yCut, float yTap, float tapD, const float2* st, float4& ti,
-                                         float4& tj) {
-  ti = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  tj = ti;
+// One term, each operation the reference's: for the seat at P (box bi) from
+// the entry at Q (mass Q.w, box bj); zero beyond the cut.
+__device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj, float cut, float cut2, float a2,
+                                          float a, float ya, float yCut, float yTap, float tapD, const float2* st) {
   float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
   float r2 = dot3(d0, d1, d2, d0, d1, d2);
-  if (r2 >= cut2) return;
+  if (r2 >= cut2) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float r = qsqrt(r2);
   float x = gclamp(qdiv_by(r, cut, yCut) * 64.0f - 0.5f, 0.0f, 63.0f);
   int k = (int)x, k1 = min(k + 1, 63);
@@ -719,127 +696,80 @@ __device__ __forceinline__ void pp_terms(float4 P, float bi, float4 Q, float bj,
   float n0 = qdiv_by(d0, rr, yr), n1 = qdiv_by(d1, rr, yr), n2 = qdiv_by(d2, rr, yr);
   // max(r, a) is rr where r > a (a > 1e-6), else the uniform a with its reciprocal ya
   float ph = -invN - qdiv_by(Ty, gmax(r, a), r > a ? yr : ya);
-  float si = Q.w * fn * taper, sj = P.w * fn * taper;
-  ti = make_float4(n0 * si, n1 * si, n2 * si, Q.w * ph * taper);
-  tj = make_float4(-(n0 * sj), -(n1 * sj), -(n2 * sj), P.w * ph * taper);
+  float s = Q.w * fn * taper;
+  return make_float4(n0 * s, n1 * s, n2 * s, Q.w * ph * taper);
 }
 
-__global__ void __launch_bounds__(256) k_pp_pairs(const float4* __restrict__ list, const float* __restrict__ box,
-                                                  int count, float cut, float a2, const float2* __restrict__ tab,
-                                                  float4* partial) {
-  __shared__ float4 tRow[PP_TILE * PP_PAD];   // [i][j]: the A side's terms of a tile
-  __shared__ float4 tCol[PP_TILE * PP_PAD];   // [j][i]: the B side's terms
-  __shared__ float4 lB[PP_CHUNK];
-  __shared__ float bB[PP_CHUNK];
-  __shared__ float colSum[PP_CHUNK * 4];      // B's running sums, [j][component]
+__global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list, const float* __restrict__ box,
+                                                 int count, float gm, float cut, float a2,
+                                                 const float2* __restrict__ tab, float4* force) {
+  __shared__ float4 term[PP_ROWS * PP_TPAD];
+  __shared__ float4 lP[PP_ROWS], lQ[PP_COLS];
+  __shared__ float bP[PP_ROWS], bQ[PP_COLS];
+  __shared__ int todo[PP_ROWS * PP_COLS];
   __shared__ float2 st[64];
-  // work-groups 0..495: the range pairs A < B; then four per range A = B, each a quarter of its rows
-  int id = blockIdx.x, gA = 0, gB, quarter = -1;
-  if (id < PP_PAIRS) {
-    while (id >= PP_RANGES - 1 - gA) { id -= PP_RANGES - 1 - gA; gA++; }
-    gB = gA + 1 + id;
-  } else {
-    id -= PP_PAIRS;
-    gA = gB = id >> 2;
-    quarter = id & 3;
+  const int lane = threadIdx.x, slice = blockIdx.y, r0 = blockIdx.x * PP_ROWS;
+  const int sr = lane >> 2, comp = lane & 3;   // step 3: seat r0 + sr, component comp
+  float* out = reinterpret_cast<float*>(force + (size_t)slice * PP_CAP + r0 + sr) + comp;
+  if (r0 >= count) { *out = 0.0f; return; }   // seats past the list: zero (the whole wavefront)
+  const int share = (count + 3) >> 2, j0 = slice * share, j1 = min(j0 + share, count);
+  st[lane] = tab[lane];
+  if (lane < PP_ROWS) {
+    const int i = r0 + lane;
+    lP[lane] = i < count ? list[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bP[lane] = i < count ? box[i] : -1.0f;
   }
-  int loA, hiA, loB, hiB;
-  pp_range(gA, count, loA, hiA);
-  pp_range(gB, count, loB, hiB);
-  if (loA >= hiA || loB >= hiB) return;
-  const bool both = gA != gB;
-  const int nA = hiA - loA, nB = hiB - loB, t = threadIdx.x;
-  if (t < 64) st[t] = tab[t];
-  lB[t] = t < nB ? list[loB + t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  bB[t] = t < nB ? box[loB + t] : -1.0f;
-  for (int k = t; k < PP_CHUNK * 4; k += 256) colSum[k] = 0.0f;
-  __syncthreads();
+  const int er = lane & (PP_ROWS - 1), ec = lane >> 4;   // step 1: seat r0 + er, entries ec, ec + 4, ...
+  const int i = r0 + er;
+  const float4 P = i < count ? list[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool live = i < count && P.w != 0.0f;
   const float cut2 = cut * cut, a = sqrtf(a2), tapD = cut - 0.8f * cut;
   const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD), ya = qdiv(1.0f, a);
-  // evaluation: lane il of a tile's rows against four consecutive columns
-  const int il = t & 31, jl0 = (t >> 6) * 8 + ((t >> 5) & 1) * 4;
-  // sums: lanes 0..127 a row's component, lanes 128..255 a column's
-  const bool rowLane = t < 128;
-  const int sl = (t & 127) >> 2, comp = t & 3;
-  const int tilesA = (nA + PP_TILE - 1) / PP_TILE, tilesB = (nB + PP_TILE - 1) / PP_TILE;
-  const int ta0 = both ? 0 : quarter * 2, ta1 = both ? tilesA : min(ta0 + 2, tilesA);
-  for (int ta = ta0; ta < ta1; ta++) {
-    const int i = ta * PP_TILE + il;
-    const float4 P = i < nA ? list[loA + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float bi = i < nA ? box[loA + i] : -1.0f;
-    float rowRun = 0.0f;
-    for (int tb = 0; tb < tilesB; tb++) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) {   // the tile's terms
-        const int jl = jl0 + k, j = tb * PP_TILE + jl;
-        float4 ti, tj;
-        if (i < nA && j < nB && (both || i != j)) {
-          pp_terms(P, bi, lB[j], bB[j], cut, cut2, a2, a, ya, yCut, yTap, tapD, st, ti, tj);
-        } else {
-          ti = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          tj = ti;
-        }
-        tRow[il * PP_PAD + jl] = ti;
-        if (both) tCol[jl * PP_PAD + il] = tj;
-      }
-      __syncthreads();
-      if (rowLane) {   // added in order to the running sums: a row's over B's tiles in turn ...
-        const float* R = reinterpret_cast<const float*>(tRow) + sl * PP_PAD * 4 + comp;
-        float v[PP_TILE];
-#pragma unroll
-        for (int j = 0; j < PP_TILE; j++) v[j] = R[j * 4];
-#pragma unroll
-        for (int j = 0; j < PP_TILE; j++) rowRun += v[j];
-      } else if (both) {   // ... a column's over A's tiles in turn
-        const float* Cc = reinterpret_cast<const float*>(tCol) + sl * PP_PAD * 4 + comp;
-        float v[PP_TILE];
-#pragma unroll
-        for (int k = 0; k < PP_TILE; k++) v[k] = Cc[k * 4];
-        float& run = colSum[(tb * PP_TILE + sl) * 4 + comp];
-        float c = run;
-#pragma unroll
-        for (int k = 0; k < PP_TILE; k++) c += v[k];
-        run = c;
-      }
-      __syncthreads();
+  const float* R = reinterpret_cast<const float*>(term) + sr * PP_TPAD * 4 + comp;
+  float run = 0.0f;
+  for (int jt = j0; jt < j1; jt += PP_COLS) {
+    if (lane < PP_COLS) {
+      const int j = jt + lane;
+      lQ[lane] = j < j1 ? list[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      bQ[lane] = j < j1 ? box[j] : -1.0f;
     }
-    const int ir = ta * PP_TILE + sl;   // a row's sum over B is complete
-    if (rowLane && ir < nA) reinterpret_cast<float*>(partial + (size_t)gB * PP_CAP + loA + ir)[comp] = rowRun;
-  }
-  if (both && t < nB) {   // j's sum over A is complete
-    const float* c = colSum + t * 4;
-    partial[(size_t)gA * PP_CAP + loB + t] = make_float4(c[0], c[1], c[2], c[3]);
-  }
-}
-// a slice's partials in order, times G m
-__global__ void __launch_bounds__(256) k_pp_sum(const float4* __restrict__ partial, const float4* __restrict__ list,
-                                                int count, float gm, float4* force) {
-  int me = blockIdx.x * blockDim.x + threadIdx.x, slice = blockIdx.y;
-  if (me >= PP_CAP) return;
-  bool active = me < count && list[me].w != 0.0f;
-  float4 s = make_float4(0, 0, 0, 0);
-  if (active) {
-    for (int c = 0; c < PP_MAXCHUNK; c++) {
-      int lo, hi;
-      pp_range(slice * PP_MAXCHUNK + c, count, lo, hi);
-      if (lo >= hi) break;
-      float4 v = partial[((size_t)slice * PP_MAXCHUNK + c) * PP_CAP + me];
-      s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    __syncthreads();
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < PP_COLS / 4; k++) {
+      const int jl = ec + 4 * k, j = jt + jl;
+      const float4 Q = lQ[jl];
+      const float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
+      const bool in = live && j < j1 && j != i && dot3(d0, d1, d2, d0, d1, d2) < cut2;
+      const unsigned long long m = __ballot(in);
+      if (in)
+        todo[n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+            er | (jl << 4);
+      else
+        term[er * PP_TPAD + jl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      n += __popcll(m);
     }
-    s = make_float4(s.x * gm, s.y * gm, s.z * gm, s.w * gm);
+    __syncthreads();
+    for (int t = lane; t < n; t += 64) {
+      const int e = todo[t], r = e & (PP_ROWS - 1), jl = e >> 4;
+      term[r * PP_TPAD + jl] = pp_term(lP[r], bP[r], lQ[jl], bQ[jl], cut, cut2, a2, a, ya, yCut, yTap, tapD, st);
+    }
+    __syncthreads();
+    float v[PP_COLS];
+#pragma unroll
+    for (int c = 0; c < PP_COLS; c++) v[c] = R[c * 4];
+#pragma unroll
+    for (int c = 0; c < PP_COLS; c++) run += v[c];
+    __syncthreads();
   }
-  force[slice * PP_CAP + me] = s;
+  const int ir = r0 + sr;
+  *out = (ir < count && list[ir].w != 0.0f) ? run * gm : 0.0f;
 }
-
-size_t pp_partial_bytes() { return (size_t)PP_RANGES * PP_CAP * sizeof(float4); }
 
 void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm, float cut,
-               float a2, const float2* tab, float4* list, float* box, float4* partial, float4* force, hipStream_t s) {
+               float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s) {
   hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, b, index, list, box);
-  if (count > 0)
-    hipLaunchKernelGGL(k_pp_pairs, dim3(PP_PAIRS + 4 * PP_RANGES), dim3(256), 0, s, list, box, count, cut, a2, tab,
-                       partial);
-  hipLaunchKernelGGL(k_pp_sum, dim3(PP_CAP / 256, 4), dim3(256), 0, s, partial, list, count, gm, force);
+  hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / PP_ROWS, 4), dim3(64), 0, s, list, box, count, gm, cut, a2, tab, force);
 }
 
 __global__ void k_pp_slot(int N, const int* __restrict__ index, int count, float* slot) {
