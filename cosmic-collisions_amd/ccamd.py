@@ -41,7 +41,7 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
-           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_crater_stats"]
+           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_crater_stats", "cc_pp_noalloc"]
 
 
 class CraterArgs(C.Structure):
@@ -186,6 +186,7 @@ def lib():
             "cc_crater_stats": ([C.c_int, C.c_double, fp, fp, fp, C.POINTER(CraterArgs), C.POINTER(CraterReport)],
                                 C.c_int),
             "cc_bodies_last_error": ([], C.c_char_p),
+            "cc_pp_noalloc": ([P, P, C.c_int, dp, dp, C.c_int, P, C.c_int, C.c_double, C.c_double, P, P, P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -568,6 +569,18 @@ class FarField:
             self.close()
         except Exception:
             pass
+
+
+def pp_noalloc(pos, mat, N, LA, LB, boxB, index, count, m, a, tab, force, stream=None):
+    """cc_pp_noalloc: the pairwise pass on device arrays (addresses as integers:
+    pos float[N*4], mat float[N], index int32[count], tab float[128], force
+    float[4*8192*4] out), queued on `stream` (a hipStream_t address, None: the
+    null stream); no allocation, no wait."""
+    r = lib().cc_pp_noalloc(C.c_void_p(pos), C.c_void_p(mat), int(N), _d3(LA), _d3(LB), int(bool(boxB)),
+                            C.c_void_p(index), int(count), float(m), float(a), C.c_void_p(tab), C.c_void_p(force),
+                            C.c_void_p(stream) if stream else None)
+    if r:
+        raise CCError("cc_pp_noalloc failed (%d): %s" % (r, lib().cc_last_error(None).decode()))
 
 
 def relax_piles(piles):

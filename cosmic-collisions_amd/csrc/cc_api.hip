@@ -3,6 +3,7 @@
 // pass sequence of its step() -- all queued on one HIP stream.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -493,7 +494,7 @@ int cc_set_pp_table(cc_ctx* c, const float* tab) {
 int cc_pp(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   if (!c->ppCount) return 0;
-  launch_pp(c->pos[c->cur], c->mat, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+  launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
             (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s);
   LAUNCHCHK(c);
   return 0;
@@ -556,7 +557,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   if (side) {
     HIPCHK(c, hipEventRecord(c->ppFork, c->s));
     HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
-    launch_pp(c->pos[c->cur], c->mat, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+    launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
               (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s2);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
@@ -1207,3 +1208,73 @@ int cc_batch_last_ms(cc_batch* b, double* ms) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------- the pairwise pass, no allocation --
+// cc_pp_noalloc: the pairwise pass on the caller's device arrays, for callers
+// that cannot allocate device memory (code under HIP graph capture, serving
+// loops).  Its workspace -- the mesh boxes as the kernels read them and the
+// gathered loose list with each seat's box -- is `scratch`, a file-scope
+// device array of a fixed size (PP_CAP seats: 256 + 8192 x 20 bytes), one copy
+// per GPU for the life of the process.  Host code never names `scratch` in a
+// launch or a copy: its address on each device comes from hipGetSymbolAddress,
+// taken once per device with that device current and kept in g_scratch.
+// Every call on a GPU shares that GPU's copy: calls that overlap on one GPU
+// (two streams, two threads) race.  Each call writes every scratch byte it
+// reads (the boxes, all 8192 seats) before reading it, so nothing needs
+// clearing between calls.
+namespace {
+constexpr size_t NA_BOX = 0;                                  // Box (32 B), 256-byte aligned
+constexpr size_t NA_LIST = 256;                               // float4[PP_CAP]
+constexpr size_t NA_SEATBOX = NA_LIST + (size_t)PP_CAP * 16;  // float[PP_CAP]
+constexpr size_t NA_BYTES = NA_SEATBOX + (size_t)PP_CAP * 4;
+constexpr int NA_DEVICES = 64;
+std::mutex g_scratchLock;
+unsigned char* g_scratch[NA_DEVICES];   // each device's address of `scratch` (null until looked up)
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// the current device's copy of `scratch`, looked up on the device's first call
+static int scratch_here(unsigned char** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) FAIL((cc_ctx*)nullptr, CC_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= NA_DEVICES) FAIL((cc_ctx*)nullptr, CC_EDEVICE, "device %d past the %d this library serves", dev, NA_DEVICES);
+  std::lock_guard<std::mutex> hold(g_scratchLock);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      FAIL((cc_ctx*)nullptr, CC_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+extern "C" int cc_pp_noalloc(const float* pos, const float* mat, int N, const double LA[3], const double LB[3],
+                             int boxB, const int32_t* index, int count, double m, double a, const float* tab,
+                             float* force, void* stream) {
+  if (!pos || !mat || !index || !tab || !force || !LA || !LB)
+    FAIL((cc_ctx*)nullptr, CC_EINVAL, "cc_pp_noalloc: a NULL array");
+  if (N <= 0) FAIL((cc_ctx*)nullptr, CC_EINVAL, "cc_pp_noalloc: N=%d", N);
+  if (count < 0 || count > PP_CAP)
+    FAIL((cc_ctx*)nullptr, CC_EINVAL, "cc_pp_noalloc: a loose list of %d seats (its workspace holds at most %d)", count,
+         PP_CAP);
+  if (!(m > 0) || !(a > 0)) FAIL((cc_ctx*)nullptr, CC_EINVAL, "cc_pp_noalloc: m and a must be > 0");
+  unsigned char* w = nullptr;
+  int r = scratch_here(&w);
+  if (r) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Box b;
+  b.invH = (float)(1.0 / FH);
+  for (int k = 0; k < 3; k++) { b.LA[k] = (float)LA[k]; b.LB[k] = (float)LB[k]; }
+  b.boxB = boxB ? 1.0f : 0.0f;
+  Box* bx = reinterpret_cast<Box*>(w + NA_BOX);
+  launch_put_box(b, bx, s);
+  launch_pp(reinterpret_cast<const float4*>(pos), mat, N, bx, index, count, (float)m, (float)(5.0 * FH), (float)(a * a),
+            reinterpret_cast<const float2*>(tab), reinterpret_cast<float4*>(w + NA_LIST),
+            reinterpret_cast<float*>(w + NA_SEATBOX), reinterpret_cast<float4*>(force), s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) FAIL((cc_ctx*)nullptr, CC_EDEVICE, "cc_pp_noalloc launch: %s", hipGetErrorString(e));
+  return 0;
+}

@@ -147,7 +147,7 @@ void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
 // the pairwise pass: the loose list's gather, then each seat's four slices (force[slice * PP_CAP + seat])
-void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm,
+void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
                float cut, float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s);
 // count: the next rebuild's count taken on the positions this pass writes (null: none)
 void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,

@@ -641,13 +641,15 @@ void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, bool zeroed, hipS
 }
 
 // ------------------------------------------------------ pairwise P3M (A11) --
+// The loose list's seats (PPGATHER_FS, src/glsl-sim.js:676-698): seat s < count
+// holds particle index[s]; the rest, and an entry outside [0, N), are empty.
 __global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ pos, const float* __restrict__ mat,
-                                                   const Box* __restrict__ bxp, const int* __restrict__ index,
-                                                   float4* list, float* box) {
+                                                   int N, const Box* __restrict__ bxp, const int* __restrict__ index,
+                                                   int count, float4* list, float* box) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= PP_CAP) return;
-  int i = index[s];
-  if (i < 0) { list[s] = make_float4(0, 0, 0, 0); box[s] = -1.0f; return; }
+  int i = s < count ? index[s] : -1;
+  if (i < 0 || i >= N) { list[s] = make_float4(0, 0, 0, 0); box[s] = -1.0f; return; }
   float4 p = pos[i];
   float gx, gy, gz;
   const Box b = *bxp;
@@ -766,9 +768,9 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
   *out = (ir < count && list[ir].w != 0.0f) ? run * gm : 0.0f;
 }
 
-void launch_pp(const float4* pos, const float* mat, const Box* b, const int* index, int count, float gm, float cut,
-               float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s) {
-  hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, b, index, list, box);
+void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
+               float cut, float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s) {
+  hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, N, b, index, count, list, box);
   hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / PP_ROWS, 4), dim3(64), 0, s, list, box, count, gm, cut, a2, tab, force);
 }
 

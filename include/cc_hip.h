@@ -96,6 +96,19 @@ int cc_set_pp(cc_ctx* ctx, const int32_t* list, int count);
 int cc_pp(cc_ctx* ctx);
 /* pp.list (8192x4), pp.box (8192), pp.force (4 slices x 8192 x 4, slice-major); any may be NULL */
 int cc_read_pp(cc_ctx* ctx, float* list, float* box, float* force);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops): the pairwise pass of
+ * cc_set_pp + cc_pp on the caller's DEVICE arrays, with no context and no
+ * allocation.  pos float[N*4], mat float[N], index int32[count] (entries in
+ * [0, N); others are empty seats), tab float[64*2] (the pair table), force
+ * float[4*8192*4] out (slice-major, as cc_read_pp); LA/LB/boxB the mesh boxes
+ * (as cc_set_boxes); m, a as in cc_params; stream a hipStream_t (NULL: the
+ * null stream).  Queued on the stream, not waited for.  count <= 8192, the
+ * size of its fixed workspace (a static device array, one per GPU, kept until
+ * the process ends); larger counts fail with CC_EINVAL.  Calls that overlap on
+ * one GPU share that workspace and race.  Errors: cc_last_error(NULL). */
+int cc_pp_noalloc(const float* pos, const float* mat, int N, const double LA[3], const double LB[3], int boxB,
+                  const int32_t* index, int count, double m, double a, const float* tab, float* force, void* stream);
 /* passSim (src/nbody.js:458-484): one step pass, state src -> dst, swap */
 int cc_sim(cc_ctx* ctx, const cc_sim_args* a);
 /* step (src/nbody.js:551-558): seats every 2nd step, mesh + pairwise every
