@@ -173,8 +173,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--state-dir", default=os.path.join(tempfile.gettempdir(), "cc_twins_state"))
     ap.add_argument("--reuse-state", action="store_true", help="use a prepared state in --state-dir if there is one")
-    ap.add_argument("--batch", type=int, default=0,
-                    help="instances in the batched-sweep leg (0, the default: skip it; 64 is the sweep's size)")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="instances in the batched-sweep leg (64, the sweep's size; 0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=240)
     ap.add_argument("--batch-dir", default=os.path.join(tempfile.gettempdir(), "cc_batch_state"))
     ap.add_argument("--runto", type=int, default=3600, help="steps of the engine-driven leg with analyses (0: skip)")
@@ -189,7 +189,6 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)
         return 2
     run, st = runs.prepare(runs.TWINS, args.state_dir, reuse=args.reuse_state)
-    import torch
     R = runs.Runner(run, st)
     N = run["sim"]["N"]
     # Warm-up: steps to a mesh step, the launch shapes timed there (cc_autotune, results unchanged), then the
@@ -204,12 +203,11 @@ def main():
     rest = max(args.warmup - pre, 8)
     R.steps(rest)
     warmup_run = pre + rest
-    torch.cuda.synchronize()
+    # the library's stream (the pairwise pass's second stream joins back into it) drained on both sides
     R.ctx.sync()
     t0 = time.perf_counter()
     dev_ms = R.steps(args.steps)
     R.ctx.sync()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = N * args.steps / wall
     # the step pass alone, 200 back to back between HIP events on the library's stream
@@ -255,8 +253,14 @@ def main():
     if args.learned > 0:
         out["learned"] = learned_leg(tempfile.gettempdir(), args.learned_steps, 200, args.reuse_state, args.ff_cycles,
                                      args.ff_iters)
+    out["hip_runtime"] = hip_runtime()
     print(json.dumps(out))
     return 0
+
+
+def hip_runtime():
+    """The HIP runtime copies this process has mapped (the library's, and no other)."""
+    return sorted({ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln})
 
 
 def twins_like(run):

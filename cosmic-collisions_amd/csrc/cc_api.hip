@@ -127,6 +127,30 @@ static void grid_make(int G, Grid& g) {   // src/gl.js:73-83
   g.cells = g.AW * g.AH;
 }
 
+// The HIP runtime this process runs the library on must be the one it was built
+// against, or newer.  A process can hold another copy of libamdhip64: a Python
+// wheel (PyTorch's) ships its own, and whichever copy a process loads first
+// serves every library that asks for "libamdhip64.so.7" after it.  The graph
+// capture and replay paths are tested on the runtime of the build; an older
+// copy is refused here rather than run.
+static int check_runtime() {
+  static int checked = 0, have = 0;   // the loaded runtime cannot change within a process
+  if (!checked) {
+    if (hipRuntimeGetVersion(&have) != hipSuccess) have = 0;
+    checked = 1;
+  }
+  if (have / 100000 < HIP_VERSION / 100000) {
+    char b[320];
+    snprintf(b, sizeof b,
+             "HIP runtime %d.%d loaded in this process, older than the %d.%d this library was built against (another "
+             "copy of libamdhip64 was loaded first, e.g. a Python wheel's: load this library before it)",
+             have / 10000000, (have / 100000) % 100, HIP_VERSION_MAJOR, HIP_VERSION_MINOR);
+    g_err = b;
+    return CC_EDEVICE;
+  }
+  return 0;
+}
+
 static int check_params(cc_ctx* c, const cc_params* p) {
   if (!p) FAIL(c, CC_EINVAL, "params is NULL");
   if (p->N <= 0 || p->W <= 0 || p->H <= 0 || (long long)p->W * p->H != p->N)
@@ -204,7 +228,9 @@ const char* cc_last_error(cc_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c
 int cc_create(const cc_params* p, cc_ctx** out) {
   if (!out) FAIL((cc_ctx*)nullptr, CC_EINVAL, "out is NULL");
   *out = nullptr;
-  int r = check_params(nullptr, p);
+  int r = check_runtime();
+  if (r) return r;
+  r = check_params(nullptr, p);
   if (r) return r;
   cc_ctx* c = new cc_ctx();
   c->p = *p;
@@ -1261,8 +1287,10 @@ extern "C" int cc_pp_noalloc(const float* pos, const float* mat, int N, const do
     FAIL((cc_ctx*)nullptr, CC_EINVAL, "cc_pp_noalloc: a loose list of %d seats (its workspace holds at most %d)", count,
          PP_CAP);
   if (!(m > 0) || !(a > 0)) FAIL((cc_ctx*)nullptr, CC_EINVAL, "cc_pp_noalloc: m and a must be > 0");
+  int r = check_runtime();
+  if (r) return r;
   unsigned char* w = nullptr;
-  int r = scratch_here(&w);
+  r = scratch_here(&w);
   if (r) return r;
   hipStream_t s = static_cast<hipStream_t>(stream);
   Box b;
