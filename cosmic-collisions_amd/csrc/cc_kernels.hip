@@ -663,20 +663,20 @@ __global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ po
 // adds it, whatever the list's length.  A wavefront (a work-group of its own)
 // owns PP_ROWS seats and one slice and walks the slice in tiles of PP_COLS
 // entries:
-//   1. its lanes take the tile's distances (a seat against every fourth
-//      entry) and list the pairs within the cut, by ballot -- the rest of the
-//      tile's terms are zero, as the reference skips those pairs;
+//   1. its lanes clear the tile's terms, take the tile's distances (a seat
+//      against every eighth entry) and list the pairs within the cut, by
+//      ballot -- the other pairs' terms stay zero, as the reference skips them;
 //   2. the listed pairs' terms, 64 at a time with no lane idle on a pair
-//      beyond the cut, into an LDS tile;
-//   3. lane (seat, component) adds the tile's terms in entry order to its
-//      running sum.
+//      beyond the cut, into the LDS tile, component-major;
+//   3. lane (component, seat) adds the tile's terms in entry order to its
+//      running sum, four entries per LDS read.
 // Every pair is evaluated once for each end, as the reference evaluates it.
-constexpr int PP_ROWS = 16;
-constexpr int PP_COLS = 32;
-constexpr int PP_TPAD = PP_COLS + 1;   // float4 row stride of the term tile (no bank conflicts in step 3)
+constexpr int PP_ROWS = 8;
+constexpr int PP_COLS = 48;
+constexpr int PP_CHAIN = PP_COLS + 4;   // floats per (component, seat) row of the term tile: 16-byte reads of
+                                        // eight consecutive rows fall in distinct banks
+// (8 seats x 48 entries keep a work-group's LDS under 10 KiB: sixteen of them, a CU's limit, fit at once)
 
-// One term, each operation the reference's: for the seat at P (box bi) from
-// the entry at Q (mass Q.w, box bj); zero beyond the cut.
 __device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj, float cut, float cut2, float a2,
                                           float a, float ya, float yCut, float yTap, float tapD, const float2* st) {
   float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
@@ -705,15 +705,18 @@ __device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj
 __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list, const float* __restrict__ box,
                                                  int count, float gm, float cut, float a2,
                                                  const float2* __restrict__ tab, float4* force) {
-  __shared__ float4 term[PP_ROWS * PP_TPAD];
+  __shared__ __align__(16) float term[4 * PP_ROWS * PP_CHAIN];
   __shared__ float4 lP[PP_ROWS], lQ[PP_COLS];
   __shared__ float bP[PP_ROWS], bQ[PP_COLS];
-  __shared__ int todo[PP_ROWS * PP_COLS];
+  __shared__ unsigned short todo[PP_ROWS * PP_COLS];
   __shared__ float2 st[64];
   const int lane = threadIdx.x, slice = blockIdx.y, r0 = blockIdx.x * PP_ROWS;
-  const int sr = lane >> 2, comp = lane & 3;   // step 3: seat r0 + sr, component comp
-  float* out = reinterpret_cast<float*>(force + (size_t)slice * PP_CAP + r0 + sr) + comp;
-  if (r0 >= count) { *out = 0.0f; return; }   // seats past the list: zero (the whole wavefront)
+  const int sc = lane / PP_ROWS, sr = lane % PP_ROWS;   // step 3 (lanes below 4 * PP_ROWS): component sc of seat r0 + sr
+  float* out = reinterpret_cast<float*>(force + (size_t)slice * PP_CAP + r0 + sr) + sc;
+  if (r0 >= count) {   // seats past the list: zero (the whole wavefront)
+    if (lane < 4 * PP_ROWS) *out = 0.0f;
+    return;
+  }
   const int share = (count + 3) >> 2, j0 = slice * share, j1 = min(j0 + share, count);
   st[lane] = tab[lane];
   if (lane < PP_ROWS) {
@@ -721,51 +724,67 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
     lP[lane] = i < count ? list[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     bP[lane] = i < count ? box[i] : -1.0f;
   }
-  const int er = lane & (PP_ROWS - 1), ec = lane >> 4;   // step 1: seat r0 + er, entries ec, ec + 4, ...
+  const int er = lane % PP_ROWS, ec = lane / PP_ROWS;   // step 1: seat r0 + er, entries ec, ec + 8, ...
   const int i = r0 + er;
   const float4 P = i < count ? list[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const bool live = i < count && P.w != 0.0f;
   const float cut2 = cut * cut, a = sqrtf(a2), tapD = cut - 0.8f * cut;
   const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD), ya = qdiv(1.0f, a);
-  const float* R = reinterpret_cast<const float*>(term) + sr * PP_TPAD * 4 + comp;
+  const float4* chain = reinterpret_cast<const float4*>(term + lane * PP_CHAIN);   // step 3's row (lane = sc * PP_ROWS + sr)
   float run = 0.0f;
+  // the tile's entries, loaded one tile ahead (lanes below PP_COLS)
+  float4 nQ = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float nB = -1.0f;
+  if (lane < PP_COLS && j0 + lane < j1) { nQ = list[j0 + lane]; nB = box[j0 + lane]; }
   for (int jt = j0; jt < j1; jt += PP_COLS) {
     if (lane < PP_COLS) {
-      const int j = jt + lane;
-      lQ[lane] = j < j1 ? list[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      bQ[lane] = j < j1 ? box[j] : -1.0f;
+      lQ[lane] = nQ;
+      bQ[lane] = nB;
+      const int j = jt + PP_COLS + lane;
+      nQ = j < j1 ? list[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      nB = j < j1 ? box[j] : -1.0f;
     }
+    float4* t4 = reinterpret_cast<float4*>(term);
+#pragma unroll
+    for (int k = lane; k < PP_ROWS * PP_CHAIN; k += 64) t4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     __syncthreads();
     int n = 0;
 #pragma unroll
-    for (int k = 0; k < PP_COLS / 4; k++) {
-      const int jl = ec + 4 * k, j = jt + jl;
+    for (int k = 0; k < PP_COLS / 8; k++) {
+      const int jl = ec + 8 * k, j = jt + jl;   // ec = lane / 8 < 8
       const float4 Q = lQ[jl];
       const float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
       const bool in = live && j < j1 && j != i && dot3(d0, d1, d2, d0, d1, d2) < cut2;
       const unsigned long long m = __ballot(in);
       if (in)
         todo[n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
-            er | (jl << 4);
-      else
-        term[er * PP_TPAD + jl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            (unsigned short)(er | (jl << 3));
       n += __popcll(m);
     }
     __syncthreads();
     for (int t = lane; t < n; t += 64) {
-      const int e = todo[t], r = e & (PP_ROWS - 1), jl = e >> 4;
-      term[r * PP_TPAD + jl] = pp_term(lP[r], bP[r], lQ[jl], bQ[jl], cut, cut2, a2, a, ya, yCut, yTap, tapD, st);
+      const int e = todo[t], r = e & (PP_ROWS - 1), jl = e >> 3;
+      const float4 v = pp_term(lP[r], bP[r], lQ[jl], bQ[jl], cut, cut2, a2, a, ya, yCut, yTap, tapD, st);
+      float* w = term + r * PP_CHAIN + jl;
+      w[0] = v.x;
+      w[PP_ROWS * PP_CHAIN] = v.y;
+      w[2 * PP_ROWS * PP_CHAIN] = v.z;
+      w[3 * PP_ROWS * PP_CHAIN] = v.w;
     }
     __syncthreads();
-    float v[PP_COLS];
+    if (lane < 4 * PP_ROWS) {
+      float4 v[PP_COLS / 4];
 #pragma unroll
-    for (int c = 0; c < PP_COLS; c++) v[c] = R[c * 4];
+      for (int c = 0; c < PP_COLS / 4; c++) v[c] = chain[c];
 #pragma unroll
-    for (int c = 0; c < PP_COLS; c++) run += v[c];
+      for (int c = 0; c < PP_COLS / 4; c++) { run += v[c].x; run += v[c].y; run += v[c].z; run += v[c].w; }
+    }
     __syncthreads();
   }
-  const int ir = r0 + sr;
-  *out = (ir < count && list[ir].w != 0.0f) ? run * gm : 0.0f;
+  if (lane < 4 * PP_ROWS) {
+    const int ir = r0 + sr;
+    *out = (ir < count && list[ir].w != 0.0f) ? run * gm : 0.0f;
+  }
 }
 
 void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
