@@ -11,7 +11,15 @@
  *   step(e,x) = x < e ? 0 : 1, mod(x,y) = x - y*floor(x/y).
  * How closely each function is pinned (golden vectors from the reference's
  * own passes, FP64 brute force, invariants) is stated in the test files.
+ *
+ * CCO_MUT (0 unless set at build time: `make mutants`) plants one known
+ * mistake in the far-field and per-cell passes, so that the tests pinning
+ * them can show each planted mistake fails them (tests/test_cell_physics.py).
+ * The library the tests check against is built without it.
  */
+#ifndef CCO_MUT
+#define CCO_MUT 0
+#endif
 #include "cc_oracle.h"
 
 #include <math.h>
@@ -177,10 +185,19 @@ void cco_far(const float* coarse, float* farA, float* farT0, float* farT1) {
     for (int kz = 0; kz < 16; kz++) for (int ky = 0; ky < 16; ky++) for (int kx = 0; kx < 16; kx++) {
       int dx = abs(kx - c[0]), dy = abs(ky - c[1]), dz = abs(kz - c[2]);
       int dm = dx > dy ? dx : dy; dm = dm > dz ? dm : dz;
+#if CCO_MUT == 7
+      if (dm <= 1) continue;
+#else
       if (dm <= 2) continue;
+#endif
       const float* m = coarse + (size_t)coarse_at(kx, ky, kz) * 4;
       if (m[3] <= 0.0f) continue;
+#if CCO_MUT == 8
+      float r[3] = { (kx * 4 + 2) * 0.1875f - 6.0f - x[0], (ky * 4 + 2) * 0.1875f - 6.0f - x[1],
+                     (kz * 4 + 2) * 0.1875f - 6.0f - x[2] };
+#else
       float r[3] = { m[0] / m[3] - x[0], m[1] / m[3] - x[1], m[2] / m[3] - x[2] };
+#endif
       float inv2 = 1.0f / dot3(r, r), inv = sqrtf(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
       float s = m[3] * inv3;
       for (int a = 0; a < 3; a++) acc[a] += r[a] * s;
@@ -208,13 +225,22 @@ void cco_cell(const float* fine, const float* coarse, const float* farA, const f
     float x[3] = { mc[0] / mc[3], mc[1] / mc[3], mc[2] / mc[3] };
     int cc[3] = { c[0] >> 2, c[1] >> 2, c[2] >> 2 };
     float acc[3] = { 0, 0, 0 }, pot = 0.0f;
+#if CCO_MUT == 3
+    eps2 = eps2 * 2.0f;
+#endif
+#if CCO_MUT == 4
+    for (int kz = -1; kz <= 1; kz++) for (int ky = -1; ky <= 1; ky++) for (int kx = -1; kx <= 1; kx++) {
+#else
     for (int kz = -2; kz <= 2; kz++) for (int ky = -2; ky <= 2; ky++) for (int kx = -2; kx <= 2; kx++) {
+#endif
       int k[3] = { cc[0] + kx, cc[1] + ky, cc[2] + kz };
       if (k[0] < 0 || k[1] < 0 || k[2] < 0 || k[0] > 15 || k[1] > 15 || k[2] > 15) continue;
       if (coarse[(size_t)coarse_at(k[0], k[1], k[2]) * 4 + 3] <= 0.0f) continue;
       for (int fz = 0; fz < 4; fz++) for (int fy = 0; fy < 4; fy++) for (int fx = 0; fx < 4; fx++) {
         int f[3] = { k[0] * 4 + fx, k[1] * 4 + fy, k[2] * 4 + fz };
+#if CCO_MUT != 6
         if (f[0] == c[0] && f[1] == c[1] && f[2] == c[2]) continue;
+#endif
         const float* mf = fine + (size_t)fine_at(f[0], f[1], f[2]) * 4;
         if (mf[3] <= 0.0f) continue;
         float r[3] = { mf[0] / mf[3] - x[0], mf[1] / mf[3] - x[1], mf[2] / mf[3] - x[2] };
@@ -227,11 +253,27 @@ void cco_cell(const float* fine, const float* coarse, const float* farA, const f
     size_t ca = (size_t)coarse_at(cc[0], cc[1], cc[2]) * 4;
     const float *mb = coarse + ca, *fa = farA + ca, *t0 = farT0 + ca, *t1 = farT1 + ca;
     float d[3] = { x[0] - mb[0] / mb[3], x[1] - mb[1] / mb[3], x[2] - mb[2] / mb[3] };
+#if CCO_MUT == 5
+    float Td[3] = { t0[0] * d[0] + t0[3] * d[1] + t1[1] * d[2],
+                    t0[3] * d[0] + t0[1] * d[1] + t1[1] * d[2],
+                    t1[0] * d[0] + t1[1] * d[1] + t0[2] * d[2] };
+#else
     float Td[3] = { t0[0] * d[0] + t0[3] * d[1] + t1[0] * d[2],
                     t0[3] * d[0] + t0[1] * d[1] + t1[1] * d[2],
                     t1[0] * d[0] + t1[1] * d[1] + t0[2] * d[2] };
+#endif
+#if CCO_MUT == 1
+    for (int a = 0; a < 3; a++) acc[a] += fa[a];
+#elif CCO_MUT == 2
+    for (int a = 0; a < 3; a++) acc[a] += fa[a] - Td[a];
+#else
     for (int a = 0; a < 3; a++) acc[a] += fa[a] + Td[a];
+#endif
+#if CCO_MUT == 9
+    pot += fa[3] - dot3(fa, d);
+#else
     pot += fa[3] - dot3(fa, d) - 0.5f * dot3(d, Td);
+#endif
     force[me] = acc[0] * gm; force[me + 1] = acc[1] * gm; force[me + 2] = acc[2] * gm; force[me + 3] = pot * gm;
   }
 }

@@ -39,39 +39,61 @@ class SimU(C.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libcc_oracle.so")
-        if not os.path.exists(path):
-            raise RuntimeError("oracle library not built: run `make -C oracle`")
-        L = C.CDLL(path)
-        L.cco_grid_make.argtypes = [C.c_int, C.POINTER(Grid)]
-        L.cco_hash_at.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
-        L.cco_slots.argtypes = [f32p, C.c_int, C.c_float, C.POINTER(Grid), u32p, u32p]
-        L.cco_box_of.argtypes = [C.POINTER(Box), f32p, f32p]
-        L.cco_box_of.restype = C.c_int
-        L.cco_deposit.argtypes = [f32p, f32p, C.c_int, C.POINTER(Box), C.c_int, f32p, C.c_float, f32p]
-        L.cco_coarse.argtypes = [f32p, f32p]
-        L.cco_mono.argtypes = [f32p, f32p]
-        L.cco_far.argtypes = [f32p, f32p, f32p, f32p]
-        L.cco_cell.argtypes = [f32p, f32p, f32p, f32p, f32p, C.c_float, C.c_float, f32p]
-        L.cco_cell_f64.argtypes = [f32p, f32p, f32p, f32p, f32p, C.c_double, C.c_double, f64p]
-        L.cco_pp_gather.argtypes = [f32p, f32p, f32p, C.POINTER(Box), f32p, f32p]
-        L.cco_pp_force.argtypes = [f32p, f32p, C.c_int, C.c_float, C.c_float, C.c_float, f32p, f32p]
-        L.cco_sim.argtypes = [C.POINTER(SimU)] + [f32p] * 4 + [u32p, u32p] + [f32p] * 6 + [f32p] * 4
-        L.cco_rigid.argtypes = [C.c_int] + [f32p] * 4 + [f32p] * 8 + [C.c_float] + [f32p] * 4
-        L.cco_perm.argtypes = [C.c_int] + [f32p] * 5 + [f32p] * 4
-        L.cco_diag.argtypes = [C.POINTER(SimU), f32p, f32p, u32p, u32p] + [f32p] * 6 + [f32p]
-        L.cco_couple.argtypes = [C.c_int, f32p] + [f32p] * 4 + [f32p] * 4
-        L.cco_morton.argtypes = [f32p, C.c_int, f64p, i32p, f32p, i32p]
-        L.cco_books.argtypes = [C.c_int, C.c_double, f32p, f32p, f32p, f32p, f64p]
-        L.cco_mulberry_next.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
-        L.cco_mulberry_next.restype = C.c_uint32
-        L.cco_direct_f64.argtypes = [C.c_int, f32p, f32p, C.c_double, C.c_double, f64p]
-        L.cco_mesh_pair_table.argtypes = [C.c_int, C.c_double, f32p, f32p]
-        L.cco_fof_link.argtypes = [C.c_int, f32p, C.c_double, C.c_double, C.c_double, i32p, i32p, C.c_void_p, C.c_long,
-                                   C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_long]
-        L.cco_fof_link.restype = C.c_long
-        _LIB = L
+        _LIB = _load(os.path.join(_HERE, "libcc_oracle.so"))
     return _LIB
+
+
+class using:
+    """Within the block, the oracle functions call the library at `path`
+    (a build with a planted mistake, `make -C oracle mutants`) instead."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _LIB
+        lib()
+        self.saved, _LIB = _LIB, _load(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _LIB
+        _LIB = self.saved
+        return False
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError("oracle library not built: run `make -C oracle`")
+    L = C.CDLL(path)
+    L.cco_grid_make.argtypes = [C.c_int, C.POINTER(Grid)]
+    L.cco_hash_at.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.cco_slots.argtypes = [f32p, C.c_int, C.c_float, C.POINTER(Grid), u32p, u32p]
+    L.cco_box_of.argtypes = [C.POINTER(Box), f32p, f32p]
+    L.cco_box_of.restype = C.c_int
+    L.cco_deposit.argtypes = [f32p, f32p, C.c_int, C.POINTER(Box), C.c_int, f32p, C.c_float, f32p]
+    L.cco_coarse.argtypes = [f32p, f32p]
+    L.cco_mono.argtypes = [f32p, f32p]
+    L.cco_far.argtypes = [f32p, f32p, f32p, f32p]
+    L.cco_cell.argtypes = [f32p, f32p, f32p, f32p, f32p, C.c_float, C.c_float, f32p]
+    L.cco_cell_f64.argtypes = [f32p, f32p, f32p, f32p, f32p, C.c_double, C.c_double, f64p]
+    L.cco_pp_gather.argtypes = [f32p, f32p, f32p, C.POINTER(Box), f32p, f32p]
+    L.cco_pp_force.argtypes = [f32p, f32p, C.c_int, C.c_float, C.c_float, C.c_float, f32p, f32p]
+    L.cco_sim.argtypes = [C.POINTER(SimU)] + [f32p] * 4 + [u32p, u32p] + [f32p] * 6 + [f32p] * 4
+    L.cco_rigid.argtypes = [C.c_int] + [f32p] * 4 + [f32p] * 8 + [C.c_float] + [f32p] * 4
+    L.cco_perm.argtypes = [C.c_int] + [f32p] * 5 + [f32p] * 4
+    L.cco_diag.argtypes = [C.POINTER(SimU), f32p, f32p, u32p, u32p] + [f32p] * 6 + [f32p]
+    L.cco_couple.argtypes = [C.c_int, f32p] + [f32p] * 4 + [f32p] * 4
+    L.cco_morton.argtypes = [f32p, C.c_int, f64p, i32p, f32p, i32p]
+    L.cco_books.argtypes = [C.c_int, C.c_double, f32p, f32p, f32p, f32p, f64p]
+    L.cco_mulberry_next.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+    L.cco_mulberry_next.restype = C.c_uint32
+    L.cco_direct_f64.argtypes = [C.c_int, f32p, f32p, C.c_double, C.c_double, f64p]
+    L.cco_mesh_pair_table.argtypes = [C.c_int, C.c_double, f32p, f32p]
+    L.cco_fof_link.argtypes = [C.c_int, f32p, C.c_double, C.c_double, C.c_double, i32p, i32p, C.c_void_p, C.c_long,
+                               C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.c_long]
+    L.cco_fof_link.restype = C.c_long
+    return L
 
 
 def c32(a):
