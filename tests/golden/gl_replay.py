@@ -107,6 +107,7 @@ class Replayer:
         self.clear_color = (0.0, 0.0, 0.0, 0.0)
         self.prog_fs = {}      # program handle -> fragment shader source
         self.cell_draws = 0
+        self.marks = []        # (tag, wall time) at each __mark
 
     # -- helpers --
     def gen(self, fn, h, kind):
@@ -229,6 +230,10 @@ class Replayer:
             if op == "__dump":
                 g.glFinish()
                 self.dump(rec, trace_dir)
+                continue
+            if op == "__mark":   # every earlier command done, then the time noted
+                g.glFinish()
+                self.marks.append((rec["tag"], time.time()))
                 continue
             if op == "createShader":
                 self.ids[rec["ret"]] = g.glCreateShader(a[0])
@@ -369,7 +374,8 @@ def main():
     t0 = time.time()
     timing = r.run(args.trace_dir)
     info = {"renderer": r.renderer, "version": r.version, "wall_s": time.time() - t0,
-            "per_op_s": {k: round(v, 4) for k, v in sorted(timing.items(), key=lambda kv: -kv[1])[:8]}}
+            "per_op_s": {k: round(v, 4) for k, v in sorted(timing.items(), key=lambda kv: -kv[1])[:8]},
+            "marks": [[tag, t - t0] for tag, t in r.marks]}
     json.dump(info, open(os.path.join(args.trace_dir, "replay.json"), "w"), indent=1)
     print(json.dumps(info))
 

@@ -239,13 +239,15 @@ def main():
             force[b] = oracle.cell(out["fine%d" % b], out["coarse%d" % b], out["farA%d" % b], out["farT0%d" % b],
                                    out["farT1%d" % b], eps2c, gm)
             f32file(os.path.join(d, "force%d.f32" % b), force[b])
-        # phase 2: DIAG and the step pass on the injected force mesh
-        ops = base + [{"op": "install_injector", "file": os.path.join(d, "force0.f32")},
+        # phase 2: DIAG and the step pass on the injected force meshes (each box its own)
+        inj = {"op": "install_injector", "file": os.path.join(d, "force0.f32")}
+        if kind == "twobox":
+            inj["file1"] = os.path.join(d, "force1.f32")
+        ops = base + [inj,
                       {"op": "set_pp", "list": os.path.join(d, "pp.i32")},
                       {"op": "diag"},
-                      {"op": "dump", "what": ["diag", "force0", "mono0"], "tag": "p2"}]
-        if kind == "twobox":
-            ops.append({"op": "upload_tex", "name": "force1", "file": os.path.join(d, "force1.f32")})
+                      {"op": "dump", "what": ["diag", "force0", "mono0"] + (["force1"] if kind == "twobox" else []),
+                       "tag": "p2"}]
         variants = [("full", 0, 0, 8 * sim["dt"]), ("settle", 1, 0, 8 * sim["dt"]), ("kick", 0, 1, 0.0)]
         for vn, st, kk, gdt in variants:
             ops += [{"op": "set_state", "dir": d, "name": "s"},
@@ -256,10 +258,34 @@ def main():
                 {"op": "set_sim", "fields": {"rel": {"r": [-6.0, 1.2, 1.0], "v": [0.8, 0.0, 0.0]}}},
                 {"op": "rigid", "dtF": 0.01},
                 {"op": "dump", "what": ["pos", "vel", "aux", "q"], "tag": "rigid"}]
+        # the reference's own step() from the same state, eight of them from a mesh step (src/nbody.js:551-558):
+        # seats every second step, the mesh, the pairwise pass and the 8 dt impulse on the first, the
+        # softening (src/nbody.js:545-550) due on the fifth; sim.t advanced by the caller after each step
+        steps = []
+        if kind == "impact":
+            soft_at = 4
+            sfields = dict(fields, stepNo=0, soft=False, impactT=fields["t"] - 1.5 + (soft_at - 0.5) * sim["dt"])
+            ops += [{"op": "set_state", "dir": d, "name": "s"},
+                    {"op": "set_sim", "fields": sfields},
+                    {"op": "set_pp", "list": os.path.join(d, "pp.i32")}]
+            for n in range(8):
+                ops.append({"op": "step", "settle": 0, "kick": 0, "advance": 1})
+                if n in (0, 1, soft_at, 7):
+                    ops.append({"op": "dump", "what": ["pos", "vel", "aux", "q"], "tag": "step%d" % n})
+                    steps.append(n)
+            ops.append({"op": "record_sim", "tag": "steps", "fields": ["t", "stepNo", "dt", "k", "c", "damp", "soft"]})
         p2 = trace_and_replay(args.work, ops, kind + "_p2")
         draws2 = uniform_snapshots(p2)
         out["diag"] = load(p2, "p2", "diag")
         assert np.array_equal(load(p2, "p2", "force0"), force[0].reshape(513, 512, 4)), "injection not exact"
+        if kind == "twobox":
+            assert np.array_equal(load(p2, "p2", "force1"), force[1].reshape(513, 512, 4)), "box B injection not exact"
+        if steps:
+            for n in steps:
+                for nm in ("pos", "vel", "aux", "q"):
+                    out["step%d_%s" % (n, nm)] = load(p2, "step%d" % n, nm)
+            rec = [o for o in json.load(open(os.path.join(p2, "host.json")))["ops"] if o["op"] == "record_sim"][-1]
+            out["steps_meta"] = json.dumps({"fields": sfields, "after": rec["sim"], "dumped": steps})
         for vn, *_ in variants:
             for nm in ("pos", "vel", "aux", "q"):
                 out["sim_%s_%s" % (vn, nm)] = load(p2, "sim_" + vn, nm)
@@ -298,7 +324,7 @@ def main():
         for nm in ("ppList", "ppBox", "ppForce", "diag"):
             fx[nm] = out[nm]
         for k, v in out.items():
-            if k.startswith("sim_") or k.startswith("rigid_"):
+            if k.startswith("sim_") or k.startswith("rigid_") or k.startswith("step"):
                 fx[k] = v
         np.savez_compressed(os.path.join(args.out, "c1_%s.npz" % kind), **fx)
         print("case %s: seats %d cells, fine %d texels" % (kind, len(fx["seatA_idx"]), len(fx["fine0_idx"])))

@@ -19,6 +19,7 @@
 //   {"op":"diag"}                               CC.nbody.passes.diag (boxes, seats, mesh, pp, DIAG)
 //   {"op":"rigid", "dtF":x}                     CC.nbody.rigidStep
 //   {"op":"dump", "what":[...], "tag":"..."}    trace command: read textures back
+//   {"op":"mark", "tag":"..."}                  trace command: finish the GPU work and note the time
 // Host-side values (sim scalars, the pair table, uniforms) are written to
 // <out>/host.json directly from the evaluated reference.
 'use strict';
@@ -104,6 +105,11 @@ const special = {
   getActiveUniform: (p, i) => ({ name: p.__uniforms[i] }),
   getUniformLocation: (p, name) => ({ __loc: name, __prog: p.__h })
 };
+// Per-box force injection (see install_injector): when the reference binds a
+// mesh box's force framebuffer, texture unit 10 is re-bound to that box's
+// injected force mesh, and the active unit put back.
+let activeUnit = ENUM.TEXTURE0;
+let injectFor = null;   // (framebuffer handle id) -> injected texture, or null
 const gl = new Proxy({}, {
   get(target, prop) {
     if (typeof prop !== 'string') return undefined;
@@ -112,11 +118,17 @@ const gl = new Proxy({}, {
       const args = Array.prototype.slice.call(arguments);
       if (prop === 'shaderSource') shaderSrc.set(args[0].__h, args[1]);
       if (prop === 'attachShader') { const l = progShaders.get(args[0].__h) || []; l.push(args[1].__h); progShaders.set(args[0].__h, l); }
+      if (prop === 'activeTexture') activeUnit = args[0];
       let ret;
       if (special[prop]) ret = special[prop].apply(null, args);
       const rec = { op: prop, args: args.map(enc) };
       if (ret && ret.__h) rec.ret = ret.__h;
       trace.push(rec);
+      if (prop === 'bindFramebuffer' && injectFor && args[1] && injectFor.has(args[1].__h)) {
+        trace.push({ op: 'activeTexture', args: [ENUM.TEXTURE10] });
+        trace.push({ op: 'bindTexture', args: [ENUM.TEXTURE_2D, enc(injectFor.get(args[1].__h))] });
+        trace.push({ op: 'activeTexture', args: [activeUnit] });
+      }
       return ret;
     };
   }
@@ -298,23 +310,35 @@ for (const op of job.ops) {
       // The per-cell gravity pass (CELL_FS) is replaced, through the
       // reference's own setProg hook, by a copy of a texture on unit 10 that
       // holds a force mesh computed elsewhere: SwiftShader cannot run CELL_FS
-      // in useful time. Every other pass runs as written.
+      // in useful time. Every other pass runs as written. op.file is the
+      // first box's force mesh; op.file1, when given, the second box's (unit
+      // 10 follows the box whose force framebuffer is bound).
       const INJ_FS = ['#version 300 es', 'precision highp float;', 'precision highp sampler2D;',
                       'uniform sampler2D uInject;', 'out vec4 o;',
                       'void main() { o = texelFetch(uInject, ivec2(gl_FragCoord.xy), 0); }'].join('\n');
       NB.setProg('cell', CC.glsl.QUAD_VS, INJ_FS);
       const cp = NB.progs.cell;
       gl.useProgram(cp.p); gl.uniform1i(cp.u.uInject, 10);
-      injTex = gl.createTexture();
-      gl.activeTexture(gl.TEXTURE10);
-      gl.bindTexture(gl.TEXTURE_2D, injTex);
-      gl.texParameteri(gl.TEXTURE_2D, gl.TEXTURE_MIN_FILTER, gl.NEAREST);
-      gl.texParameteri(gl.TEXTURE_2D, gl.TEXTURE_MAG_FILTER, gl.NEAREST);
-      const data = op.file ? readF32(path.dirname(op.file), path.basename(op.file)) : new Float32Array(512 * 513 * 4);
-      gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 512, 513, 0, gl.RGBA, gl.FLOAT, data);
+      const tex = (file) => {
+        const t = gl.createTexture();
+        gl.activeTexture(gl.TEXTURE10);
+        gl.bindTexture(gl.TEXTURE_2D, t);
+        gl.texParameteri(gl.TEXTURE_2D, gl.TEXTURE_MIN_FILTER, gl.NEAREST);
+        gl.texParameteri(gl.TEXTURE_2D, gl.TEXTURE_MAG_FILTER, gl.NEAREST);
+        const data = file ? readF32(path.dirname(file), path.basename(file)) : new Float32Array(512 * 513 * 4);
+        gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 512, 513, 0, gl.RGBA, gl.FLOAT, data);
+        return t;
+      };
+      injTex = tex(op.file);
+      if (op.file1) {
+        const t1 = tex(op.file1);
+        injectFor = new Map([[NB.pm[0].forceFbo.__h, injTex], [NB.pm[1].forceFbo.__h, t1]]);
+        gl.bindTexture(gl.TEXTURE_2D, injTex);
+      }
       gl.activeTexture(gl.TEXTURE0);
       break;
     }
+    case 'mark': trace.push({ op: '__mark', args: [], tag: op.tag }); break;
     case 'upload_tex': {   // overwrite one engine texture (e.g. the second box's force mesh)
       const t = texOf(op.name);
       upload(t.tex, t.w, t.h, readF32(path.dirname(op.file), path.basename(op.file)), gl.RGBA);

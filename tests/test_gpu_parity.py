@@ -175,8 +175,6 @@ def test_step_pass(case, ctxs, variant):
 
 
 def test_diag(case, ctxs):
-    if case["name"] == "twobox":
-        pytest.skip("the DIAG golden is recorded with the second box's force mesh not injected")
     ctx = ctxs(case)
     _prepare_step(ctx, case)
     out = ctx.diag()

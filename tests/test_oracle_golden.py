@@ -142,8 +142,6 @@ def test_step_pass(case, variant):
 
 
 def test_diag(case):
-    if case["name"] == "twobox":
-        pytest.skip("the DIAG golden is recorded with the second box's force mesh not injected")
     u = gu.draws(case, "diag")[-1]
     su = gu.sim_u(u)
     sa, sb, f0, m0, f1, m1, slot = _inputs(case)
@@ -207,3 +205,68 @@ def test_force_error_vs_fp64_direct_sum(name):
     print(name, out)
     assert out["a"][0] < 0.10 and out["a"][1] < 0.08
     assert out["mesh"][1] < 0.02
+
+
+def step_chain(fx, force, mono, ppf, gdt_check=True):
+    """The reference's step() schedule (src/nbody.js:551-558) restated over the
+    oracle's passes: seats rebuilt on even steps, the step pass every step with
+    the uniforms the reference's pass received (gdt 8 dt on the mesh step, 0
+    otherwise), from the impact state; the mesh step's force mesh, monopole and
+    pairwise sums given.  Returns {step: (pos, vel, aux, q)} at the dumped steps."""
+    meta = json.loads(str(fx["steps_meta"]))
+    sims = gu.draws(fx, "sim")[3:]
+    N = fx["pos"].shape[0]
+    slot = gu.pp_slot(N, fx["pp_index"])
+    P, V, Q = fx["pos"], fx["vel"], fx["q"]
+    out = {}
+    for n in range(8):
+        u = sims[n]
+        if gdt_check:
+            assert u["uGDt"][0] == (np.float32(8 * u["uDt"][0]) if n == 0 else 0.0), ("impulse", n)
+        if n % 2 == 0:
+            sa, sb = oracle.slots(P, gu.f(u, "uInvCell"), oracle.grid(int(u["uGMask"][0]) + 1))
+        P, V, A, Q = oracle.sim(gu.sim_u(u), P, V, fx["mat"], Q, sa, sb, force, mono, np.zeros_like(force),
+                                np.zeros(4, np.float32), slot, ppf)
+        if n in meta["dumped"]:
+            out[n] = (P, V, A, Q)
+    return out
+
+
+def test_step_schedule_reproduces_reference_steps():
+    """A13: eight of the reference's own step() calls from a mesh step, with
+    the softening due on the fifth (the golden's dumps after steps 0, 1, 4 and
+    7).  Read from the reference's recorded draws: the seats are rebuilt
+    before steps 0, 2, 4, 6 (eight depth-peel passes each), the mesh and the
+    pairwise pass run before step 0 only, gdt is 8 dt on step 0 and 0 after,
+    and dt, k, c and the damping change from step 4 on.  The oracle's passes
+    chained on that schedule reproduce the reference's states bit for bit."""
+    fx = gu.fixture("c1_impact")
+    meta = json.loads(str(fx["steps_meta"]))
+    draws = json.loads(str(fx["draws"]))
+    first = [i for i, d in enumerate(draws) if d["kind"] == "sim"][3]
+    kinds = [d["kind"] for d in draws[first - 1:]]
+    # the draws of the eight steps, split at each step pass
+    per_step, cur = [], []
+    for k in [d["kind"] for d in draws[[i for i, d in enumerate(draws) if d["kind"] == "sim"][2] + 1:]]:
+        cur.append(k)
+        if k == "sim":
+            per_step.append(cur)
+            cur = []
+    per_step = per_step[-8:]
+    for n, ks in enumerate(per_step):
+        assert ks.count("slot") == (8 if n % 2 == 0 else 0), (n, ks)
+        assert ("dep" in ks) == (n == 0) and ("ppforce" in ks) == (n == 0), (n, ks)
+    assert kinds[-1] == "sim"
+    sims = gu.draws(fx, "sim")[3:]
+    dts = [float(u["uDt"][0]) for u in sims]
+    soft = meta["fields"]["impactT"]
+    assert all(d == dts[0] for d in dts[:4]) and all(d == dts[4] for d in dts[4:]), dts
+    assert dts[4] == pytest.approx(dts[0] * np.sqrt(2), rel=1e-6)
+    assert float(sims[4]["uK"][0]) == pytest.approx(float(sims[0]["uK"][0]) / 2, rel=1e-6)
+    assert meta["after"]["soft"] and meta["after"]["stepNo"] == 8 and soft < 0
+    out = step_chain(fx, gu.dense(fx, "force0", (513, 512)), gu.mesh(fx, 0)["mono"], fx["ppForce"])
+    N = fx["pos"].shape[0]
+    for n, (P, V, A, Q) in out.items():
+        for mine, nm in ((P, "pos"), (V, "vel"), (A, "aux")):
+            assert np.array_equal(mine, fx["step%d_%s" % (n, nm)].reshape(N, 4)), (n, nm)
+        assert np.array_equal(Q, fx["step%d_q" % n].reshape(N, 4)[:, 0]), (n, "q")
