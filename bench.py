@@ -64,6 +64,33 @@ def cpu_baseline(sc, pos, vel, mat, q, budget_s=15.0):
                       "262,144-particle state, one core, %.1f s" % (n, el)}
 
 
+def reference_cpu():
+    """The reference's own step() timed on SwiftShader in the development
+    container (profiles/reference_swiftshader_cpu.md, written by
+    tests/golden/time_reference.py): its twins figure, the workload of this
+    benchmark, or None."""
+    path = os.path.join(REPO, "profiles", "reference_swiftshader_cpu.md")
+    if not os.path.exists(path):
+        return None
+    rows = {}
+    for line in open(path):
+        if line.startswith("cpu_reference "):
+            name, rest = line[len("cpu_reference "):].split(":", 1)
+            f = [x.strip() for x in rest.split(",")]
+            rows[name] = {"value": float(f[0].split()[0]), "N": int(f[1].split()[1]), "steps": int(f[2].split()[1]),
+                          "wall_s": float(f[3].split()[1]), "cores": int(f[4].split()[1])}
+    tw = rows.get("twins")
+    if tw is None:
+        return None
+    return {"value": tw["value"], "unit": "particle-steps/s", "cores": tw["cores"], "kind": "reference-swiftshader",
+            "sample": "%d steps of the reference's own step() (src/nbody.js:551-558, its GLSL passes) on this "
+                      "benchmark's twins state (262,144 particles, 8192-seat loose list) under SwiftShader 4.1.0.7 on "
+                      "the development container's %d logical CPUs (Intel Xeon @ 2.70GHz), %.1f s; the per-cell pass "
+                      "CELL_FS, which SwiftShader cannot run, replaced by a copy of an injected force mesh (its cost "
+                      "left out); the reference cannot run on the GPU host" % (tw["steps"], tw["cores"], tw["wall_s"]),
+            "source": "profiles/reference_swiftshader_cpu.md", "other": {k: v for k, v in rows.items() if k != "twins"}}
+
+
 def measured_traffic():
     """k_sim's HBM bytes per launch as the committed counter profile states it
     (FETCH_SIZE + WRITE_SIZE scaled by the same counters on a plain copy of
@@ -137,16 +164,20 @@ def learned_leg(state_root, steps, warmup, reuse, cycles, iters):
 
 
 def runto_leg(steps, every=360):
-    """The JavaScript engine driven as the reference's page drives a run: full
-    steps with a synchronous analysis (groups, books, boxes, loose list, a
-    Morton sort every eighth) every 360 steps."""
-    cmd = ["node", os.path.join(PKG, "js", "run_timed.js"), json.dumps(runs.TWINS), str(steps), str(every)]
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
-    if r.returncode != 0:
-        raise RuntimeError("run_timed.js failed (exit %d):\n%s" % (r.returncode, r.stderr[-3000:]))
-    out = json.loads(r.stdout.strip().splitlines()[-1])
-    out["workload"] = "plain twins through the JavaScript engine (js/nbody.js over the N-API addon), an analysis " \
-                      "every %d steps as index.html's runTo drives it" % every
+    """The twins run as configured, through the JavaScript engine as the
+    reference's page drives it: from first touch, full steps with an analysis
+    (groups, books, boxes, loose list, a Morton sort every eighth) every 360
+    steps -- synchronous as runTo calls it, and overlapped (the host's sums
+    running while the next 64 steps are taken, the worker's one-report lag)."""
+    out = {"workload": "plain twins through the JavaScript engine (js/nbody.js over the N-API addon), %d steps from "
+                       "first touch, an analysis every %d steps as index.html's runTo drives it; window_ms_per_step: "
+                       "the step time in each window between looks" % (steps, every)}
+    for mode in ("sync", "overlap"):
+        cmd = ["node", os.path.join(PKG, "js", "run_timed.js"), json.dumps(runs.TWINS), str(steps), str(every), mode]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("run_timed.js failed (exit %d):\n%s" % (r.returncode, r.stderr[-3000:]))
+        out[mode] = json.loads(r.stdout.strip().splitlines()[-1])
     return out
 
 
@@ -177,7 +208,7 @@ def main():
                     help="instances in the batched-sweep leg (64, the sweep's size; 0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=240)
     ap.add_argument("--batch-dir", default=os.path.join(tempfile.gettempdir(), "cc_batch_state"))
-    ap.add_argument("--runto", type=int, default=3600, help="steps of the engine-driven leg with analyses (0: skip)")
+    ap.add_argument("--runto", type=int, default=5000, help="steps of the engine-driven leg with analyses (0: skip)")
     ap.add_argument("--learned", type=int, default=1, help="the learned far field leg (0: skip it)")
     ap.add_argument("--build", type=int, default=1, help="the body setup leg on the host (0: skip it)")
     ap.add_argument("--tune", type=int, default=1, help="cc_autotune the plain leg's launch shapes first (0: defaults)")
@@ -240,9 +271,12 @@ def main():
                           "what": "the whole step at the metric's value: compulsory bytes per particle-step "
                                   "117 + 12.6e6 / (8 N)"},
     }
+    out["cpu_baseline"] = reference_cpu()
     if not args.no_cpu:
         P, V, _, Q, M = R.ctx.download()
-        out["cpu_baseline"] = cpu_baseline(twins_like(run), P, V, M, Q)
+        out["cpu_baseline_port"] = cpu_baseline(twins_like(run), P, V, M, Q)
+        if out["cpu_baseline"] is None:
+            out["cpu_baseline"] = out["cpu_baseline_port"]
     R.close()
     if args.runto > 0:
         out["engine_runto"] = runto_leg(args.runto)

@@ -41,7 +41,7 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
-           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_crater_stats", "cc_pp_noalloc"]
+           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_crater_stats", "cc_pp_noalloc", "cc_analyse_begin", "cc_analyse_end"]
 
 
 class CraterArgs(C.Structure):
@@ -166,6 +166,8 @@ def lib():
             "cc_morton": ([P, dp, ip, ip], C.c_int), "cc_books": ([P, dp], C.c_int), "cc_sync": ([P], C.c_int),
             "cc_analyse": ([P, C.POINTER(AnalyseArgs), C.POINTER(Report), ip], C.c_int),
             "cc_last_step_ms": ([P, dp], C.c_int),
+            "cc_analyse_begin": ([P, C.POINTER(AnalyseArgs)], C.c_int),
+            "cc_analyse_end": ([P, C.c_int, C.POINTER(Report), ip], C.c_int),
             "cc_batch_create": ([C.POINTER(P), C.c_int, C.POINTER(P)], C.c_int), "cc_batch_destroy": ([P], C.c_int),
             "cc_batch_last_error": ([P], C.c_char_p),
             "cc_batch_step_n": ([P, C.c_int, C.POINTER(SimArgs), dp, ip], C.c_int),
@@ -421,6 +423,25 @@ class Context:
         rep = Report()
         lst = np.zeros(max(int(cap), 1), np.int32)
         self._chk(lib().cc_analyse(self.h, C.byref(args), C.byref(rep), _ptr(lst, C.c_int32)), "cc_analyse")
+        return report_dict(rep, lst)
+
+    def analyse_begin(self, m, a, spring, touch, cap=PP_CAP, diag=None, flags=0):
+        """cc_analyse_begin: the analysis' device half now, its host sums on a thread
+        (diag: an N x 4 DIAG read-out to use instead of the one on the device)."""
+        d = None if diag is None else _f(diag).reshape(-1)
+        if d is not None and d.size != self.N * 4:
+            raise CCError("diag must hold N x 4 floats")
+        args = AnalyseArgs(float(m), float(a), float(spring), float(touch), int(cap), int(flags), _ptr(d))
+        self._chk(lib().cc_analyse_begin(self.h, C.byref(args)), "cc_analyse_begin")
+
+    def analyse_end(self, wait=True):
+        """cc_analyse_end: the begun analysis' report (as analyse()), or None while its sums still run."""
+        rep = Report()
+        lst = np.zeros(PP_CAP, np.int32)
+        r = lib().cc_analyse_end(self.h, 1 if wait else 0, C.byref(rep), _ptr(lst, C.c_int32))
+        if r == 1:
+            return None
+        self._chk(r, "cc_analyse_end")
         return report_dict(rep, lst)
 
     def autotune(self, reps=20):

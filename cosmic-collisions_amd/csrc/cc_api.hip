@@ -992,6 +992,31 @@ int cc_analyse(cc_ctx* c, const cc_analyse_args* a, cc_report* out, int32_t* lis
   return 0;
 }
 
+// The analysis in two halves, the reference's analyze(false) and collect()
+// (src/nbody.js:895-1025): the device's part and the read-back now, the host's
+// FP64 sums on a thread of their own while the caller steps on.
+int cc_analyse_begin(cc_ctx* c, const cc_analyse_args* a) {
+  if (!c || !a) return CC_EINVAL;
+  if (a->cap < 0 || a->cap > PP_CAP) FAIL(c, CC_EINVAL, "cap %d outside 0..%d", a->cap, PP_CAP);
+  if (!(a->a > 0) || !(a->touch >= 0) || !(a->m > 0)) FAIL(c, CC_EINVAL, "m, a and touch must be positive");
+  if (!c->fof) c->fof = fof_create();
+  FofIn in{ c->N, c->pos[c->cur], c->vel[c->cur], c->mat, c->diag, a->diag, a->m, a->a, a->spring, a->touch,
+            a->cap, a->flags };
+  int r = fof_begin(c->fof, in, c->s, c->err);
+  if (r) return r;
+  LAUNCHCHK(c);
+  return 0;
+}
+
+int cc_analyse_end(cc_ctx* c, int wait, cc_report* out, int32_t* list) {
+  if (!c || !out) return CC_EINVAL;
+  if (!c->fof || !fof_pending(c->fof)) FAIL(c, CC_ESTATE, "no analysis was begun");
+  bool ready = false;
+  int r = fof_end(c->fof, out, list, wait != 0, &ready, c->err);
+  if (r) return r;
+  return ready ? 0 : 1;
+}
+
 int cc_books(cc_ctx* c, double* out) {
   if (!c || !out) return CC_EINVAL;
   launch_books(c->N, c->p.m, c->pos[c->cur], c->vel[c->cur], c->mat, c->diag, c->bpart, c->bout, c->s);

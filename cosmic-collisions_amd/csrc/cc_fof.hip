@@ -26,11 +26,13 @@
 // reference's JavaScript numbers do; Math.cbrt and Math.acos are V8's, whose
 // algorithms are restated below.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cc_hip.h"
@@ -207,6 +209,17 @@ struct FofWork {
   int *hLab = nullptr, *hCounts = nullptr;
   double* hE2 = nullptr;
   size_t hE2Cap = 0;
+  // a begun analysis (fof_begin): its host half on a thread of its own, over
+  // the pinned copies above, filling rep and list; fof_end joins it
+  std::thread host;
+  bool pending = false;
+  std::atomic<bool> done{ false };   // the host half has finished
+  FofHostIn hin{};
+  FofPick pick;
+  cc_report rep{};
+  std::vector<int32_t> list;
+  std::string hostErr;
+  double msCells = 0.0, msCopy = 0.0, msHost = 0.0;
 };
 
 #define FCHK(call)                                                                      \
@@ -217,6 +230,7 @@ struct FofWork {
 
 void fof_destroy(FofWork* w) {
   if (!w) return;
+  if (w->host.joinable()) w->host.join();   // a begun analysis' host half reads the pinned buffers
   for (void* p : { (void*)w->cell, (void*)w->key, (void*)w->end, (void*)w->order, (void*)w->parent,
                    (void*)w->nLink, (void*)w->nTouch, (void*)w->blockSums, (void*)w->partner, (void*)w->e2 })
     if (p) (void)hipFree(p);
@@ -275,7 +289,12 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipStream_t s, std::string& err) {
+// The device half: the cells, the components and the touching pairs on the
+// GPU, the arrays the host sums over copied to pinned memory, and which groups
+// are the largest and the second (with the reference's unions replayed when a
+// tie can reach the report).  Synchronous on stream s; afterwards w->hin and
+// w->pick hold everything the host half needs and the device may move on.
+static int fof_device(FofWork* w, const FofIn& in, hipStream_t s, std::string& err) {
   const int N = in.N;
   auto tA = std::chrono::steady_clock::now();
   int r = fof_alloc(w, N, err);
@@ -300,7 +319,7 @@ int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipS
   FCHK(hipGetLastError());
   FCHK(hipMemcpyAsync(w->hCounts, w->nTouch + N, 4, hipMemcpyDeviceToHost, s));
   FCHK(hipStreamSynchronize(s));
-  const double msCells = ms_since(tA);
+  w->msCells = ms_since(tA);
   auto tB = std::chrono::steady_clock::now();
   // the touching pairs' squared distances in the reference's order, and the
   // arrays the host sums over
@@ -318,15 +337,16 @@ int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipS
   if (in.hostDiag) std::memcpy(w->hD, in.hostDiag, (size_t)N * 16);
   else FCHK(hipMemcpyAsync(w->hD, in.diag, (size_t)N * 16, hipMemcpyDeviceToHost, s));
   FCHK(hipStreamSynchronize(s));
-  const double msCopy = ms_since(tB);
+  w->msCopy = ms_since(tB);
   auto tC = std::chrono::steady_clock::now();
 
-  FofHostIn h{ N, reinterpret_cast<const F4*>(w->hP), reinterpret_cast<const F4*>(w->hV),
-               reinterpret_cast<const F4*>(w->hD), w->hR, w->hLab, w->hE2, nT, in.m, in.a, in.spring, in.touch,
-               in.cap, in.flags };
+  w->hin = FofHostIn{ N, reinterpret_cast<const F4*>(w->hP), reinterpret_cast<const F4*>(w->hV),
+                      reinterpret_cast<const F4*>(w->hD), w->hR, w->hLab, w->hE2, nT, in.m, in.a, in.spring,
+                      in.touch, in.cap, in.flags };
+  const FofHostIn& h = w->hin;
   try {
-    FofPick pick = fof_pick(h);
-    if (pick.replay) {
+    w->pick = fof_pick(h);
+    if (w->pick.replay) {
       // every linked partner in the reference's order, for the host to replay its unions
       launch_scan_excl(w->nLink, N + 1, w->blockSums, s);
       FCHK(hipGetLastError());
@@ -341,17 +361,80 @@ int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipS
       FCHK(hipMemcpyAsync(off.data(), w->nLink, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, s));
       if (nLk) FCHK(hipMemcpyAsync(partner.data(), w->partner, (size_t)nLk * 4, hipMemcpyDeviceToHost, s));
       FCHK(hipStreamSynchronize(s));
-      fof_pick_replay(h, off.data(), partner.data(), pick);
+      fof_pick_replay(h, off.data(), partner.data(), w->pick);
     }
-    fof_report(h, pick, out, list);
   } catch (const std::exception& x) {
     err = std::string("the analysis' host half: ") + x.what();
     return CC_EDEVICE;
   }
-  out->ms[0] = msCells;
-  out->ms[1] = msCopy;
-  out->ms[2] = ms_since(tC);
+  w->msHost = ms_since(tC);
   return 0;
 }
+
+// the host half: every FP64 sum of the report, over the pinned copies
+static void fof_host(FofWork* w) {
+  auto t0 = std::chrono::steady_clock::now();
+  w->hostErr.clear();
+  try {
+    fof_report(w->hin, w->pick, &w->rep, w->list.data());
+  } catch (const std::exception& x) {
+    w->hostErr = std::string("the analysis' host half: ") + x.what();
+  }
+  w->msHost += ms_since(t0);
+  w->done.store(true, std::memory_order_release);
+}
+
+// a host half still running from fof_begin, ended and its result dropped
+static void fof_drop(FofWork* w) {
+  if (w->host.joinable()) w->host.join();
+  w->pending = false;
+}
+
+static int fof_deliver(FofWork* w, cc_report* out, int32_t* list, std::string& err) {
+  if (!w->hostErr.empty()) { err = w->hostErr; return CC_EHOST; }
+  *out = w->rep;
+  if (list && w->rep.ppCount > 0) std::memcpy(list, w->list.data(), (size_t)w->rep.ppCount * 4);
+  out->ms[0] = w->msCells;
+  out->ms[1] = w->msCopy;
+  out->ms[2] = w->msHost;
+  return 0;
+}
+
+int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipStream_t s, std::string& err) {
+  fof_drop(w);   // a synchronous analysis supersedes one begun before it (the reference drops the worker's)
+  int r = fof_device(w, in, s, err);
+  if (r) return r;
+  w->list.assign((size_t)std::max(in.cap, 1), 0);
+  fof_host(w);
+  return fof_deliver(w, out, list, err);
+}
+
+int fof_begin(FofWork* w, const FofIn& in, hipStream_t s, std::string& err) {
+  if (w->pending) { err = "an analysis is still running: collect it (cc_analyse_end) first"; return CC_ESTATE; }
+  int r = fof_device(w, in, s, err);
+  if (r) return r;
+  w->list.assign((size_t)std::max(in.cap, 1), 0);
+  w->done.store(false, std::memory_order_relaxed);
+  try {
+    w->host = std::thread(fof_host, w);
+  } catch (const std::exception& x) {
+    err = std::string("the analysis' host thread: ") + x.what();
+    return CC_EHOST;
+  }
+  w->pending = true;
+  return 0;
+}
+
+int fof_end(FofWork* w, cc_report* out, int32_t* list, bool wait, bool* ready, std::string& err) {
+  *ready = false;
+  if (!w->pending) { err = "no analysis was begun"; return CC_ESTATE; }
+  if (!wait && !w->done.load(std::memory_order_acquire)) return 0;
+  w->host.join();
+  w->pending = false;
+  *ready = true;
+  return fof_deliver(w, out, list, err);
+}
+
+bool fof_pending(const FofWork* w) { return w && w->pending; }
 
 }  // namespace cc

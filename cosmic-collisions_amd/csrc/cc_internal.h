@@ -210,6 +210,11 @@ FofWork* fof_create();
 void fof_destroy(FofWork* w);
 // fills *out (a cc_report) and list; synchronous on stream s
 int fof_analyse(FofWork* w, const FofIn& in, cc_report* out, int32_t* list, hipStream_t s, std::string& err);
+// the same in two halves: the device's part and the read-back (synchronous on s), then the host's
+// sums on a thread of their own; fof_end waits for them and delivers the report
+int fof_begin(FofWork* w, const FofIn& in, hipStream_t s, std::string& err);
+int fof_end(FofWork* w, cc_report* out, int32_t* list, bool wait, bool* ready, std::string& err);
+bool fof_pending(const FofWork* w);
 
 }  // namespace cc
 

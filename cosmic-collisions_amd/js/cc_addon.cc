@@ -423,26 +423,10 @@ napi_value set_vec(napi_env env, napi_value o, const char* k, const double* v, i
 }
 
 // analyse(ctx, {m, a, spring, touch, cap, flags}) -> fof()'s report (src/nbody.js:888-892)
-napi_value Analyse(napi_env env, napi_callback_info info) {
-  CTX0();
-  napi_value o = a.argv[1];
-  cc_analyse_args in;
-  memset(&in, 0, sizeof in);
-  in.m = num(env, prop(env, o, "m"));
-  in.a = num(env, prop(env, o, "a"));
-  in.spring = num(env, prop(env, o, "spring"));
-  in.touch = num(env, prop(env, o, "touch"));
-  in.cap = (int32_t)num(env, prop(env, o, "cap"), 8192);
-  in.flags = (int32_t)num(env, prop(env, o, "flags"));
-  in.diag = nullptr;   // the DIAG read-out cc_diag left on the device
-  if (in.cap < 0 || in.cap > 8192) { napi_throw_range_error(env, nullptr, "cap outside 0..8192"); return nullptr; }
-  cc_report r;
-  napi_value ab, list;
-  void* data;
-  NAPI_OK(env, napi_create_arraybuffer(env, (size_t)(in.cap > 0 ? in.cap : 1) * 4, &data, &ab));
-  int code = cc_analyse(c, &in, &r, static_cast<int32_t*>(data));
-  if (code) return throw_cc(env, c, "cc_analyse", code);
-  NAPI_OK(env, napi_create_typedarray(env, napi_int32_array, (size_t)r.ppCount, ab, 0, &list));
+// fof()'s report (src/nbody.js:888-892) as the reference's object, pp.list over `ab`
+static napi_value report_object(napi_env env, const cc_report& r, napi_value ab) {
+  napi_value list;
+  if (napi_create_typedarray(env, napi_int32_array, (size_t)r.ppCount, ab, 0, &list) != napi_ok) return nullptr;
   napi_value out, second, pp, nat;
   napi_create_object(env, &out);
   set_num(env, out, "largest", r.largest); set_num(env, out, "largestImp", r.largestImp);
@@ -483,6 +467,55 @@ napi_value Analyse(napi_env env, napi_callback_info info) {
   set_vec(env, nat, "ms", r.ms, 3);
   napi_set_named_property(env, out, "analysis", nat);
   return out;
+}
+
+static bool analyse_args(napi_env env, napi_value o, cc_analyse_args* in) {
+  memset(in, 0, sizeof *in);
+  in->m = num(env, prop(env, o, "m"));
+  in->a = num(env, prop(env, o, "a"));
+  in->spring = num(env, prop(env, o, "spring"));
+  in->touch = num(env, prop(env, o, "touch"));
+  in->cap = (int32_t)num(env, prop(env, o, "cap"), 8192);
+  in->flags = (int32_t)num(env, prop(env, o, "flags"));
+  in->diag = nullptr;   // the DIAG read-out cc_diag left on the device
+  if (in->cap < 0 || in->cap > 8192) { napi_throw_range_error(env, nullptr, "cap outside 0..8192"); return false; }
+  return true;
+}
+
+napi_value Analyse(napi_env env, napi_callback_info info) {
+  CTX0();
+  cc_analyse_args in;
+  if (!analyse_args(env, a.argv[1], &in)) return nullptr;
+  cc_report r;
+  napi_value ab;
+  void* data;
+  NAPI_OK(env, napi_create_arraybuffer(env, (size_t)(in.cap > 0 ? in.cap : 1) * 4, &data, &ab));
+  int code = cc_analyse(c, &in, &r, static_cast<int32_t*>(data));
+  if (code) return throw_cc(env, c, "cc_analyse", code);
+  return report_object(env, r, ab);
+}
+
+// analyseBegin(ctx, args): the device's half now, the host's sums on a thread (src/nbody.js:967-1008)
+napi_value AnalyseBegin(napi_env env, napi_callback_info info) {
+  CTX0();
+  cc_analyse_args in;
+  if (!analyse_args(env, a.argv[1], &in)) return nullptr;
+  RET(cc_analyse_begin(c, &in), "cc_analyse_begin");
+}
+
+// analyseEnd(ctx, wait): the begun analysis' report, or null while its sums still run (collect, src/nbody.js:1011-1025)
+napi_value AnalyseEnd(napi_env env, napi_callback_info info) {
+  CTX0();
+  bool wait = false;
+  napi_get_value_bool(env, a.argv[1], &wait);
+  cc_report r;
+  napi_value ab;
+  void* data;
+  NAPI_OK(env, napi_create_arraybuffer(env, (size_t)8192 * 4, &data, &ab));
+  int code = cc_analyse_end(c, wait ? 1 : 0, &r, static_cast<int32_t*>(data));
+  if (code < 0) return throw_cc(env, c, "cc_analyse_end", code);
+  if (code == 1) { napi_value nul; napi_get_null(env, &nul); return nul; }
+  return report_object(env, r, ab);
 }
 
 // relaxPiles([{p: Float32Array (n x 3), n, a, A, C, confine, sweeps}, ...]) -> [{sweeps, worst}, ...]:
@@ -579,7 +612,7 @@ napi_value Init(napi_env env, napi_value exports) {
     { "writeMesh", WriteMesh }, { "setPPTable", SetPPTable }, { "setPP", SetPP }, { "pp", PP }, { "readPP", ReadPP },
     { "sim", Sim }, { "step", Step }, { "stepN", StepN }, { "getStepNo", GetStepNo }, { "setStepNo", SetStepNo },
     { "setGraphs", SetGraphs },
-    { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "analyse", Analyse }, { "books", Books }, { "sync", Sync },
+    { "rigid", Rigid }, { "couple", Couple }, { "diag", Diag }, { "morton", Morton }, { "analyse", Analyse }, { "analyseBegin", AnalyseBegin }, { "analyseEnd", AnalyseEnd }, { "books", Books }, { "sync", Sync },
     { "relaxPiles", RelaxPiles }, { "craterStats", CraterStats },
   };
   for (auto& f : fns) {

@@ -202,6 +202,7 @@ function createEngine(opts) {
     run = run || buildRun(opt, addon);
     sim = run.sim;
     sim.gen = ++gen;
+    pending = null;                // a look at the last run is no longer wanted
     if (ctx) addon.destroy(ctx);   // a rebuild frees the last run's device objects (src/nbody.js:208-245)
     ctx = addon.create(params());
     addon.uploadState(ctx, run.pos, run.vel, sim.matR, new Float32Array(sim.N), sim.N);
@@ -336,19 +337,11 @@ function createEngine(opts) {
     pp.list = list; pp.count = list.length; pp.loose = loose;
     addon.setPP(ctx, list, list.length);
   }
-  // the analysis (src/nbody.js:967-1008, synchronous): groups, books, boxes, the loose list and,
-  // every eighth look in the full phase, the Morton order -- the GPU's, from this look's core
-  function analyze() {
-    sim.gen = ++gen;
-    analyzedAt = sim.stepNo;
-    const sort = sim.phase === 'full' && analyses++ % 8 === 0;
-    const ms = () => Number(process.hrtime.bigint()) / 1e6;
-    const t0 = ms();
-    passDiag(true);
-    const t1 = ms();
-    const r = addon.analyse(ctx, { m: sim.m, a: sim.a, spring: sim.k, touch: 2 * sim.a, cap: PP_CAP });
-    const t2 = ms();
-    r.t = sim.t;
+  // a report into the run (applyReport, src/nbody.js:912-928): boxes, the books' reference, every eighth
+  // look in the full phase the Morton order -- the GPU's, from the report's core, on the state as it is now --
+  // and the loose list
+  function applyReport(r, sort) {
+    const t2 = Number(process.hrtime.bigint()) / 1e6;
     report = r; sim.com1 = r.com;
     placeBoxes(r);
     const going = sim.phase === 'full' || sim.bulk0[0] !== 0 || sim.bulk0[1] !== 0 || sim.bulk0[2] !== 0;
@@ -367,9 +360,53 @@ function createEngine(opts) {
       r.perm = perm;
     }
     setPP(list, r.pp.loose);
-    const am = r.analysis.ms;
-    lookMs = { diag: t1 - t0, analyse: t2 - t1, cells: am[0], copy: am[1], host: am[2], rest: ms() - t2, sort };
     emit('report', r);
+    return Number(process.hrtime.bigint()) / 1e6 - t2;
+  }
+  // the analysis (src/nbody.js:967-1008): groups, books, boxes, the loose list.  sync: at once, the
+  // report returned.  Otherwise (and asyncRead, the default) the device's half and the read-back now,
+  // the host's sums on a thread of the library while the caller steps on; collect() applies the
+  // report when they are done, one report behind, as the reference's worker does.  A call while one
+  // is pending does nothing, as the reference's does.
+  let pending = null, asyncRead = true;
+  function analyze(sync) {
+    if (pending && !sync) return null;
+    if (sync) { sim.gen = ++gen; pending = null; }   // a synchronous look drops the pending one (the library does too)
+    analyzedAt = sim.stepNo;
+    const sort = sim.phase === 'full' && analyses++ % 8 === 0;
+    const ms = () => Number(process.hrtime.bigint()) / 1e6;
+    const t0 = ms();
+    passDiag(true);
+    const t1 = ms();
+    const args = { m: sim.m, a: sim.a, spring: sim.k, touch: 2 * sim.a, cap: PP_CAP };
+    if (!sync && asyncRead) {
+      addon.analyseBegin(ctx, args);
+      pending = { gen: sim.gen, sort, t: sim.t, diag: t1 - t0, begin: ms() - t1 };
+      return null;
+    }
+    const r = addon.analyse(ctx, args);
+    const t2 = ms();
+    r.t = sim.t;
+    const rest = applyReport(r, sort);
+    const am = r.analysis.ms;
+    lookMs = { diag: t1 - t0, analyse: t2 - t1, cells: am[0], copy: am[1], host: am[2], rest, sort };
+    return r;
+  }
+  // the pending analysis' report applied if its sums are done (wait: wait for them); null otherwise
+  function collect(wait) {
+    if (!pending) return null;
+    const t0 = Number(process.hrtime.bigint()) / 1e6;
+    const r = addon.analyseEnd(ctx, !!wait);
+    if (!r) return null;
+    const p = pending;
+    pending = null;
+    if (p.gen !== sim.gen) return null;   // the run was rebuilt, or looked at synchronously, since
+    r.t = p.t;
+    const t1 = Number(process.hrtime.bigint()) / 1e6;
+    const rest = applyReport(r, p.sort);
+    const am = r.analysis.ms;
+    lookMs = { diag: p.diag, analyse: p.begin + (t1 - t0), cells: am[0], copy: am[1], host: am[2], rest, sort: p.sort,
+               overlapped: true };
     return r;
   }
 
@@ -378,22 +415,22 @@ function createEngine(opts) {
     // the device objects freed now rather than when the handle is collected
     destroy() { if (ctx) { addon.destroy(ctx); ctx = null; } },
     build, step, stepMany, settleSome, rigidStep, soften,
-    analyze, collect() {}, readState,
-    sortNow() { analyses = 0; return analyze(); },
+    analyze, collect, readState,
+    sortNow() { analyses = 0; return analyze(true); },
     passes: {
       slots: () => addon.slots(ctx), deposit: (b) => addon.deposit(ctx, b || 0), coarse: (b) => addon.coarse(ctx, b || 0),
       cell: (b) => addon.cell(ctx, b || 0), mesh: () => addon.mesh(ctx), pp: () => addon.pp(ctx),
       sim: (settle, kick, gdt) => addon.sim(ctx, simArgs(settle, kick, gdt)), diag: passDiag
     },
     onSim(f) { taps.sim.push(f); }, onReport(f) { taps.report.push(f); },
-    get sim() { return sim; }, get report() { return report; }, get pending() { return null; },
+    get sim() { return sim; }, get report() { return report; }, get pending() { return pending; },
     get analyzedAt() { return analyzedAt; }, get pp() { return pp; }, get ctx() { return ctx; }, get addon() { return addon; },
     // the last analysis's phases, ms: seats+mesh+pairwise+DIAG; cc_analyse, of it the device's cells and
     // groups, the read-back, the host's sums; the rest (boxes, Morton sort, loose list)
     get lookMs() { return lookMs; },
-    get state() { return { analyzing: false, pending: false, worker: false, analyses, analyzedAt, stepNo: sim && sim.stepNo }; },
+    get state() { return { analyzing: !!pending, pending: !!pending, worker: true, analyses, analyzedAt, stepNo: sim && sim.stepNo }; },
     set meshEvery(n) { MESH_EVERY = n; if (ctx) addon.setParams(ctx, params()); },
-    set asyncRead(v) {},
+    set asyncRead(v) { asyncRead = !!v; },
     pairTable: PP_TAB,
     T_UNIT, V_UNIT, E_KG, CP_ROCK, CP_IRON, L_EM, JAM, PP_CAP, meanK
   };

@@ -190,6 +190,17 @@ typedef struct cc_report {   /* fof()'s report (src/nbody.js:888-892), same name
   double ms[3];       /* device, transfer and host milliseconds of this call */
 } cc_report;
 int cc_analyse(cc_ctx* ctx, const cc_analyse_args* args, cc_report* out, int32_t* list);
+/* analyze(false) and collect() (src/nbody.js:967-1025): the same analysis in
+ * two halves.  cc_analyse_begin runs the device's part and reads the state back
+ * (synchronously, as the reference's passDiag and read-back are issued), then
+ * starts the report's FP64 sums on a host thread and returns; the caller steps
+ * on meanwhile.  cc_analyse_end delivers that report: 0 with *out and list
+ * filled, 1 (wait = 0) while the sums still run; wait != 0 waits for them.  One
+ * analysis at a time: cc_analyse_begin while one is pending is CC_ESTATE, and a
+ * synchronous cc_analyse drops a pending one (as the reference drops the
+ * worker's result when it analyses synchronously). */
+int cc_analyse_begin(cc_ctx* ctx, const cc_analyse_args* args);
+int cc_analyse_end(cc_ctx* ctx, int wait, cc_report* out, int32_t* list);
 
 /* wait for the ctx's stream */
 int cc_sync(cc_ctx* ctx);

@@ -36,7 +36,7 @@ while (sim.phase === 'settle') NB.settleSome(64);
 while (sim.phase === 'approach') NB.rigidStep(0.01);
 const books = [];
 const look = () => {
-  const r = NB.analyze(), ref = sim.ref, E = r.KE + r.PE + r.EL;
+  const r = NB.analyze(true), ref = sim.ref, E = r.KE + r.PE + r.EL;
   books.push({ step: sim.stepNo, t: sim.t, L: r.L, dL: (r.L - ref.L) / ref.L, balance: (E + r.Q - ref.EQ) / ref.scale,
                drift: r.drift, Q: r.Q, KE: r.KE, PE: r.PE, EL: r.EL, unseated: r.unseated, loose: r.pp.loose });
   console.error(JSON.stringify(books[books.length - 1]));

@@ -122,3 +122,35 @@ def test_analysis_262k_against_oracle(tmp_path):
     same_report(again, ref)
     assert mine["pp"]["loose"] == ref["ppLoose"]
     assert np.array_equal(mine["pp"]["list"], lst)
+
+
+def test_overlapped_analysis_equals_synchronous():
+    """cc_analyse_begin / cc_analyse_end (the reference's analyze(false) and
+    collect(), src/nbody.js:967-1025): the device's half and the read-back at
+    the call, the host's sums on a thread while the context steps on.  The
+    report is the synchronous report of the state it was begun on, bit for
+    bit; a second begin before the end, and an end with none begun, are
+    refused."""
+    fx = gu.fixture("c1_impact")
+    s = json.loads(str(fx["sim"]))
+    ctx = context(fx["pos"], fx["vel"], fx["mat"], s, s["W"])
+    try:
+        ref = analyse(ctx, s, fx["diag"])
+        ctx.analyse_begin(s["m"], s["a"], s["k"], 2 * s["a"], diag=fx["diag"])
+        with pytest.raises(ccamd.CCError):
+            ctx.analyse_begin(s["m"], s["a"], s["k"], 2 * s["a"], diag=fx["diag"])
+        print("overlapped analysis: stepping while the host sums run", flush=True)
+        ctx.step_n(16, ccamd.sim_args())
+        got = ctx.analyse_end(wait=True)
+        assert not np.array_equal(ctx.download()[0], fx["pos"]), "the state moved meanwhile"
+        for k, v in ref.items():
+            if k == "ms":
+                continue
+            if k == "pp":
+                assert np.array_equal(got["pp"]["list"], v["list"]) and got["pp"]["loose"] == v["loose"]
+            else:
+                assert got[k] == v, k
+        with pytest.raises(ccamd.CCError):
+            ctx.analyse_end(wait=True)
+    finally:
+        ctx.close()
