@@ -837,14 +837,17 @@ __device__ __forceinline__ void cross_at(float4 mo, float4 mm, float gm, float e
 // A wavefront runs the union of its lanes' work, so the contacts are found
 // in three compacted phases instead of eight predicated seat slots per cell:
 //   A  the seats that pass the cell and seat-place prefilters, in order,
-//      into the lane's LDS list;
-//   B  their positions: linked pairs counted, touching pairs kept in order
+//      into the lane's LDS list (the particle's own seat among them);
+//   B  their positions: linked pairs other than the particle itself counted
+//      (the reference skips its own index first), touching pairs kept in order
 //      (r^2 < touch2r: the r^2 below which sqrt(r^2) rounds under the contact
 //      distance -- the reference's touch test without its square root);
 //   C  the touching pairs' velocities and masses, and the sums.
 // A lane whose list would overflow finishes alone on the direct path.
 constexpr int SIM_BLOCK = 64;
-constexpr int CAND_CAP = 39;
+constexpr int CAND_CAP = 31;                 // a list longer than this takes the direct path
+constexpr int CAND_ROWS = CAND_CAP + 9;      // A's writes run up to a cell's eight seats past the row after the
+                                             // cap; B reads eight rows at a time from below the cap
 
 struct Contact {
   float pushx, pushy, pushz, nb, lost, flow;
@@ -896,25 +899,28 @@ __device__ __forceinline__ float bload_f1(__amdgpu_buffer_rsrc_t r, uint32_t off
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
+// The lattice offsets lie within +-2048, so the squares are exact in the
+// 24-bit multiply-add, which reads its operands' low 24 bits as signed: the
+// sign extension that __mul24 spells out is a no-op here and is left out.
 __device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int bz) {
   int ex = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B, 4u) << 5) + bx;
   int ey = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B + 4u, 4u) << 5) + by;
   int ez = (int)(__builtin_amdgcn_ubfe(sv, (uint32_t)B + 8u, 4u) << 5) + bz;
-  return __mul24(ex, ex) + __mul24(ey, ey) + __mul24(ez, ez);
+  int d2;
+  asm("v_mul_i32_i24 %0, %1, %1\n\tv_mad_i32_i24 %0, %2, %2, %0\n\tv_mad_i32_i24 %0, %3, %3, %0"
+      : "=&v"(d2) : "v"(ez), "v"(ey), "v"(ex));
+  return d2;
 }
 
+// (at most 128 registers: four wavefronts a SIMD, as the 10 KiB of LDS allow)
 template <bool COUNT>
-__global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict__ bxp, float touch2r,
-                                                   const float4* __restrict__ pos,
-                                                   const float4* __restrict__ vel, const float* __restrict__ mat,
-                                                   const float* __restrict__ q, const uint32_t* __restrict__ seats,
-                                                   const float4* __restrict__ force0, const float4* __restrict__ mono0,
-                                                   const float4* __restrict__ force1, const float4* __restrict__ mono1,
-                                                   const float* __restrict__ ppSlot,
-                                                   const float4* __restrict__ ppForce, float4* oPos, float4* oVel,
-                                                   float4* oAux, float* oQ, SlotsOut cnt) {
-  __shared__ int cand[CAND_CAP + 1][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts;
-                                                  // the last row takes the writes past the list's end
+__global__ void __launch_bounds__(SIM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restrict__ pos,
+      const float4* __restrict__ vel, const float* __restrict__ mat, const float* __restrict__ q,
+      const uint32_t* __restrict__ seats, const float4* __restrict__ force0, const float4* __restrict__ mono0,
+      const float4* __restrict__ force1, const float4* __restrict__ mono1, const float* __restrict__ ppSlot,
+      const float4* __restrict__ ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, SlotsOut cnt) {
+  __shared__ int cand[CAND_ROWS][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts
   int lane = threadIdx.x;
   int me = (u.order ? (int)blockIdx.x : xcd_block()) * blockDim.x + lane;
   if (me >= u.N) return;
@@ -925,9 +931,10 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   float c0x = (float)cx * u.cell, c0y = (float)cy * u.cell, c0z = (float)cz * u.cell;
   // A: candidates in order.  A z-plane's nine cells have their seat loads
   // issued together (each under the cell's cull test), then are walked in
-  // order.  Every seat's index is written at the list's end and kept by
+  // order.  Every seat's word is written at the list's end and kept by
   // counting it when it passes; a wavefront leaves a cell at the first seat
-  // that is empty for all its lanes.
+  // that is empty for all its lanes (seats fill from the first, so an empty
+  // seat ends its cell).
   const __amdgpu_buffer_rsrc_t seatR = buf(seats, (uint32_t)(u.grid.cells + 1) * (SLOTS * 4));
   const __amdgpu_buffer_rsrc_t posR = buf(pos, (uint32_t)u.N * 16), velR = buf(vel, (uint32_t)u.N * 16),
                                matR = buf(mat, (uint32_t)u.N * 4);
@@ -938,7 +945,11 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
             pqz = min(max((int)rintf((pz - c0z) * qScale), -1024), 1024);
   const int B = u.grid.IB;
   const uint32_t idxMask = (1u << B) - 1u;
-  int n = 0;
+  // the list's next entry as a byte offset into the LDS array (row stride SIM_BLOCK ints), held at the row
+  // past the cap after each cell
+  const uint32_t rowB = SIM_BLOCK * 4u, wa0 = (uint32_t)lane * 4u, waEnd = (CAND_CAP + 1) * rowB + wa0;
+  char* const candB = reinterpret_cast<char*>(&cand[0][0]);
+  uint32_t wa = wa0;
   for (int dz = -1; dz <= 1; dz++) {
     uint4 sa[9], sb[9];
     bool ok[9];
@@ -954,25 +965,28 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
       sa[i] = bload_u4(seatR, off);
       sb[i] = bload_u4(seatR, off + 16);
     }
-    const int bz = SEAT_Q * dz + 16 - pqz;
+    int bz = SEAT_Q * dz + 16 - pqz;
+    asm volatile("" : "+v"(bz));   // held in a register: one shift-add per seat, not two
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      if (__ballot(ok[i]) == 0ull) continue;
+      const unsigned long long okm = __ballot(ok[i]);
+      if (okm == 0ull) continue;
       const int bx = SEAT_Q * (i % 3 - 1) + 16 - pqx, by = SEAT_Q * (i / 3 - 1) + 16 - pqy;
       uint32_t sv8[8] = { sa[i].x, sb[i].x, sa[i].y, sb[i].y, sa[i].z, sb[i].z, sa[i].w, sb[i].w };
-      bool open = ok[i];
 #pragma unroll
       for (int k = 0; k < SLOTS; k++) {
         uint32_t sv = sv8[k];
-        open = open & ((int)sv >= 0);
-        if (__ballot(open) == 0ull) break;
-        int j = (int)(sv & idxMask);
-        bool pass = open & (j != me) & (seat_d2(sv, B, bx, by, bz) <= u.reachQ2);
-        cand[min(n, CAND_CAP)][lane] = j;
-        n += (int)pass;
+        if ((__ballot((int)sv >= 0) & okm) == 0ull) break;
+        const bool open = ok[i] & ((int)sv >= 0);
+        // (the particle's own seat passes here and is dropped by the link test below)
+        const bool pass = open & (seat_d2(sv, B, bx, by, bz) <= u.reachQ2);
+        *reinterpret_cast<uint32_t*>(candB + wa) = sv;   // the seat word; its index is taken below
+        wa += pass ? rowB : 0u;
       }
+      wa = min(wa, waEnd);
     }
   }
+  const int n = (int)((wa - wa0) / rowB);
   const bool over = n > CAND_CAP;
   // the particle's own velocity, temperature and mass, needed from the contacts on
   float4 V = vel[me];
@@ -980,34 +994,42 @@ __global__ void __launch_bounds__(SIM_BLOCK) k_sim(SimU u, const Box* __restrict
   float ci = ri * (mod4(P.w) < 0.5f ? u.cpIron : 1.0f);
   float vx = V.x, vy = V.y, vz = V.z, Ti = V.w;
   if (!over) {
-    // B: linked pairs counted, touching ones kept (in place, in order); positions fetched eight at a time
+    // B: linked pairs counted, touching ones kept (in place, in order); positions fetched eight at a time.
+    // Branch-free: the eight rows are read at once, rows past the list stand for the particle itself (which
+    // the link test drops), and each entry is written at the kept list's end, which only a kept one advances
+    // (that row is never above the rows already read).
     int nt = 0;
     for (int t0 = 0; t0 < n; t0 += 8) {
       float4 Pj[8];
       int jj[8];
+      uint32_t w[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) w[k] = (uint32_t)cand[t0 + k][lane];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        jj[k] = t0 + k < n ? cand[t0 + k][lane] : me;
+        jj[k] = t0 + k < n ? (int)(w[k] & idxMask) : me;
         Pj[k] = bload_f4(posR, (uint32_t)jj[k] << 4);
       }
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        if (t0 + k >= n) break;
         float ddx = Pj[k].x - px, ddy = Pj[k].y - py, ddz = Pj[k].z - pz;
         float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
-        if (r2 < u.link2) {
-          acc.nb += 1.0f;
-          if (r2 < touch2r) cand[nt++][lane] = jj[k];
-        }
+        const bool linked = (r2 < u.link2) & (jj[k] != me);
+        acc.nb += linked ? 1.0f : 0.0f;
+        cand[nt][lane] = jj[k];
+        nt += (int)(linked & (r2 < touch2r));
       }
     }
     // C: the contacts, in order; their data fetched four at a time
     for (int t0 = 0; t0 < nt; t0 += 4) {
       float4 Pj[4], Vj[4];
       float rj[4];
+      uint32_t w[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) w[k] = (uint32_t)cand[t0 + k][lane];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        uint32_t j = (uint32_t)(t0 + k < nt ? cand[t0 + k][lane] : me);
+        uint32_t j = t0 + k < nt ? w[k] : (uint32_t)me;
         Pj[k] = bload_f4(posR, j << 4); Vj[k] = bload_f4(velR, j << 4); rj[k] = bload_f1(matR, j << 2);
       }
 #pragma unroll
