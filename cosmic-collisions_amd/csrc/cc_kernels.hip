@@ -209,32 +209,36 @@ __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ 
 // writes what the other reads.
 __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN,
                                                     const int* __restrict__ occPrev, const int* __restrict__ segPrev,
-                                                    int N, const int* __restrict__ cnt, int* cntPrev,
+                                                    int N, const int* __restrict__ cnt, int* __restrict__ cntPrev,
                                                     const uint32_t* __restrict__ bucket, const int2* __restrict__ ovf,
-                                                    const int* ovfN, int* ovfNOther, uint32_t idxMask,
-                                                    uint32_t* seats) {
+                                                    const int* __restrict__ ovfN, int* __restrict__ ovfNOther,
+                                                    uint32_t idxMask, uint32_t* __restrict__ seats) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) *ovfNOther = 0;   // the overflow counter of the next rebuild
   if (t >= N) return;
-  if ((t & 63) < segPrev[t >> 6]) {
-    int h = occPrev[t];
-    if (cnt[h] == 0) {
+  // The loads that wait on nothing but each other, issued before any store: the segment lengths, the
+  // cells, their counters and the first four arrivals (an occupied cell has at least one).
+  const int sp = segPrev[t >> 6], sn = segN[t >> 6];
+  const bool prev = (t & 63) < sp, cur = (t & 63) < sn;
+  const int hp = prev ? occPrev[t] : 0, h = cur ? occ[t] : 0;
+  const int np = prev ? cnt[hp] : 1, n = cur ? cnt[h] : 0;
+  const uint4* b4 = reinterpret_cast<const uint4*>(bucket + (size_t)h * BUCKET_CAP);
+  const uint4 v0 = cur ? b4[0] : make_uint4(0u, 0u, 0u, 0u);
+  if (prev) {
+    if (np == 0) {
       uint4 e = make_uint4(EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT, EMPTY_SEAT);
-      uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)h * SLOTS);
+      uint4* sv = reinterpret_cast<uint4*>(seats + (size_t)hp * SLOTS);
       sv[0] = e; sv[1] = e;
     }
-    cntPrev[h] = 0;
+    cntPrev[hp] = 0;
   }
-  if ((t & 63) >= segN[t >> 6]) return;
-  int h = occ[t];
-  int n = cnt[h];
-  int m = n < BUCKET_CAP ? n : BUCKET_CAP;
+  if (!cur) return;
+  const int m = n < BUCKET_CAP ? n : BUCKET_CAP;
   uint32_t wd[BUCKET_CAP];   // the arrivals' seat words
   int ix[BUCKET_CAP];        // and their particle indices (past the arrivals: above any index)
-  const uint4* b4 = reinterpret_cast<const uint4*>(bucket + (size_t)h * BUCKET_CAP);
 #pragma unroll
   for (int q = 0; q < BUCKET_CAP / 4; q++) {
-    uint4 v = (q * 4 < m) ? b4[q] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 v = q == 0 ? v0 : (q * 4 < m) ? b4[q] : make_uint4(0u, 0u, 0u, 0u);
     wd[q * 4] = v.x; wd[q * 4 + 1] = v.y; wd[q * 4 + 2] = v.z; wd[q * 4 + 3] = v.w;
 #pragma unroll
     for (int u = 0; u < 4; u++) ix[q * 4 + u] = q * 4 + u < m ? (int)(wd[q * 4 + u] & idxMask) : 0x7fffffff;
