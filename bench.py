@@ -181,6 +181,35 @@ def runto_leg(steps, every=360):
     return out
 
 
+def configs_leg(theia_steps, crater_steps):
+    """The other plain configurations on the GPU, each as configured: the
+    theia-like impact (16,384 particles) through the JavaScript engine from
+    first touch with an analysis every 360 steps (js/run_timed.js), and the
+    Chicxulub crater patch at 524,288 grains through the crater engine --
+    build, settle, approach, coupling, then the full steps (js/crater_timed.js)."""
+    out = {}
+    if theia_steps > 0:
+        cmd = ["node", os.path.join(PKG, "js", "run_timed.js"), json.dumps(runs.THEIA), str(theia_steps), "360", "sync"]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("run_timed.js failed (exit %d):\n%s" % (r.returncode, r.stderr[-3000:]))
+        t = json.loads(r.stdout.strip().splitlines()[-1])
+        out["theia"] = {k: t[k] for k in ("particles", "steps", "every", "looks", "with_analysis", "steps_only",
+                                          "ms_per_step", "ms_per_look", "books")}
+        out["theia"]["workload"] = "plain theia-like impact: 16,384 particles (0.89 + 0.11 M_earth, 45 deg, 1.1 v_esc) " \
+                                   "from first touch, an analysis every 360 steps"
+    if crater_steps > 0:
+        opt = {"imp": 14, "vimp": 20, "angle": 60, "sed": 3, "dens": 2500, "rest": 0.25, "nIdx": 4, "N": 524288}
+        cmd = ["node", os.path.join(PKG, "js", "crater_timed.js"), json.dumps(opt), str(crater_steps)]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("crater_timed.js failed (exit %d):\n%s" % (r.returncode, r.stderr[-3000:]))
+        out["crater"] = json.loads(r.stdout.strip().splitlines()[-1])
+        out["crater"]["workload"] = "Chicxulub crater patch, constant g: 524,288 grains (carbonate over basement, a 14 " \
+                                    "km impactor at 20 km/s and 60 deg, the Z-model coupling), full steps after touchdown"
+    return out
+
+
 def build_leg():
     """Body setup on the host: the twins build with its bodies relaxed in
     JavaScript and by the library, and the batched sweep's 64 instances'
@@ -212,6 +241,8 @@ def main():
     ap.add_argument("--learned", type=int, default=1, help="the learned far field leg (0: skip it)")
     ap.add_argument("--build", type=int, default=1, help="the body setup leg on the host (0: skip it)")
     ap.add_argument("--tune", type=int, default=1, help="cc_autotune the plain leg's launch shapes first (0: defaults)")
+    ap.add_argument("--theia-steps", type=int, default=1000, help="the theia-like leg's steps (0: skip it)")
+    ap.add_argument("--crater-steps", type=int, default=20000, help="the 524,288-grain crater leg's steps (0: skip it)")
     ap.add_argument("--learned-steps", type=int, default=1000)
     ap.add_argument("--ff-cycles", type=int, default=48)
     ap.add_argument("--ff-iters", type=int, default=3000)
@@ -280,6 +311,8 @@ def main():
     R.close()
     if args.runto > 0:
         out["engine_runto"] = runto_leg(args.runto)
+    if args.theia_steps > 0 or args.crater_steps > 0:
+        out["configs"] = configs_leg(args.theia_steps, args.crater_steps)
     if args.batch > 0:
         out["batched"] = batch_leg(args.batch_dir, args.batch, args.batch_steps, 24, args.reuse_state)
     if args.build > 0:

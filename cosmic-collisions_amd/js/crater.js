@@ -161,6 +161,13 @@ function createCraterEngine(opts) {
     addon.step(ctx, args(settle));
     sim.stepNo++;
   }
+  // n full steps at once (cc_step_n: the schedule of step(false) n times, each 8-step cycle one graph launch)
+  function stepMany(n) {
+    addon.setStepNo(ctx, sim.stepNo);
+    const ms = addon.stepN(ctx, n, args(false), null);
+    sim.stepNo += n;
+    return ms;
+  }
   function settleSome(n) {
     n = Math.min(n, sim.settleLeft);
     for (let i = 0; i < n; i++) step(true);
@@ -206,7 +213,7 @@ function createCraterEngine(opts) {
     return report;
   }
   function destroy() { if (ctx) { addon.destroy(ctx); ctx = null; } }
-  return { build, step, settleSome, approachStep, couple, readState, analyze, destroy, get sim() { return sim; },
+  return { build, step, stepMany, settleSome, approachStep, couple, readState, analyze, destroy, get sim() { return sim; },
            get ctx() { return ctx; }, get report() { return report; }, PRESETS, DEFAULTS };
 }
 

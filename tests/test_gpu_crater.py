@@ -170,6 +170,41 @@ console.log(JSON.stringify({ frames: frames, phase: sim.phase, N: sim.N, rep, m:
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_crater_step_many_equals_single_steps(fx, tmp_path):
+    """The crater engine's stepMany (cc_step_n, one graph launch per 8-step
+    cycle) against the same full steps taken one call at a time: two engines
+    built alike, through touchdown and the coupling, then 21 steps each way
+    (starting and ending off a cycle boundary); the states bit-identical."""
+    script = tmp_path / "many.js"
+    script.write_text("""
+const cr = require('%s/crater.js'); const fs = require('fs');
+const run = (many) => {
+  const eng = cr.createCraterEngine();
+  const sim = eng.build(%s);
+  eng.settleSome(2);
+  while (sim.phase === 'settle') eng.settleSome(64);
+  while (sim.phase !== 'full') eng.approachStep();
+  eng.step(false);
+  if (many) { eng.stepMany(13); eng.stepMany(8); } else for (let i = 0; i < 21; i++) eng.step(false);
+  const st = eng.readState();
+  for (const k of ['pos', 'vel', 'aux', 'q']) fs.writeFileSync('%s/' + (many ? 'm_' : 's_') + k, Buffer.from(st[k].buffer));
+  const n = sim.stepNo;
+  eng.destroy();
+  return n;
+};
+console.log(JSON.stringify({ single: run(false), many: run(true) }));
+""" % (JS, json.dumps(json.loads(str(fx["opt"]))), tmp_path))
+    print("crater engine stepMany vs step (node)", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "180", "node", str(script)], capture_output=True, text=True, cwd=JS)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["single"] == out["many"] == 22
+    for k in ("pos", "vel", "aux", "q"):
+        a, b = (np.fromfile(tmp_path / (w + k), np.float32) for w in ("s_", "m_"))
+        assert np.array_equal(a, b), k
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
 def test_crater_524k(tmp_path):
     """The crater at its configuration's stated size, 524,288 grains: past the
     reference's 262,144, so the seat words carry a 19-bit index (the place
