@@ -40,6 +40,7 @@ struct cc_ctx {
   int* ppIndex = nullptr;
   float4* ppList = nullptr;
   float* ppBox = nullptr;
+  float4* ppBounds = nullptr;   // the pairwise pass's tile boxes
   float4* ppForce = nullptr;
   float* ppSlot = nullptr;
   float2* ppTab = nullptr;
@@ -264,6 +265,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       ALLOC(m.occ, (size_t)SCRATCH * 4); ALLOC(m.occN, 4); ALLOC(m.coarseDone, 4);
     }
     ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
+    ALLOC(c->ppBounds, PP_TILES_MAX * 32);
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
     ALLOC(c->diag, N * 16);
     ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
@@ -521,7 +523,7 @@ int cc_pp(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   if (!c->ppCount) return 0;
   launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
-            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s);
+            (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppBounds, c->ppForce, c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -584,7 +586,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
     HIPCHK(c, hipEventRecord(c->ppFork, c->s));
     HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
     launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
-              (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppForce, c->s2);
+              (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppBounds, c->ppForce, c->s2);
     LAUNCHCHK(c);
     HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
   }
@@ -1271,13 +1273,14 @@ int cc_batch_last_ms(cc_batch* b, double* ms) {
 // taken once per device with that device current and kept in g_scratch.
 // Every call on a GPU shares that GPU's copy: calls that overlap on one GPU
 // (two streams, two threads) race.  Each call writes every scratch byte it
-// reads (the boxes, all 8192 seats) before reading it, so nothing needs
-// clearing between calls.
+// reads (the boxes, all 8192 seats, the tile boxes of its own list's length)
+// before reading it, so nothing needs clearing between calls.
 namespace {
 constexpr size_t NA_BOX = 0;                                  // Box (32 B), 256-byte aligned
 constexpr size_t NA_LIST = 256;                               // float4[PP_CAP]
 constexpr size_t NA_SEATBOX = NA_LIST + (size_t)PP_CAP * 16;  // float[PP_CAP]
-constexpr size_t NA_BYTES = NA_SEATBOX + (size_t)PP_CAP * 4;
+constexpr size_t NA_BOUNDS = NA_SEATBOX + (size_t)PP_CAP * 4;  // float4[2 * PP_TILES_MAX]
+constexpr size_t NA_BYTES = NA_BOUNDS + (size_t)PP_TILES_MAX * 32;
 constexpr int NA_DEVICES = 64;
 std::mutex g_scratchLock;
 unsigned char* g_scratch[NA_DEVICES];   // each device's address of `scratch` (null until looked up)
@@ -1326,7 +1329,8 @@ extern "C" int cc_pp_noalloc(const float* pos, const float* mat, int N, const do
   launch_put_box(b, bx, s);
   launch_pp(reinterpret_cast<const float4*>(pos), mat, N, bx, index, count, (float)m, (float)(5.0 * FH), (float)(a * a),
             reinterpret_cast<const float2*>(tab), reinterpret_cast<float4*>(w + NA_LIST),
-            reinterpret_cast<float*>(w + NA_SEATBOX), reinterpret_cast<float4*>(force), s);
+            reinterpret_cast<float*>(w + NA_SEATBOX), reinterpret_cast<float4*>(w + NA_BOUNDS),
+            reinterpret_cast<float4*>(force), s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) FAIL((cc_ctx*)nullptr, CC_EDEVICE, "cc_pp_noalloc launch: %s", hipGetErrorString(e));
   return 0;

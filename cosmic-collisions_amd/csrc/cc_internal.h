@@ -32,6 +32,8 @@ constexpr int SLOTS = 8;             // seats per contact cell
 constexpr int SEAT_Q = 480;          // the seat prefilter's lattice: 1/480 of a cell (32 per fifteenth)
 constexpr int BUCKET_CAP = 16;       // arrivals kept per cell before the overflow list
 constexpr int PP_CAP = 8192;
+constexpr int PP_TILE = 48;          // entries per tile of a slice in the pairwise pass
+constexpr int PP_TILES_MAX = 4 * ((PP_CAP / 4 + PP_TILE - 1) / PP_TILE);   // tiles of the four slices at the cap
 constexpr int FINE_TEXELS = 513 * 512;
 constexpr int SCRATCH = 512 * 512;   // texel (0, 512)
 constexpr uint32_t EMPTY_SEAT = 0xBF800000u;
@@ -146,9 +148,11 @@ void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, co
 void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
-// the pairwise pass: the loose list's gather, then each seat's four slices (force[slice * PP_CAP + seat])
+// the pairwise pass: the loose list's gather, its tiles' boxes (bounds: 2 x PP_TILES_MAX), then each seat's
+// four slices (force[slice * PP_CAP + seat])
 void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
-               float cut, float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s);
+               float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds, float4* force,
+               hipStream_t s);
 // count: the next rebuild's count taken on the positions this pass writes (null: none)
 void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,

@@ -676,16 +676,48 @@ __global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ po
 //      running sum, four entries per LDS read.
 // Every pair is evaluated once for each end, as the reference evaluates it.
 constexpr int PP_ROWS = 8;
-constexpr int PP_COLS = 48;
+constexpr int PP_COLS = PP_TILE;
 constexpr int PP_CHAIN = PP_COLS + 4;   // floats per (component, seat) row of the term tile: 16-byte reads of
                                         // eight consecutive rows fall in distinct banks
 // (8 seats x 48 entries keep a work-group's LDS under 10 KiB: sixteen of them, a CU's limit, fit at once)
 
-__device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj, float cut, float cut2, float a2,
+// Each tile's bounding box (its entries with mass; an empty tile's box is empty), so a wavefront can pass
+// over a tile whose box lies beyond the cut from its seats' box: every distance is at least the boxes'
+// gap, and FP32 rounding keeps that order (the differences, squares and sums are monotone in their
+// non-negative operands), so none of the tile's pairs is within the cut and each of its terms would be
+// zero.  A running sum that starts at +0 never becomes -0, so leaving out adds of +0 or -0 changes nothing.
+__global__ void __launch_bounds__(64) k_pp_bounds(const float4* __restrict__ list, int count, float4* bounds) {
+  const int lane = threadIdx.x, share = (count + 3) >> 2, tps = (share + PP_COLS - 1) / PP_COLS;
+  const int slice = blockIdx.x / tps, t = blockIdx.x % tps;
+  const int j0 = slice * share + t * PP_COLS, j1 = min(min(j0 + PP_COLS, (slice + 1) * share), count);
+  const int j = j0 + lane;
+  float4 q = (lane < PP_COLS && j < j1) ? list[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool has = q.w != 0.0f;
+  float lx = has ? q.x : INFINITY, ly = has ? q.y : INFINITY, lz = has ? q.z : INFINITY;
+  float hx = has ? q.x : -INFINITY, hy = has ? q.y : -INFINITY, hz = has ? q.z : -INFINITY;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, o)); ly = fminf(ly, __shfl_xor(ly, o)); lz = fminf(lz, __shfl_xor(lz, o));
+    hx = fmaxf(hx, __shfl_xor(hx, o)); hy = fmaxf(hy, __shfl_xor(hy, o)); hz = fmaxf(hz, __shfl_xor(hz, o));
+  }
+  if (lane == 0) {
+    bounds[2 * blockIdx.x] = make_float4(lx, ly, lz, 0.0f);
+    bounds[2 * blockIdx.x + 1] = make_float4(hx, hy, hz, 0.0f);
+  }
+}
+
+// the squared gap between two boxes along the axes, in the order dot3 takes a distance
+__device__ __forceinline__ float box_gap2(float4 alo, float4 ahi, float4 blo, float4 bhi) {
+  const float gx = fmaxf(fmaxf(blo.x - ahi.x, alo.x - bhi.x), 0.0f);
+  const float gy = fmaxf(fmaxf(blo.y - ahi.y, alo.y - bhi.y), 0.0f);
+  const float gz = fmaxf(fmaxf(blo.z - ahi.z, alo.z - bhi.z), 0.0f);
+  return dot3(gx, gy, gz, gx, gy, gz);
+}
+
+__device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj, float cut, float a2,
                                           float a, float ya, float yCut, float yTap, float tapD, const float2* st) {
   float d0 = Q.x - P.x, d1 = Q.y - P.y, d2 = Q.z - P.z;
-  float r2 = dot3(d0, d1, d2, d0, d1, d2);
-  if (r2 >= cut2) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float r2 = dot3(d0, d1, d2, d0, d1, d2);   // below cut2: only such pairs are listed
   float r = qsqrt(r2);
   float x = gclamp(qdiv_by(r, cut, yCut) * 64.0f - 0.5f, 0.0f, 63.0f);
   int k = (int)x, k1 = min(k + 1, 63);
@@ -707,8 +739,8 @@ __device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj
 }
 
 __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list, const float* __restrict__ box,
-                                                 int count, float gm, float cut, float a2,
-                                                 const float2* __restrict__ tab, float4* force) {
+                                                 const float4* __restrict__ bounds, int count, float gm, float cut,
+                                                 float a2, const float2* __restrict__ tab, float4* force) {
   __shared__ __align__(16) float term[4 * PP_ROWS * PP_CHAIN];
   __shared__ float4 lP[PP_ROWS], lQ[PP_COLS];
   __shared__ float bP[PP_ROWS], bQ[PP_COLS];
@@ -736,18 +768,34 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
   const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD), ya = qdiv(1.0f, a);
   const float4* chain = reinterpret_cast<const float4*>(term + lane * PP_CHAIN);   // step 3's row (lane = sc * PP_ROWS + sr)
   float run = 0.0f;
+  // the seats' box (the live ones; lanes differing in their low three bits hold the eight seats), the same
+  // in every lane, and this slice's tile boxes
+  float4 rlo = make_float4(live ? P.x : INFINITY, live ? P.y : INFINITY, live ? P.z : INFINITY, 0.0f);
+  float4 rhi = make_float4(live ? P.x : -INFINITY, live ? P.y : -INFINITY, live ? P.z : -INFINITY, 0.0f);
+#pragma unroll
+  for (int o = 1; o < PP_ROWS; o <<= 1) {
+    rlo.x = fminf(rlo.x, __shfl_xor(rlo.x, o)); rlo.y = fminf(rlo.y, __shfl_xor(rlo.y, o));
+    rlo.z = fminf(rlo.z, __shfl_xor(rlo.z, o));
+    rhi.x = fmaxf(rhi.x, __shfl_xor(rhi.x, o)); rhi.y = fmaxf(rhi.y, __shfl_xor(rhi.y, o));
+    rhi.z = fmaxf(rhi.z, __shfl_xor(rhi.z, o));
+  }
+  const float4* tb = bounds + 2 * slice * ((share + PP_COLS - 1) / PP_COLS);
   // the tile's entries, loaded one tile ahead (lanes below PP_COLS)
   float4 nQ = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float nB = -1.0f;
   if (lane < PP_COLS && j0 + lane < j1) { nQ = list[j0 + lane]; nB = box[j0 + lane]; }
-  for (int jt = j0; jt < j1; jt += PP_COLS) {
+  for (int jt = j0, tt = 0; jt < j1; jt += PP_COLS, tt++) {
+    const bool beyond = box_gap2(rlo, rhi, tb[2 * tt], tb[2 * tt + 1]) >= cut2;   // every lane alike
     if (lane < PP_COLS) {
-      lQ[lane] = nQ;
-      bQ[lane] = nB;
+      if (!beyond) {
+        lQ[lane] = nQ;
+        bQ[lane] = nB;
+      }
       const int j = jt + PP_COLS + lane;
       nQ = j < j1 ? list[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       nB = j < j1 ? box[j] : -1.0f;
     }
+    if (beyond) continue;
     float4* t4 = reinterpret_cast<float4*>(term);
 #pragma unroll
     for (int k = lane; k < PP_ROWS * PP_CHAIN; k += 64) t4[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -768,7 +816,7 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
     __syncthreads();
     for (int t = lane; t < n; t += 64) {
       const int e = todo[t], r = e & (PP_ROWS - 1), jl = e >> 3;
-      const float4 v = pp_term(lP[r], bP[r], lQ[jl], bQ[jl], cut, cut2, a2, a, ya, yCut, yTap, tapD, st);
+      const float4 v = pp_term(lP[r], bP[r], lQ[jl], bQ[jl], cut, a2, a, ya, yCut, yTap, tapD, st);
       float* w = term + r * PP_CHAIN + jl;
       w[0] = v.x;
       w[PP_ROWS * PP_CHAIN] = v.y;
@@ -792,9 +840,13 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
 }
 
 void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
-               float cut, float a2, const float2* tab, float4* list, float* box, float4* force, hipStream_t s) {
+               float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds, float4* force,
+               hipStream_t s) {
   hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, N, b, index, count, list, box);
-  hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / PP_ROWS, 4), dim3(64), 0, s, list, box, count, gm, cut, a2, tab, force);
+  const int tiles = 4 * (((count + 3) / 4 + PP_COLS - 1) / PP_COLS);
+  if (tiles > 0) hipLaunchKernelGGL(k_pp_bounds, dim3(tiles), dim3(64), 0, s, list, count, bounds);
+  hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / PP_ROWS, 4), dim3(64), 0, s, list, box, bounds, count, gm, cut, a2, tab,
+                     force);
 }
 
 __global__ void k_pp_slot(int N, const int* __restrict__ index, int count, float* slot) {
