@@ -909,22 +909,37 @@ struct Contact {
   float pushx, pushy, pushz, nb, lost, flow;
 };
 
-// one candidate pair's share, exactly as the step pass computes it
+// The pair's dashpot and conduction factors depend only on the two masses and
+// heat capacities; for a partner of the particle's own kind they are the
+// particle's own (the same operations on the same operands), taken once.
+struct PairSelf {
+  float cii, hii;
+};
+__device__ __forceinline__ PairSelf pair_self(const SimU& u, float ri, float ci) {
+  return { u.c * qsqrt(qdiv(2.0f * ri * ri, ri + ri)), qdiv(2.0f * ci * ci, ci + ci) };
+}
+
+// one candidate pair's share, exactly as the step pass computes it; the
+// factors are computed when any lane of the wavefront has a partner of
+// another kind, and taken from `self` otherwise
 __device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, float pz, float vx, float vy, float vz,
-                                             float Ti, float ri, float ci, float4 Pj, float4 Vj, float rj,
-                                             Contact& a) {
+                                             float Ti, float ri, float ci, const PairSelf& self, float4 Pj, float4 Vj,
+                                             float rj, Contact& a) {
   float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
   float r = qsqrt(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
   float rr = gmax(r, 1e-6f);
   float nx = qdiv(ddx, rr), ny = qdiv(ddy, rr), nz = qdiv(ddz, rr);
   float ov = u.touch - r;
-  float cij = u.c * qsqrt(qdiv(2.0f * ri * rj, ri + rj));
+  float cij = self.cii;
+  if (__any(rj != ri)) cij = u.c * qsqrt(qdiv(2.0f * ri * rj, ri + rj));
   float vn = dot3(Vj.x - vx, Vj.y - vy, Vj.z - vz, nx, ny, nz);
   float fs = u.k * ov - cij * vn, fn = gmax(fs, 0.0f), on = fs < 0.0f ? 0.0f : 1.0f;
   a.pushx -= nx * fn; a.pushy -= ny * fn; a.pushz -= nz * fn;
   a.lost += cij * vn * vn * on + u.k * ov * vn * (1.0f - on);
   float cj = rj * (mod4(Pj.w) < 0.5f ? u.cpIron : 1.0f);
-  a.flow += (Vj.w - Ti) * qdiv(2.0f * ci * cj, ci + cj);
+  float hij = self.hii;
+  if (__any(cj != ci)) hij = qdiv(2.0f * ci * cj, ci + cj);
+  a.flow += (Vj.w - Ti) * hij;
 }
 
 // The seat prefilter (seatPos and the reach test, src/glsl-sim.js:109,
@@ -1049,6 +1064,7 @@ k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restri
   float ri = mat[me];
   float ci = ri * (mod4(P.w) < 0.5f ? u.cpIron : 1.0f);
   float vx = V.x, vy = V.y, vz = V.z, Ti = V.w;
+  const PairSelf self = pair_self(u, ri, ci);
   if (!over) {
     // B: linked pairs counted, touching ones kept (in place, in order); positions fetched eight at a time.
     // Branch-free: the eight rows are read at once, rows past the list stand for the particle itself (which
@@ -1091,7 +1107,7 @@ k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restri
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         if (t0 + k >= nt) break;
-        contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, Pj[k], Vj[k], rj[k], acc);
+        contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, self, Pj[k], Vj[k], rj[k], acc);
       }
     }
   } else {
@@ -1112,7 +1128,7 @@ k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restri
         float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
         float r2 = dot3(ddx, ddy, ddz, ddx, ddy, ddz);
         if (r2 < u.link2) {
-          if (r2 < touch2r) contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, Pj, vel[j], mat[j], acc);
+          if (r2 < touch2r) contact_pair(u, px, py, pz, vx, vy, vz, Ti, ri, ci, self, Pj, vel[j], mat[j], acc);
           acc.nb += 1.0f;
         }
       }
