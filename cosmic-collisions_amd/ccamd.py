@@ -41,7 +41,7 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
-           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_crater_stats", "cc_pp_noalloc", "cc_analyse_begin", "cc_analyse_end"]
+           "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_set_visit", "cc_crater_stats", "cc_pp_noalloc", "cc_analyse_begin", "cc_analyse_end"]
 
 
 class CraterArgs(C.Structure):
@@ -185,6 +185,7 @@ def lib():
             "cc_relax_piles": ([C.POINTER(Pile), C.c_int, C.POINTER(PileReport)], C.c_int),
             "cc_autotune": ([P, C.c_int, C.POINTER(TuneReport)], C.c_int),
             "cc_set_tuning": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
+            "cc_set_visit": ([P, C.c_int], C.c_int),
             "cc_crater_stats": ([C.c_int, C.c_double, fp, fp, fp, C.POINTER(CraterArgs), C.POINTER(CraterReport)],
                                 C.c_int),
             "cc_bodies_last_error": ([], C.c_char_p),
@@ -450,6 +451,10 @@ class Context:
         self._chk(lib().cc_autotune(self.h, int(reps), C.byref(rep)), "cc_autotune")
         return {"cellBlocks": rep.cellBlocks, "simOrder": rep.simOrder, "cellMs": list(rep.cellMs),
                 "simMs": list(rep.simMs)}
+
+    def set_visit(self, every):
+        """The step pass's visit order re-sorted every `every` steps (0: index order); results unchanged."""
+        self._chk(lib().cc_set_visit(self.h, int(every)), "cc_set_visit")
 
     def set_tuning(self, cell_blocks, sim_order, fuse_count=1):
         self._chk(lib().cc_set_tuning(self.h, int(cell_blocks), int(sim_order), int(fuse_count)), "cc_set_tuning")

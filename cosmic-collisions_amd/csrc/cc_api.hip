@@ -56,6 +56,11 @@ struct cc_ctx {
   CycleGraph graphs[4];    // captured step cycles, by what their launches bake in
   int nextGraph = 0;
   int useGraphs = 1;
+  // the step pass's visit order (a permutation that only the caches see), re-sorted at a cycle's start once
+  // visitEvery steps have passed since the last sort (visitEvery 0: index order)
+  int* visit = nullptr;
+  int visitEvery = 256;
+  int visitAt = -(1 << 30);
   // launch shapes that cannot change a result (cc_autotune, cc_set_tuning): the cell pass's
   // work-groups, the step pass's work-group order
   int cellBlocks = 2048;
@@ -194,6 +199,7 @@ static SimU sim_uniforms(const cc_ctx* c, const cc_sim_args* a) {
   for (int k = 0; k < 3; k++) { u.spin0[k] = (float)p.spin0[k]; u.spin1[k] = (float)p.spin1[k]; }
   u.grid = c->grid;
   u.order = c->simOrder;
+  u.visit = c->visitEvery > 0 ? c->visit : nullptr;
   u.gm = (float)p.m;
   u.eps2 = (float)(FH * FH);
   u.field = p.field;
@@ -269,7 +275,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
     ALLOC(c->diag, N * 16);
     ALLOC(c->mkeys, N * 4); ALLOC(c->mslot, N * 4); ALLOC(c->mhist, ((1 << 21) + 1) * 4);
-    ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4);
+    ALLOC(c->mblock, 4096 * 4); ALLOC(c->mperm, N * 4); ALLOC(c->minto, N * 4); ALLOC(c->visit, N * 4);
     ALLOC(c->bpart, 256 * 13 * 8); ALLOC(c->bout, 13 * 8);
     ALLOC(c->boxDev, sizeof(Box)); ALLOC(c->boxIdx, 4);
     ALLOC(c->boxSched, (size_t)SCHED_CAP * sizeof(Box));
@@ -297,6 +303,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     }
     launch_fill_u32(reinterpret_cast<uint32_t*>(c->ppIndex), PP_CAP, 0xFFFFFFFFu, c->s);
     launch_fill_f32(c->ppSlot, N, -1.0f, c->s);
+    launch_iota(c->visit, N, c->s);
     HIPCHK(c, hipMemsetAsync(c->ppForce, 0, 4 * PP_CAP * 16, c->s));
     HIPCHK(c, hipMemsetAsync(c->ppList, 0, PP_CAP * 16, c->s));
     HIPCHK(c, hipMemsetAsync(c->ppTab, 0, 64 * 8, c->s));
@@ -359,6 +366,7 @@ int cc_upload_state(cc_ctx* c, const float* pos, const float* vel, const float* 
   if (mat) HIPCHK(c, hipMemcpyAsync(c->mat, mat, N * 4, hipMemcpyHostToDevice, c->s));
   if (q) HIPCHK(c, hipMemcpyAsync(c->q[c->cur], q, N * 4, hipMemcpyHostToDevice, c->s));
   HIPCHK(c, hipStreamSynchronize(c->s));
+  if (pos) c->visitAt = -(1 << 30);   // new positions: the visit order is re-sorted at the next cycle's start
   return 0;
 }
 
@@ -573,6 +581,16 @@ int cc_sim(cc_ctx* c, const cc_sim_args* a) {
   return sim_pass(c, a);
 }
 
+// the visit order re-sorted when due, at a cycle's start (outside any captured graph: the sort runs on the
+// stream between cycles, and the graphs read the order's buffer at their replay)
+static int visit_due(cc_ctx* c, hipStream_t s) {
+  if (c->visitEvery <= 0 || c->stepNo % c->p.meshEvery != 0 || c->stepNo - c->visitAt < c->visitEvery) return 0;
+  launch_visit(c->pos[c->cur], c->N, (float)(1.0 / c->p.cell), c->mkeys, c->mslot, c->mhist, c->mblock, c->visit, s);
+  LAUNCHCHK(c);
+  c->visitAt = c->stepNo;
+  return 0;
+}
+
 static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   int n = c->stepNo++;
   int r;
@@ -618,6 +636,8 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
 
 int cc_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   if (!c || !a) return CC_EINVAL;
+  int r = visit_due(c, c->s);
+  if (r) return r;
   return one_step(c, settle, kick, a);
 }
 
@@ -638,6 +658,7 @@ static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
     k.push_back(a->bulk0[i]); k.push_back(a->bulk1[i]); k.push_back(a->cen0[i]); k.push_back(a->cen1[i]);
   }
   k.push_back((double)c->ppCount);
+  k.push_back((double)(c->visitEvery > 0));
   k.push_back((double)(c->box.boxB > 0.5f));
   k.push_back((double)c->cur);
   k.push_back((double)c->gb.cur);
@@ -715,6 +736,8 @@ int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nB
       int e = cc_set_boxes(c, L, L + 3, L[6] > 0.5);
       if (e) return e;
     }
+    int e = visit_due(c, c->s);
+    if (e) return e;
     return one_step(c, a->settle, a->kick, a);
   };
   while (i < n && c->stepNo % M != 0) { if ((r = eager())) return r; i++; }
@@ -728,6 +751,7 @@ int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nB
     LAUNCHCHK(c);
     for (int k = 0; k < chunk; k++) {
       c->box = c->boxHost[k];
+      if ((r = visit_due(c, c->s))) return r;
       CycleGraph* g = nullptr;
       if ((r = cycle_graph(c, a, &g))) return r;
       HIPCHK(c, hipGraphLaunch(g->exec, c->s));
@@ -895,6 +919,19 @@ int cc_ff_record(cc_ff* f, cc_ctx* c, int box) {
   return 0;
 }
 
+int cc_set_visit(cc_ctx* c, int every) {
+  if (!c || every < 0) return CC_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->s));
+  c->visitEvery = every;
+  c->visitAt = -(1 << 30);
+  if (every == 0) {   // index order again
+    launch_iota(c->visit, c->N, c->s);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+  }
+  return 0;
+}
+
 int cc_set_graphs(cc_ctx* c, int on) {
   if (!c) return CC_EINVAL;
   c->useGraphs = on ? 1 : 0;
@@ -972,6 +1009,7 @@ int cc_morton(cc_ctx* c, const double core[3], int32_t* perm, int32_t* into) {
   LAUNCHCHK(c);
   std::swap(c->mat, c->matTmp);
   c->cur = d;
+  c->visitAt = -(1 << 30);   // the visit order names particles by their old places: re-sorted at the next cycle
   int r = cc_slots(c);   // applyPerm rebuilds the seats: the old ones name particles by their old places
   if (r) return r;
   if (perm) HIPCHK(c, hipMemcpyAsync(perm, c->mperm, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->s));
@@ -1191,7 +1229,8 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
       int e = cc_set_boxes(c, q, q + 3, q[6] > 0.5);
       if (e) { b->err = c->err; return e; }
     }
-    int e = one_step(c, args[i].settle, args[i].kick, &args[i]);
+    int e = visit_due(c, c->s);
+    if (!e) e = one_step(c, args[i].settle, args[i].kick, &args[i]);
     if (e) b->err = c->err;
     return e;
   };
@@ -1227,7 +1266,10 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
       launch_fill_u32(reinterpret_cast<uint32_t*>(c->boxIdx), 1, 0u, b->s);
     }
     for (int k = 0; k < chunk; k++) {
-      for (size_t i = 0; i < m; i++) b->ctx[i]->box = b->ctx[i]->boxHost[k];
+      for (size_t i = 0; i < m; i++) {
+        b->ctx[i]->box = b->ctx[i]->boxHost[k];
+        if ((r = visit_due(b->ctx[i], b->s))) { b->err = b->ctx[i]->err; return r; }
+      }
       if ((r = batch_graph(b, args))) return r;
       BCHK(b, hipGraphLaunch(b->exec, b->s));
       for (size_t i = 0; i < m; i++) {

@@ -76,6 +76,7 @@ struct SimU {           // the step pass's uniforms, already FP32
   int reachQ2;          // the seat prefilter's bound in 1/480-cell units, squared (see k_sim)
   int order;            // 0: work-group b takes particles (b % 8) * (grid / 8) + b / 8 (consecutive
                         // work-groups' neighbourhoods spread over the XCDs' L2s), 1: particles b
+  const int* visit;     // the order the particles are taken in (a permutation of 0..N-1), or null: index order
 };
 
 struct CoupleU {        // COUPLE_FS's uniforms (chicxulub.html:260-276), FP32
@@ -153,6 +154,11 @@ void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0,
 void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
                float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds, float4* force,
                hipStream_t s);
+// the step pass's visit order: the particles sorted by the Morton code of their contact cell (coordinates
+// modulo 128); keys, slot: N ints, hist: KEYS + 1 ints, blockSums: scan scratch
+void launch_visit(const float4* pos, int N, float invCell, int* keys, int* slot, int* hist, int* blockSums,
+                  int* visit, hipStream_t s);
+void launch_iota(int* a, int n, hipStream_t s);
 // count: the next rebuild's count taken on the positions this pass writes (null: none)
 void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,

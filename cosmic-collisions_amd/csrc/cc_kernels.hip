@@ -993,8 +993,9 @@ k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restri
       const float4* __restrict__ ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, SlotsOut cnt) {
   __shared__ int cand[CAND_ROWS][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts
   int lane = threadIdx.x;
-  int me = (u.order ? (int)blockIdx.x : xcd_block()) * blockDim.x + lane;
-  if (me >= u.N) return;
+  const int at = (u.order ? (int)blockIdx.x : xcd_block()) * blockDim.x + lane;   // the place in the visit order
+  if (at >= u.N) return;
+  const int me = u.visit ? u.visit[at] : at;
   float4 P = pos[me];
   float px = P.x, py = P.y, pz = P.z;
   Contact acc = { 0, 0, 0, 0, 0, 0 };
@@ -1207,10 +1208,11 @@ k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restri
   oVel[me] = make_float4(vx, vy, vz, u.tK * Q / ci);
   oAux[me] = make_float4(gvx, gvy, gvz, nb);
   oQ[me] = Q;
-  if (COUNT) {   // the next rebuild's count, on the position just written (a block is one wavefront)
+  if (COUNT) {   // the next rebuild's count, on the position just written (a block is one wavefront, whose
+                 // places in the visit order make the occupied list's segment)
     int h;
     uint32_t w = seat_word(px, py, pz, u.invCell, u.grid, me, h);
-    count_arrival(me, w, h, cnt);
+    count_arrival(at, w, h, cnt);
   }
 }
 
@@ -1485,6 +1487,46 @@ void launch_morton(const float4* pos, int N, double cx, double cy, double cz, in
   hipLaunchKernelGGL(k_morton_rank, dim3(blocks), dim3(256), 0, s, N, keys, hist, into, perm, slot);
   hipLaunchKernelGGL(k_copy_int, dim3(blocks), dim3(256), 0, s, N, slot, into);
 }
+
+// The step pass's visit order.  Any permutation gives the same results (a
+// particle's sums depend only on the seats, which name particles by index), so
+// it is chosen for the caches alone: the particles counting-sorted by the
+// Morton code of their contact cell, its coordinates taken modulo 128, so that
+// a wavefront's lanes share most of their 27 cells.  Within a code the order is
+// whatever the atomics give.
+__global__ void __launch_bounds__(256) k_visit_keys(const float4* __restrict__ pos, int N, float invCell, int* keys,
+                                                    int* slot, int* hist) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float4 p = pos[i];
+  const int m[3] = { (int)floorf(p.x * invCell) & 127, (int)floorf(p.y * invCell) & 127,
+                     (int)floorf(p.z * invCell) & 127 };
+  int mk = 0;
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+    mk |= ((m[0] >> k) & 1) << (3 * k) | ((m[1] >> k) & 1) << (3 * k + 1) | ((m[2] >> k) & 1) << (3 * k + 2);
+  keys[i] = mk;
+  slot[i] = atomicAdd(&hist[mk], 1);
+}
+__global__ void __launch_bounds__(256) k_visit_scatter(int N, const int* __restrict__ keys,
+                                                       const int* __restrict__ slot, const int* __restrict__ start,
+                                                       int* visit) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) visit[start[keys[i]] + slot[i]] = i;
+}
+void launch_visit(const float4* pos, int N, float invCell, int* keys, int* slot, int* hist, int* blockSums,
+                  int* visit, hipStream_t s) {
+  launch_fill_u32(reinterpret_cast<uint32_t*>(hist), (size_t)KEYS + 1, 0u, s);
+  int blocks = (N + 255) / 256;
+  hipLaunchKernelGGL(k_visit_keys, dim3(blocks), dim3(256), 0, s, pos, N, invCell, keys, slot, hist);
+  launch_scan_excl(hist, KEYS + 1, blockSums, s);
+  hipLaunchKernelGGL(k_visit_scatter, dim3(blocks), dim3(256), 0, s, N, keys, slot, hist, visit);
+}
+__global__ void k_iota(int* a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+void launch_iota(int* a, int n, hipStream_t s) { hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, a, n); }
 
 __global__ void __launch_bounds__(256) k_permute(int N, const int* __restrict__ perm, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel, const float4* __restrict__ aux,

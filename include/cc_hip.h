@@ -287,6 +287,16 @@ typedef struct {
 int cc_autotune(cc_ctx* ctx, int reps, cc_tune_report* out);
 int cc_set_tuning(cc_ctx* ctx, int cellBlocks, int simOrder, int fuseCount);
 
+/* The order the step pass takes the particles in (no reference counterpart;
+ * like the tuning, it cannot change a result: each particle's sums depend only
+ * on the seats, which name particles by index).  The library sorts the
+ * particles by the Morton code of their contact cell at the start of a mesh
+ * cycle once `every` steps have passed since the last sort (and after an
+ * upload of positions or a Morton permutation), so that a wavefront's lanes
+ * share their neighbourhoods' cells and the caches hold them; every = 0 takes
+ * the particles in index order.  Default 256. */
+int cc_set_visit(cc_ctx* ctx, int every);
+
 /* Body setup on the host (no device work): relax() of src/bodies.js:97-146,
  * the overlap relaxation of a freshly laid pile (Gauss-Seidel over a cell grid
  * rebuilt every fourth sweep, pairs closer than 0.96 x 2a pushed apart by half

@@ -52,6 +52,36 @@ def test_every_tuning_gives_the_same_bits(name):
             assert np.array_equal(a, b), (cb, order, fuse, what)
 
 
+@pytest.mark.parametrize("name", ["impact", "twobox"])
+def test_visit_order_gives_the_same_bits(name):
+    """The step pass's visit order (cc_set_visit): index order, a sort every
+    8 steps and every 64, with cycle graphs and eagerly, through a Morton sort
+    of the indices and an upload in mid-run: the same states and seats."""
+    fx = gu.fixture("c1_" + name)
+    N = fx["pos"].shape[0]
+    perm_core = np.array([0.1, -0.2, 0.05])
+    ref = None
+    for every, graphs in ((0, True), (8, True), (64, True), (8, False)):
+        print(name, "visit every", every, "graphs", graphs, flush=True)
+        ctx = make(fx, graphs)
+        try:
+            ctx.set_visit(every)
+            ctx.step_n(21, ccamd.sim_args(), boxes(fx, 21))
+            ctx.morton(perm_core)
+            ctx.step_n(19, ccamd.sim_args(), boxes(fx, 19, 21))
+            P, V, A, Q, M = ctx.download()
+            ctx.upload(P, V, M, Q)
+            ctx.step_n(16, ccamd.sim_args(), boxes(fx, 16, 40))
+            got = list(ctx.download()[:5]) + list(ctx.read_seats())
+        finally:
+            ctx.close()
+        assert got[0].shape[0] == N
+        if ref is None:
+            ref = got
+        for a, b, what in zip(ref, got, ("pos", "vel", "aux", "q", "mat", "seatA", "seatB")):
+            assert np.array_equal(a, b), (every, graphs, what)
+
+
 def test_count_in_step_pass_through_interruptions():
     """The rebuild's count taken by the step pass before it, against a count of
     its own, through what can leave a taken count stale: eager steps, a change
