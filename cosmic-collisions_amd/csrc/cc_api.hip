@@ -465,7 +465,8 @@ int cc_read_seats(cc_ctx* c, uint32_t* seatA, uint32_t* seatB) {
 
 int cc_deposit(cc_ctx* c, int box) {
   if (!c || box < 0 || box > 1) return CC_EINVAL;
-  launch_deposit(c->pos[c->cur], c->mat, c->N, c->boxDev, box, (float)(FH * FH), c->mesh[box], false, c->s);
+  launch_deposit(c->pos[c->cur], c->mat, c->N, c->visitEvery > 0 ? c->visit : nullptr, c->boxDev, box,
+                 (float)(FH * FH), c->mesh[box], false, c->s);
   LAUNCHCHK(c);
   return 0;
 }
@@ -492,7 +493,8 @@ static int mesh_all(cc_ctx* c, int stepIdx = -1) {
     if (b == 1 && !(c->box.boxB > 0.5f)) break;
     MeshBuf& m = c->mesh[b];
     // the deposit also zeroes the force mesh's empty texels, which the cell pass below leaves
-    launch_deposit(c->pos[c->cur], c->mat, c->N, c->boxDev, b, (float)(FH * FH), m, true, c->s);
+    launch_deposit(c->pos[c->cur], c->mat, c->N, c->visitEvery > 0 ? c->visit : nullptr, c->boxDev, b,
+                   (float)(FH * FH), m, true, c->s);
     LAUNCHCHK(c);
     if ((r = cc_coarse(c, b))) return r;
     float eps2 = (float)(0.5 * FH * FH), gm = (float)c->p.m;

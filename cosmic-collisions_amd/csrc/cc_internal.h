@@ -137,7 +137,7 @@ void launch_copy4(const float4* src, float4* dst, int n, hipStream_t s);
 void launch_put_box(const Box& b, Box* dst, hipStream_t s);
 void launch_box_next(const Box* sched, int* idx, Box* dst, hipStream_t s);
 // zeroForce: also zero the force mesh's empty texels (a cell pass follows that writes the rest)
-void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2,
+void launch_deposit(const float4* pos, const float* mat, int N, const int* visit, const Box* b, int box, float eps2,
                     MeshBuf& m, bool zeroForce, hipStream_t s);
 void launch_coarse(MeshBuf& m, hipStream_t s);
 // the cell pass over the occupied fine texels, `blocks` work-groups of four wavefronts striding over them;

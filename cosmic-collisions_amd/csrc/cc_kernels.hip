@@ -327,15 +327,18 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
 // the sums are exact and independent of the order the adds land in.
 constexpr int TILE = 8;
 __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos, const float* __restrict__ mat, int N,
-                                                 const Box* __restrict__ bxp, int which,
+                                                 const int* __restrict__ visit, const Box* __restrict__ bxp, int which,
                                                  const float4* __restrict__ mono, float eps2, unsigned long long* acc,
                                                  int* occN) {
   __shared__ unsigned long long tile[TILE * TILE * TILE * 4];
   __shared__ unsigned long long stray[4];   // the work-group's strays, summed before one atomic per channel
   __shared__ int sum[4];                    // the deposited particles' cells, summed, and their count
-  int i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i == 0) *occN = 0;   // the occupied list k_deposit_finish appends to
-  bool live = i < N;
+  // the particles in the step pass's visit order (or index order): a work-group's particles then lie in a
+  // small block of cells, whatever the index order; the sums are integers, so the order changes nothing
+  const int at = xcd_block() * blockDim.x + threadIdx.x;
+  if (at == 0) *occN = 0;   // the occupied list k_deposit_finish appends to
+  bool live = at < N;
+  const int i = live && visit ? visit[at] : at;
   float4 p = live ? pos[i] : make_float4(0, 0, 0, 0);
   float mr = live ? mat[i] : 0.0f;
   float gx = 0, gy = 0, gz = 0;
@@ -434,10 +437,11 @@ void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s) {
 }
 
 
-void launch_deposit(const float4* pos, const float* mat, int N, const Box* b, int box, float eps2, MeshBuf& m,
-                    bool zeroForce, hipStream_t s) {
+void launch_deposit(const float4* pos, const float* mat, int N, const int* visit, const Box* b, int box, float eps2,
+                    MeshBuf& m, bool zeroForce, hipStream_t s) {
   int blocks = (N + 255) / 256;
-  hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, b, box, m.mono, eps2, m.acc, m.occN);
+  hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, visit, b, box, m.mono, eps2, m.acc,
+                     m.occN);
   hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.com, m.occ,
                      m.occN, zeroForce ? m.force : nullptr);
 }
@@ -566,9 +570,19 @@ __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ ccom, fl
 }
 
 // ------------------------------------------------- per-cell gravity (A7) --
+// 1/sqrt(x) for a normal x > 0: the hardware estimate and one Newton step
+// (within an ulp; the cell pass sums in its own order, so the reference's
+// 1.0 / sqrt() need not be rounded as it rounds).
+__device__ __forceinline__ float rsqrt_nr(float x) {
+  const float y = __builtin_amdgcn_rsqf(x), h = 0.5f * x;
+  const float e = fmaf(-h * y, y, 0.5f);
+  return fmaf(y, e, y);
+}
+
 // One wavefront per occupied fine cell; lane l takes fine cell l of each of
 // the 5^3 surrounding coarse blocks; the far field comes from the block's
 // tidal tensor (src/glsl-sim.js:303-352).
+constexpr int CELL_AHEAD = 4;   // blocks whose cells are loaded together
 __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, const float4* __restrict__ com,
                                               const float4* __restrict__ coarse, const float4* __restrict__ ccom,
                                               const float4* __restrict__ farA, const float4* __restrict__ farT0,
@@ -590,27 +604,42 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
     int q0 = c0 >> 2, q1 = c1 >> 2, q2 = c2 >> 2;
     float a0 = 0, a1 = 0, a2 = 0, pot = 0;
     // The 5^3 coarse blocks around, b = (kz+2)*25 + (ky+2)*5 + (kx+2): which hold
-    // mass, read all at once (two blocks a lane) into two wavefront masks, so
-    // the walk below does not wait on one block's load before the next.
+    // mass, read all at once (two blocks a lane) into two wavefront masks; the
+    // walk then loads the cells of CELL_AHEAD held blocks together before
+    // taking their terms, in the blocks' order.
     auto held = [&](int b) {
       if (b >= 125) return false;
       int k0 = q0 + b % 5 - 2, k1 = q1 + (b / 5) % 5 - 2, k2 = q2 + b / 25 - 2;
       if (k0 < 0 || k1 < 0 || k2 < 0 || k0 > 15 || k1 > 15 || k2 > 15) return false;
       return coarse[coarse_at(k0, k1, k2)].w > 0.0f;
     };
-    const unsigned long long held0 = __ballot(held(lane)), held1 = __ballot(held(lane + 64));
-    for (int half = 0; half < 2; half++) for (unsigned long long m = half ? held1 : held0; m; m &= m - 1) {
-      int b = (__ffsll((long long)m) - 1) + 64 * half;
-      int k0 = q0 + b % 5 - 2, k1 = q1 + (b / 5) % 5 - 2, k2 = q2 + b / 25 - 2;
-      int f0 = k0 * 4 + fx, f1 = k1 * 4 + fy, f2 = k2 * 4 + fz;
-      if (f0 == c0 && f1 == c1 && f2 == c2) continue;
-      float4 mf = com[fine_at(f0, f1, f2)];
-      if (mf.w <= 0.0f) continue;
-      float r0 = mf.x - x0, r1 = mf.y - x1, r2 = mf.z - x2;
-      float inv = qdiv(1.0f, qsqrt(dot3(r0, r1, r2, r0, r1, r2) + eps2));
-      float s = mf.w * inv * inv * inv;
-      a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
-      pot -= mf.w * inv;
+    unsigned long long m0 = __ballot(held(lane)), m1 = __ballot(held(lane + 64));
+    while (m0 | m1) {
+      float4 mf[CELL_AHEAD];
+      bool self[CELL_AHEAD];
+#pragma unroll
+      for (int k = 0; k < CELL_AHEAD; k++) {
+        int b = -1;
+        if (m0) { b = __ffsll((long long)m0) - 1; m0 &= m0 - 1; }
+        else if (m1) { b = __ffsll((long long)m1) - 1 + 64; m1 &= m1 - 1; }
+        self[k] = true;
+        mf[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (b >= 0) {
+          int k0 = q0 + b % 5 - 2, k1 = q1 + (b / 5) % 5 - 2, k2 = q2 + b / 25 - 2;
+          int f0 = k0 * 4 + fx, f1 = k1 * 4 + fy, f2 = k2 * 4 + fz;
+          self[k] = f0 == c0 && f1 == c1 && f2 == c2;
+          mf[k] = com[fine_at(f0, f1, f2)];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CELL_AHEAD; k++) {
+        if (self[k] || mf[k].w <= 0.0f) continue;
+        float r0 = mf[k].x - x0, r1 = mf[k].y - x1, r2 = mf[k].z - x2;
+        float inv = rsqrt_nr(dot3(r0, r1, r2, r0, r1, r2) + eps2);
+        float s = mf[k].w * inv * inv * inv;
+        a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
+        pot -= mf[k].w * inv;
+      }
     }
     a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2); pot = wave_sum(pot);
     if (lane == 0) {
