@@ -1521,8 +1521,9 @@ void launch_morton(const float4* pos, int N, double cx, double cy, double cz, in
 // particle's sums depend only on the seats, which name particles by index), so
 // it is chosen for the caches alone: the particles counting-sorted by the
 // Morton code of their contact cell, its coordinates taken modulo 128, so that
-// a wavefront's lanes share most of their 27 cells.  Within a code the order is
-// whatever the atomics give.
+// a wavefront's lanes share most of their 27 cells (and a work-group of the
+// deposit most of its mesh cells).  Within a code the order is whatever the
+// atomics give.
 __global__ void __launch_bounds__(256) k_visit_keys(const float4* __restrict__ pos, int N, float invCell, int* keys,
                                                     int* slot, int* hist) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
