@@ -355,7 +355,12 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
   if (threadIdx.x < 4) { sum[threadIdx.x] = 0; stray[threadIdx.x] = 0ull; }
   for (int t = threadIdx.x; t < TILE * TILE * TILE * 4; t += blockDim.x) tile[t] = 0ull;
   __syncthreads();
-  if (dep) { atomicAdd(&sum[0], cx); atomicAdd(&sum[1], cy); atomicAdd(&sum[2], cz); atomicAdd(&sum[3], 1); }
+  {   // the cells summed per wavefront first: one LDS atomic per wavefront and component, not one per lane
+    int sx = wave_sum(dep ? cx : 0), sy = wave_sum(dep ? cy : 0), sz = wave_sum(dep ? cz : 0), sn = wave_sum(dep ? 1 : 0);
+    if ((threadIdx.x & 63) == 0 && sn > 0) {
+      atomicAdd(&sum[0], sx); atomicAdd(&sum[1], sy); atomicAdd(&sum[2], sz); atomicAdd(&sum[3], sn);
+    }
+  }
   __syncthreads();
   // The tile is centred on the work-group's mean cell; a particle whose corners
   // fall outside it (an outlier of a work-group the motion has spread since the
@@ -364,8 +369,13 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
   int ox = sum[0] / nd - (TILE / 2 - 1), oy = sum[1] / nd - (TILE / 2 - 1), oz = sum[2] / nd - (TILE / 2 - 1);
   bool tiled = dep && cx >= ox && cy >= oy && cz >= oz && cx + 1 < ox + TILE && cy + 1 < oy + TILE && cz + 1 < oz + TILE;
   if (dep) {
+    // Each lane walks its eight corners from its own starting corner: the lanes of a wavefront lie in a few
+    // cells, and in step they would all add to the same few texels at once, which the LDS serialises.  The
+    // sums are integers, so the order of the adds changes nothing.
+    const int k0 = threadIdx.x & 7;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+    for (int kk = 0; kk < 8; kk++) {
+      const int k = (kk + k0) & 7;
       int o0 = k & 1, o1 = (k >> 1) & 1, o2 = k >> 2;
       int c0 = cx + o0, c1 = cy + o1, c2 = cz + o2;
       if (c0 > 63 || c1 > 63 || c2 > 63) continue;
