@@ -37,6 +37,17 @@ def build_hip():
     return out
 
 
+def build_checks():
+    """tests/recip_check: the GPU check of the correctly rounded helpers
+    (csrc/cc_fpmath.h), run by tests/test_gpu_fpmath.py."""
+    src = os.path.join(REPO, "tests", "recip_check.hip")
+    out = os.path.join(REPO, "tests", "recip_check")
+    run([HIPCC] + [f for f in HIP_FLAGS if f not in ("-fPIC", "-fhip-fp32-correctly-rounded-divide-sqrt")] +
+        ["-o", out + ".tmp", src], HERE)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_addon():
     """The N-API addon (js/cc_addon.cc) linked against libcc_hip.so; skipped
     where node's headers are absent."""
@@ -56,4 +67,5 @@ def build_addon():
 
 if __name__ == "__main__":
     print(build_hip())
+    print(build_checks())
     print(build_addon())

@@ -17,57 +17,14 @@
 #include <cmath>
 #include <cstring>
 
+#include "cc_fpmath.h"
 #include "cc_internal.h"
 
 namespace cc {
 
 // ---------------------------------------------------------------- helpers --
-__device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }        // GLSL inversesqrt
-// The correctly rounded quotient a / b for normal operands and a normal (or
-// zero) quotient, without the scaling and fix-up steps that only extreme
-// exponents need: the hardware reciprocal refined once, then two residual
-// corrections (each residual exact by FMA).  Checked bit for bit against the
-// compiler's correctly rounded division on 4.3e9 random pairs with exponents
-// in [-30, 30] on gfx950: no difference.  Used where the operands are
-// distances, masses and forces, far from the exponent limits.
-__device__ __forceinline__ float qdiv(float a, float b) {
-  float y = __builtin_amdgcn_rcpf(b);
-  float e = fmaf(-b, y, 1.0f);
-  y = fmaf(e, y, y);
-  float q = a * y;
-  float r = fmaf(-b, q, a);
-  q = fmaf(r, y, q);
-  r = fmaf(-b, q, a);
-  return fmaf(r, y, q);
-}
-// a / b correctly rounded for a divisor whose correctly rounded reciprocal y
-// is at hand (a uniform divisor, or one shared by several quotients): one
-// residual correction of a * y (Markstein's: y within half an ulp of 1/b makes
-// q + (a - b q) y round to a / b), valid while the residual a - b q does not
-// underflow, i.e. for dividends above about 2^-100 (or 0) and normal
-// quotients.  Checked on the CPU (tests/markstein_check.c): exhaustively for
-// the pairwise pass's two fixed divisors over every dividend in [2^-80, 1],
-// and on random pairs with exponents in [-60, 60] (1.6e10 of them with no
-// difference when this was written).  The pass's dividends are distances and
-// coordinate differences (0, or far above 2^-75) and the interpolated pair
-// table (0, or of the table's size: its potential column is at least 0.039).
-__device__ __forceinline__ float qdiv_by(float a, float b, float y) {
-  float q = a * y;
-  float r = fmaf(-b, q, a);
-  return fmaf(r, y, q);
-}
-// The correctly rounded square root of a normal x > 0 (or of 0) without the
-// scaling for tiny x: the hardware root, then the neighbour above or below
-// where its residual says the root lies.  Checked bit for bit against the
-// compiler's correctly rounded sqrtf on 4.3e9 random x with exponents in
-// [-60, 60] on gfx950: no difference.
-__device__ __forceinline__ float qsqrt(float x) {
-  float s = __builtin_amdgcn_sqrtf(x);
-  float sd = __uint_as_float(__float_as_uint(s) - 1u), su = __uint_as_float(__float_as_uint(s) + 1u);
-  float rd = fmaf(-sd, s, x), ru = fmaf(-su, s, x);
-  s = rd <= 0.0f ? sd : s;
-  return ru > 0.0f ? su : s;
-}
+// GLSL inversesqrt as 1/sqrt(x), both correctly rounded, for a normal x > 0 (a squared distance plus a softening)
+__device__ __forceinline__ float rsq(float x) { return qrcp(qsqrt(x)); }
 __device__ __forceinline__ float gmax(float a, float b) { return a > b ? a : b; }  // GLSL max
 __device__ __forceinline__ float gmin(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
@@ -393,7 +350,8 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
     }
   } else if (live && mine < 0) {
     float4 mo = mono[0];
-    float rx = mo.x / gmax(mo.w, 1e-6f) - p.x, ry = mo.y / gmax(mo.w, 1e-6f) - p.y, rz = mo.z / gmax(mo.w, 1e-6f) - p.z;
+    const float w = gmax(mo.w, 1e-6f), yw = qrcp(w);
+    float rx = qdiv_by(mo.x, w, yw) - p.x, ry = qdiv_by(mo.y, w, yw) - p.y, rz = qdiv_by(mo.z, w, yw) - p.z;
     float inv = rsq(dot3(rx, ry, rz, rx, ry, rz) + eps2);
     float s = mr * inv * inv * inv;
     atomicAdd(&stray[0], to_fixed(rx * s, SCR_SCALE)); atomicAdd(&stray[1], to_fixed(ry * s, SCR_SCALE));
@@ -417,7 +375,9 @@ __global__ void __launch_bounds__(256) k_deposit(const float4* __restrict__ pos,
 // mass: computed once per deposit, where every cell pass used to recompute it
 // for each of the ~100 texels that read it.
 __device__ __forceinline__ float4 centre_of(float4 v) {
-  return v.w > 0.0f ? make_float4(v.x / v.w, v.y / v.w, v.z / v.w, v.w) : make_float4(0.0f, 0.0f, 0.0f, v.w);
+  if (!(v.w > 0.0f)) return make_float4(0.0f, 0.0f, 0.0f, v.w);
+  const float y = qrcp(v.w);   // the three quotients share 1/w
+  return make_float4(qdiv_by(v.x, v.w, y), qdiv_by(v.y, v.w, y), qdiv_by(v.z, v.w, y), v.w);
 }
 
 // The accumulator to the FP32 mesh (and back to zero), each texel's centre of
@@ -557,7 +517,7 @@ __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ ccom, fl
       float4 m = mq[u];
       if (m.w <= 0.0f) continue;
       float r0 = m.x - x0, r1 = m.y - x1, r2 = m.z - x2;
-      float inv2 = qdiv(1.0f, dot3(r0, r1, r2, r0, r1, r2)), inv = qsqrt(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
+      float inv2 = qrcp(dot3(r0, r1, r2, r0, r1, r2)), inv = qsqrt(inv2), inv3 = inv * inv2, inv5 = inv3 * inv2;
       float s = m.w * inv3;
       a0 += r0 * s; a1 += r1 * s; a2 += r2 * s;
       pot -= m.w * inv;
@@ -761,7 +721,7 @@ __device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj
   float x = gclamp(qdiv_by(r, cut, yCut) * 64.0f - 0.5f, 0.0f, 63.0f);
   int k = (int)x, k1 = min(k + 1, 63);
   float t = x - (float)k;
-  float invN = qdiv(1.0f, qsqrt(r2 + a2));
+  float invN = qrcp(qsqrt(r2 + a2));
   // smoothstep(cut, 0.8 cut, r): (r - cut) / (0.8 cut - cut), both signs turned (exactly) to divide by a positive
   float taper = gclamp(qdiv_by(cut - r, tapD, yTap), 0.0f, 1.0f);
   taper = taper * taper * (3.0f - 2.0f * taper);
@@ -769,7 +729,7 @@ __device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj
   float2 tk = st[k], tk1 = st[k1];
   float Tx = gmix(tk.x, tk1.x, t) * same, Ty = gmix(tk.y, tk1.y, t) * same;
   float fn = invN * invN * invN * r - qdiv(Tx, gmax(r2, a2));
-  float rr = gmax(r, 1e-6f), yr = qdiv(1.0f, rr);   // 1/rr correctly rounded, shared by four quotients
+  float rr = gmax(r, 1e-6f), yr = qrcp(rr);   // 1/rr correctly rounded, shared by four quotients
   float n0 = qdiv_by(d0, rr, yr), n1 = qdiv_by(d1, rr, yr), n2 = qdiv_by(d2, rr, yr);
   // max(r, a) is rr where r > a (a > 1e-6), else the uniform a with its reciprocal ya
   float ph = -invN - qdiv_by(Ty, gmax(r, a), r > a ? yr : ya);
@@ -804,7 +764,7 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
   const float4 P = i < count ? list[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const bool live = i < count && P.w != 0.0f;
   const float cut2 = cut * cut, a = sqrtf(a2), tapD = cut - 0.8f * cut;
-  const float yCut = qdiv(1.0f, cut), yTap = qdiv(1.0f, tapD), ya = qdiv(1.0f, a);
+  const float yCut = qrcp(cut), yTap = qrcp(tapD), ya = qrcp(a);
   const float4* chain = reinterpret_cast<const float4*>(term + lane * PP_CHAIN);   // step 3's row (lane = sc * PP_ROWS + sr)
   float run = 0.0f;
   // the seats' box (the live ones; lanes differing in their low three bits hold the eight seats), the same
@@ -904,19 +864,22 @@ void launch_pp_slot(int N, const int* index, int count, float* slot, hipStream_t
 // -------------------------------------------------------- the step (A12) --
 __device__ __forceinline__ void mass_at(float px, float py, float pz, float4 mo, float gm, float eps2, float& ox,
                                         float& oy, float& oz) {
-  float rx = mo.x / gmax(mo.w, 1e-6f) - px, ry = mo.y / gmax(mo.w, 1e-6f) - py, rz = mo.z / gmax(mo.w, 1e-6f) - pz;
+  const float w = gmax(mo.w, 1e-6f), yw = qrcp(w);   // the centre's three quotients share 1/w
+  float rx = qdiv_by(mo.x, w, yw) - px, ry = qdiv_by(mo.y, w, yw) - py, rz = qdiv_by(mo.z, w, yw) - pz;
   float inv = rsq(dot3(rx, ry, rz, rx, ry, rz) + eps2);
   float s = gm * mo.w * inv * inv * inv;
   ox = rx * s; oy = ry * s; oz = rz * s;
 }
 __device__ __forceinline__ float pot_at(float px, float py, float pz, float4 mo, float gm, float eps2) {
-  float rx = mo.x / gmax(mo.w, 1e-6f) - px, ry = mo.y / gmax(mo.w, 1e-6f) - py, rz = mo.z / gmax(mo.w, 1e-6f) - pz;
+  const float w = gmax(mo.w, 1e-6f), yw = qrcp(w);   // the centre's three quotients share 1/w
+  float rx = qdiv_by(mo.x, w, yw) - px, ry = qdiv_by(mo.y, w, yw) - py, rz = qdiv_by(mo.z, w, yw) - pz;
   return -gm * mo.w * rsq(dot3(rx, ry, rz, rx, ry, rz) + eps2);
 }
 __device__ __forceinline__ void cross_at(float4 mo, float4 mm, float gm, float eps2, float& ox, float& oy, float& oz) {
-  float dx = mo.x / gmax(mo.w, 1e-6f) - mm.x / gmax(mm.w, 1e-6f);
-  float dy = mo.y / gmax(mo.w, 1e-6f) - mm.y / gmax(mm.w, 1e-6f);
-  float dz = mo.z / gmax(mo.w, 1e-6f) - mm.z / gmax(mm.w, 1e-6f);
+  const float wo = gmax(mo.w, 1e-6f), wm = gmax(mm.w, 1e-6f), yo = qrcp(wo), ym = qrcp(wm);
+  float dx = qdiv_by(mo.x, wo, yo) - qdiv_by(mm.x, wm, ym);
+  float dy = qdiv_by(mo.y, wo, yo) - qdiv_by(mm.y, wm, ym);
+  float dz = qdiv_by(mo.z, wo, yo) - qdiv_by(mm.z, wm, ym);
   float inv = rsq(dot3(dx, dy, dz, dx, dy, dz) + eps2);
   float s = gm * mo.w * inv * inv * inv;
   ox = dx * s; oy = dy * s; oz = dz * s;
@@ -967,7 +930,8 @@ __device__ __forceinline__ void contact_pair(const SimU& u, float px, float py, 
   float ddx = Pj.x - px, ddy = Pj.y - py, ddz = Pj.z - pz;
   float r = qsqrt(dot3(ddx, ddy, ddz, ddx, ddy, ddz));
   float rr = gmax(r, 1e-6f);
-  float nx = qdiv(ddx, rr), ny = qdiv(ddy, rr), nz = qdiv(ddz, rr);
+  const float yr = qrcp(rr);   // 1/rr correctly rounded, shared by the three quotients
+  float nx = qdiv_by(ddx, rr, yr), ny = qdiv_by(ddy, rr, yr), nz = qdiv_by(ddz, rr, yr);
   float ov = u.touch - r;
   float cij = self.cii;
   if (__any(rj != ri)) cij = u.c * qsqrt(qdiv(2.0f * ri * rj, ri + rj));
@@ -1348,7 +1312,7 @@ __global__ void __launch_bounds__(256) k_couple(int N, CoupleU c, const float4* 
   float th = acosf(ct);
   float f = (c.z - 2.0f) * tanf(0.5f * th);
   float dx = nx + f * ex, dy = ny + f * ey, dz = nz + f * ez;
-  float inv = rsq(dot3(dx, dy, dz, dx, dy, dz));
+  float inv = 1.0f / sqrtf(dot3(dx, dy, dz, dx, dy, dz));   // (a direction of any length: the general forms)
   dx *= inv; dy *= inv; dz *= inv;
   bool imp = w > 3.5f;
   float vx = dx * v, vy = dy * v, vz = dz * v;
