@@ -381,21 +381,42 @@ __device__ __forceinline__ float4 centre_of(float4 v) {
 }
 
 // The accumulator to the FP32 mesh (and back to zero), each texel's centre of
-// mass; the occupied texels listed.
-__global__ void __launch_bounds__(256) k_deposit_finish(unsigned long long* acc, float4* fine, float4* com, int* occ,
-                                                        int* occN, float4* force) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= FINE_TEXELS) return;
-  unsigned long long* a = acc + (size_t)t * 4;
-  double sc = t == SCRATCH ? 1.0 / SCR_SCALE : 1.0 / DEP_SCALE;
-  float4 v = make_float4((float)((double)(long long)a[0] * sc), (float)((double)(long long)a[1] * sc),
-                         (float)((double)(long long)a[2] * sc), (float)((double)(long long)a[3] * sc));
-  a[0] = 0; a[1] = 0; a[2] = 0; a[3] = 0;
-  fine[t] = v;
-  com[t] = centre_of(v);
-  if (t < SCRATCH && v.w > 0.0f) occ[atomicAdd(occN, 1)] = t;
-  // given a force mesh: its empty texels, which the cell pass (it writes the occupied ones and the scratch row) leaves
-  else if (force && t < SCRATCH) force[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// mass; the occupied texels listed.  The list's order is immaterial (each
+// entry's cell pass writes its own texel); a work-group of 1024 texels takes
+// its places in it with one atomic, where one per wavefront had thousands of
+// them queue on the counter once the ejecta spread over the mesh.
+constexpr int FINISH_BLOCK = 1024;
+__global__ void __launch_bounds__(FINISH_BLOCK) k_deposit_finish(unsigned long long* acc, float4* fine, float4* com,
+                                                                 int* occ, int* occN, float4* force) {
+  __shared__ int waveBase[FINISH_BLOCK / 64];
+  __shared__ int groupBase;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool app = false;
+  if (t < FINE_TEXELS) {
+    unsigned long long* a = acc + (size_t)t * 4;
+    double sc = t == SCRATCH ? 1.0 / SCR_SCALE : 1.0 / DEP_SCALE;
+    float4 v = make_float4((float)((double)(long long)a[0] * sc), (float)((double)(long long)a[1] * sc),
+                           (float)((double)(long long)a[2] * sc), (float)((double)(long long)a[3] * sc));
+    a[0] = 0; a[1] = 0; a[2] = 0; a[3] = 0;
+    fine[t] = v;
+    com[t] = centre_of(v);
+    app = t < SCRATCH && v.w > 0.0f;
+    // given a force mesh: its empty texels, which the cell pass (it writes the occupied ones and the scratch row) leaves
+    if (!app && force && t < SCRATCH) force[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  const unsigned long long m = __ballot(app);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) waveBase[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int w = 0; w < FINISH_BLOCK / 64; w++) { const int c = waveBase[w]; waveBase[w] = n; n += c; }
+    groupBase = n ? atomicAdd(occN, n) : 0;
+  }
+  __syncthreads();
+  if (app)
+    occ[groupBase + waveBase[wave] +
+        (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = t;
 }
 // the centres of a mesh written from outside (fine or coarse)
 __global__ void __launch_bounds__(256) k_centres(const float4* __restrict__ mesh, float4* com, int n) {
@@ -412,7 +433,8 @@ void launch_deposit(const float4* pos, const float* mat, int N, const int* visit
   int blocks = (N + 255) / 256;
   hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), 0, s, pos, mat, N, visit, b, box, m.mono, eps2, m.acc,
                      m.occN);
-  hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + 255) / 256), dim3(256), 0, s, m.acc, m.fine, m.com, m.occ,
+  hipLaunchKernelGGL(k_deposit_finish, dim3((FINE_TEXELS + FINISH_BLOCK - 1) / FINISH_BLOCK), dim3(FINISH_BLOCK), 0, s,
+                     m.acc, m.fine, m.com, m.occ,
                      m.occN, zeroForce ? m.force : nullptr);
 }
 
