@@ -24,6 +24,7 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(REPO, "cosmic-collisions_amd")
 sys.path.insert(0, PKG)
+import ccamd  # noqa: E402
 import runs  # noqa: E402  (the library's prepared runs; loads the HIP library lazily)
 
 FL = 6.0
@@ -121,40 +122,61 @@ def batch_leg(out_dir, count, steps, warmup, reuse):
     return out
 
 
-def learned_leg(state_root, steps, warmup, reuse, cycles, iters):
-    """The learned far field: trained on the GPU from the exact far field of
-    the theia-like, twins and shatter runs, then judged in the shatter run
-    (every 16 steps, fallback above a 1e-3 relative force error) beside the
-    same run with the exact far field."""
+def _books(ctx):
+    """The FP64 books after a DIAG pass: angular momentum and energy (KE + PE + heat)."""
+    ctx.diag()
+    b = ctx.books()
+    return np.array([b["Jx"], b["Jy"], b["Jz"]]), b["KE"] + b["PE"] + b["Q"]
+
+
+def learned_leg(state_root, batch_dir, steps, warmup, reuse, cycles, sweep, sweep_cycles, iters):
+    """The learned far field on the shatter impact, a configuration it was not
+    trained on: trained on the GPU from the exact far field of the theia-like and
+    twins runs and the batched sweep's instances, then judged in the shatter run
+    (every 16 steps, fallback above a 1e-3 relative force error) beside the same
+    run with the exact far field and with a zero network (each mesh step between
+    judges keeps the base's far field: what the network's prediction adds to)."""
     import ff_train
-    ff, rep = ff_train.build(["THEIA", "TWINS", "SHATTER"], state_root, cycles, iters, reuse=reuse,
+    ff, rep = ff_train.build(["THEIA", "TWINS"], state_root, cycles, iters, reuse=reuse, sweep=sweep,
+                             sweep_cycles=sweep_cycles, batch_dir=batch_dir,
                              log=lambda m: print(m, file=sys.stderr, flush=True))
+    zero = ccamd.FarField(capacity=0)
+    zero.set_weights(np.zeros(zero.nw, np.float32))
     run, st = runs.prepare(runs.SHATTER, os.path.join(state_root, "cc_shatter_state"), reuse=True)
     N = run["sim"]["N"]
-    out = {"workload": "shatter: 262,144 particles, 0.5 M_earth impactor head-on at 2.6 v_esc; the far field from "
-                       "a 4-level 32-channel 3D conv operator (BF16 on the matrix cores) trained here on the GPU, "
-                       "exact far field every 16 steps as judge, fallback above 1e-3 relative force error",
+    out = {"workload": "shatter: 262,144 particles, 0.5 M_earth impactor head-on at 2.6 v_esc, %d steps from first "
+                       "touch; between exact judges every 16 steps the far field's change since the last judge from "
+                       "a 4-level 32-channel 3D conv operator (BF16 on the matrix cores) trained here on the GPU on "
+                       "other impacts; fallback above 1e-3 relative force error" % (warmup + steps),
            "particles": N, "steps": steps, "training": rep}
-    for mode in ("exact", "learned"):
+    for mode, net in (("exact", None), ("learned", ff), ("zero_network", zero)):
         R = runs.Runner(run, st)
-        if mode == "learned":
-            R.ctx.set_far_field(ff, 16, 1e-3)
+        J0, E0 = _books(R.ctx)
+        if net is not None:
+            R.ctx.set_far_field(net, 16, 1e-3)
         R.steps(warmup)
         R.ctx.sync()
         t0 = time.perf_counter()
         R.steps(steps)
         R.ctx.sync()
         wall = time.perf_counter() - t0
-        out[mode] = {"value": N * steps / wall, "ms_per_step": wall * 1e3 / steps,
-                     "mesh_ms": R.ctx.time_pass("mesh_between", 20)}
-        if mode == "learned":
-            st_ = R.ctx.ff_stats()
-            out[mode].update({"last_force_err": float(st_[0, 0]), "judged": int(st_[0, 1]),
-                              "fallbacks": int(st_[0, 2]), "learned_mesh_steps": int(st_[0, 3])})
+        J, E = _books(R.ctx)
+        o = {"value": N * steps / wall, "ms_per_step": wall * 1e3 / steps,
+             "dJ_rel": float(np.linalg.norm(J - J0) / np.linalg.norm(J0)), "E_end_rel": E / abs(E0)}
+        if net is not None:
+            st_ = dict(zip(ccamd.Context.FF_STATS, R.ctx.ff_stats()[0].tolist()))
+            o.update({"judged": int(st_["judged"]), "fallbacks": int(st_["fallbacks"]),
+                      "learned_mesh_steps": int(st_["learned"]), "last_force_err": st_["last_err"],
+                      "max_force_err": st_["max_err"],
+                      "mean_force_err": st_["sum_err"] / max(st_["usable"], 1.0)})
+        out[mode] = o
         R.close()
     ff.close()
+    zero.close()
+    for mode in ("learned", "zero_network"):   # energy (KE + PE + heat, no contact springs) against the exact run's
+        out[mode]["dE_vs_exact"] = out[mode]["E_end_rel"] - out["exact"]["E_end_rel"]
     # the learned far field's throughput only where the network stood in on some mesh step; otherwise every
-    # mesh step ran the exact far field (and the network beside it), which is no learned figure
+    # mesh step ran the exact far field (and the judge beside it), which is no learned figure
     stood_in = out["learned"]["learned_mesh_steps"] > 0
     out["value"] = out["learned"]["value"] if stood_in else None
     if not stood_in:
@@ -244,8 +266,10 @@ def main():
     ap.add_argument("--theia-steps", type=int, default=1000, help="the theia-like leg's steps (0: skip it)")
     ap.add_argument("--crater-steps", type=int, default=20000, help="the 524,288-grain crater leg's steps (0: skip it)")
     ap.add_argument("--learned-steps", type=int, default=1000)
-    ap.add_argument("--ff-cycles", type=int, default=48)
-    ap.add_argument("--ff-iters", type=int, default=3000)
+    ap.add_argument("--ff-cycles", type=int, default=250, help="8-step cycles recorded per training run")
+    ap.add_argument("--ff-sweep", type=int, default=64, help="sweep instances recorded for training (0: none)")
+    ap.add_argument("--ff-sweep-cycles", type=int, default=40)
+    ap.add_argument("--ff-iters", type=int, default=6000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)
@@ -318,8 +342,9 @@ def main():
     if args.build > 0:
         out["body_build"] = build_leg()
     if args.learned > 0:
-        out["learned"] = learned_leg(tempfile.gettempdir(), args.learned_steps, 200, args.reuse_state, args.ff_cycles,
-                                     args.ff_iters)
+        out["learned"] = learned_leg(tempfile.gettempdir(), args.batch_dir, args.learned_steps, 200,
+                                     True, args.ff_cycles, args.ff_sweep,
+                                     args.ff_sweep_cycles, args.ff_iters)
     out["hip_runtime"] = hip_runtime()
     print(json.dumps(out))
     return 0

@@ -41,6 +41,7 @@ EXPORTS = ["cc_create", "cc_destroy", "cc_last_error", "cc_set_params", "cc_uplo
            "cc_batch_step_n", "cc_batch_last_ms", "cc_ff_create", "cc_ff_destroy", "cc_ff_last_error", "cc_ff_set_slope",
            "cc_ff_weight_count", "cc_ff_get_weights", "cc_ff_set_weights", "cc_ff_record", "cc_ff_push_sample",
            "cc_ff_read_sample", "cc_ff_samples", "cc_ff_clear", "cc_ff_train", "cc_ff_eval", "cc_ff_gradient",
+           "cc_ff_passes", "cc_ff_read_pass", "cc_ff_clear_passes", "cc_ff_pair",
            "cc_set_far_field", "cc_ff_stats", "cc_relax_piles", "cc_bodies_last_error", "cc_autotune", "cc_set_tuning", "cc_set_visit", "cc_crater_stats", "cc_pp_noalloc", "cc_analyse_begin", "cc_analyse_end"]
 
 
@@ -178,6 +179,8 @@ def lib():
             "cc_ff_record": ([P, P, C.c_int], C.c_int), "cc_ff_push_sample": ([P, fp, fp, fp], C.c_int),
             "cc_ff_read_sample": ([P, C.c_int, fp, fp, fp], C.c_int), "cc_ff_samples": ([P, ip], C.c_int),
             "cc_ff_clear": ([P], C.c_int), "cc_ff_set_slope": ([P, C.c_double], C.c_int),
+            "cc_ff_passes": ([P, ip], C.c_int), "cc_ff_read_pass": ([P, C.c_int, fp, fp, fp], C.c_int),
+            "cc_ff_clear_passes": ([P], C.c_int), "cc_ff_pair": ([P, C.c_int, C.c_int, ip], C.c_int),
             "cc_ff_train": ([P, C.c_int, C.c_int, C.c_double, C.c_uint32, dp], C.c_int),
             "cc_ff_eval": ([P, C.c_int, C.c_int, fp, dp, dp], C.c_int),
             "cc_ff_gradient": ([P, C.c_int, C.c_int, fp, dp], C.c_int),
@@ -372,11 +375,14 @@ class Context:
         self._chk(lib().cc_set_far_field(self.h, ff.h if ff is not None else None, int(judge_every), float(tol)),
                   "cc_set_far_field")
 
+    FF_STATS = ("last_err", "judged", "fallbacks", "learned", "max_err", "sum_err", "usable", "spare")
+
     def ff_stats(self):
-        """Per box: last relative force error, judge passes, fallbacks, mesh steps on the network."""
-        out = np.zeros(8, np.float64)
+        """Per box (2 x 8, FF_STATS): last relative force error, judge passes, fallbacks, mesh steps on the
+        network, the largest and the summed error and the count of judges measured from a usable base."""
+        out = np.zeros(16, np.float64)
         self._chk(lib().cc_ff_stats(self.h, _ptr(out, C.c_double)), "cc_ff_stats")
-        return out.reshape(2, 4)
+        return out.reshape(2, 8)
 
     def set_graphs(self, on):
         self._chk(lib().cc_set_graphs(self.h, int(bool(on))), "cc_set_graphs")
@@ -547,7 +553,30 @@ class FarField:
         self._chk(lib().cc_ff_set_weights(self.h, _ptr(w), w.size), "cc_ff_set_weights")
 
     def record(self, ctx, box=0):
+        """Record box `box` of ctx's last (exact) mesh pass; returns the pass index."""
         self._chk(lib().cc_ff_record(self.h, ctx.h, int(box)), "cc_ff_record")
+        return self.passes - 1
+
+    @property
+    def passes(self):
+        n = C.c_int()
+        self._chk(lib().cc_ff_passes(self.h, C.byref(n)), "cc_ff_passes")
+        return n.value
+
+    def read_pass(self, k):
+        """(feat[4096,4], meta[4], record[4096,12]) of recorded pass k."""
+        f, m, r = np.empty((4096, 4), np.float32), np.empty(4, np.float32), np.empty((4096, 12), np.float32)
+        self._chk(lib().cc_ff_read_pass(self.h, int(k), _ptr(f), _ptr(m), _ptr(r)), "cc_ff_read_pass")
+        return f, m, r
+
+    def clear_passes(self):
+        self._chk(lib().cc_ff_clear_passes(self.h), "cc_ff_clear_passes")
+
+    def pair(self, base, cur):
+        """A training sample: the prediction from recorded pass `base` to pass `cur`; returns its index."""
+        k = C.c_int()
+        self._chk(lib().cc_ff_pair(self.h, int(base), int(cur), C.byref(k)), "cc_ff_pair")
+        return k.value
 
     def push(self, x, target, mask):
         self._chk(lib().cc_ff_push_sample(self.h, _ptr(_f(x)), _ptr(_f(target)), _ptr(_f(mask))), "cc_ff_push_sample")

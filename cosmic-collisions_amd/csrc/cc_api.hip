@@ -78,8 +78,11 @@ struct cc_ctx {
   int* ffGate = nullptr;
   float* ffStats = nullptr;
   double* ffSums = nullptr;
-  float4* ffFar = nullptr;     // 3 x 4096: the network's farA, farT0, farT1
-  float4* ffForce = nullptr;   // the force mesh from the network's far field (judge steps)
+  float4* ffFar = nullptr;     // 3 x 4096: the network's farA, farT0, farT1 (judge steps)
+  // a judge runs on the second stream beside the steps after it: forked after the exact pass, joined by the end
+  // of its cycle (or of the call that forked it), ffPending while it is not joined
+  hipEvent_t ffFork = nullptr, ffJoin = nullptr;
+  bool ffPending = false;
   int stepNo = 0;
   // the pairwise pass of a mesh step runs on its own stream beside the seats and the mesh
   hipStream_t s2 = nullptr;
@@ -340,6 +343,8 @@ int cc_destroy(cc_ctx* c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ppFork) hipEventDestroy(c->ppFork);
   if (c->ppJoin) hipEventDestroy(c->ppJoin);
+  if (c->ffFork) hipEventDestroy(c->ffFork);
+  if (c->ffJoin) hipEventDestroy(c->ffJoin);
   if (c->s2) hipStreamDestroy(c->s2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
@@ -365,8 +370,11 @@ int cc_upload_state(cc_ctx* c, const float* pos, const float* vel, const float* 
   if (vel) HIPCHK(c, hipMemcpyAsync(c->vel[c->cur], vel, N * 16, hipMemcpyHostToDevice, c->s));
   if (mat) HIPCHK(c, hipMemcpyAsync(c->mat, mat, N * 4, hipMemcpyHostToDevice, c->s));
   if (q) HIPCHK(c, hipMemcpyAsync(c->q[c->cur], q, N * 4, hipMemcpyHostToDevice, c->s));
+  // new positions: the visit order is re-sorted at the next cycle's start, and a learned far field's bases,
+  // recorded from the old ones, serve no prediction before the next judge
+  if (pos && c->ffWork) ff_forget_bases(c->ffWork, c->s);
   HIPCHK(c, hipStreamSynchronize(c->s));
-  if (pos) c->visitAt = -(1 << 30);   // new positions: the visit order is re-sorted at the next cycle's start
+  if (pos) c->visitAt = -(1 << 30);
   return 0;
 }
 
@@ -482,11 +490,21 @@ int cc_cell(cc_ctx* c, int box) {
   LAUNCHCHK(c);
   return 0;
 }
+// A judge still running on the second stream joined into stream s (work on s
+// after this waits for it).
+static int ff_join(cc_ctx* c, hipStream_t s) {
+  if (!c->ffPending) return 0;
+  HIPCHK(c, hipStreamWaitEvent(s, c->ffJoin, 0));
+  c->ffPending = false;
+  return 0;
+}
+
 // Deposit, coarse and per-cell passes of each box that is up.  With a
 // learned far field attached, stepIdx (the step of this mesh pass) decides:
-// on a judge step the exact far field makes the force mesh and the network's
-// is measured against it, which sets the gate; otherwise the gate picks the
-// exact far field or the network's.  stepIdx < 0: exact only.
+// on a judge step the exact far field makes the force mesh, the network's
+// prediction from the box's base is measured against it (which sets the gate),
+// and the exact pass becomes the base; otherwise the gate picks the exact far
+// field or the network's.  stepIdx < 0: exact only.
 static int mesh_all(cc_ctx* c, int stepIdx = -1) {
   int r;
   for (int b = 0; b < 2; b++) {
@@ -501,16 +519,26 @@ static int mesh_all(cc_ctx* c, int stepIdx = -1) {
     if (!c->ff || stepIdx < 0) {
       launch_cell(m, eps2, gm, c->cellBlocks, true, c->s);
     } else if (stepIdx % c->ffJudge == 0) {
-      float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
       launch_far(m, m.farA, m.farT0, m.farT1, nullptr, c->s);
       launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
-      ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, A, T0, T1, nullptr, c->s);
-      launch_cell_only(m, A, T0, T1, c->ffForce, eps2, gm, c->cellBlocks, false, c->s);
-      launch_judge(c->ffForce, m.force, c->ffSums + 2 * b, c->ffTol, c->ffStats + 4 * b, c->ffGate + b, c->s);
+      // The judge: the network's prediction from the box's base, its error against this exact pass and the
+      // verdict, then this pass becomes the base.  Nothing of this step waits for it, so it runs on the second
+      // stream (after the pairwise pass there) beside the steps that follow, joined by the end of the cycle;
+      // it reads this pass's mesh and box, which only the next cycle overwrites.
+      float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
+      HIPCHK(c, hipEventRecord(c->ffFork, c->s));
+      HIPCHK(c, hipStreamWaitEvent(c->s2, c->ffFork, 0));
+      ff_predict(c->ff, c->ffWork, b, m.coarse, m.mono, c->boxDev, A, T0, T1, nullptr, c->s2);
+      ff_judge_rebase(c->ffWork, b, m.fine, m.com, m.ccom, m.coarse, c->ffFar, m.farA, m.farT0, m.farT1, m.force,
+                      c->boxDev, gm, c->ffSums + 2 * b, c->ffTol, c->ffStats + FF_STATS * b, c->ffGate + b, c->s2);
+      HIPCHK(c, hipEventRecord(c->ffJoin, c->s2));
+      c->ffPending = true;
     } else {
-      launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
-      ff_infer(c->ff, c->ffWork, m.coarse, m.mono, c->boxDev, b, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
-      launch_count_learned(c->ffGate + b, c->ffStats + 4 * b, c->s);
+      // the prediction first: it clears the gate when the base cannot serve; then the exact far field, for
+      // every block when the network does not stand in, else for the blocks it has no base for
+      ff_predict(c->ff, c->ffWork, b, m.coarse, m.mono, c->boxDev, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
+      launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s, ff_base_features(c->ffWork, b));
+      launch_count_learned(c->ffGate + b, c->ffStats + FF_STATS * b, c->s);
       launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
     }
     LAUNCHCHK(c);
@@ -597,6 +625,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   int n = c->stepNo++;
   int r;
   bool mesh = !c->p.field && n % c->p.meshEvery == 0;
+  if (mesh && (r = ff_join(c, c->s))) return r;
   // The pairwise pass reads the positions, masses and boxes and writes only its
   // own buffers, as the seats and the mesh do theirs: on a mesh step it runs on
   // the second stream beside them (a fork and a join, which a captured cycle
@@ -633,14 +662,17 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   b.settle = settle;
   b.kick = kick;
   b.gdt = mesh ? c->p.meshEvery * c->p.dt : 0.0;
-  return sim_pass(c, &b, count);
+  if ((r = sim_pass(c, &b, count))) return r;
+  // a judge forked on this cycle's mesh step is joined by the cycle's end (inside a captured cycle too)
+  return n % c->p.meshEvery == c->p.meshEvery - 1 ? ff_join(c, c->s) : 0;
 }
 
 int cc_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
   if (!c || !a) return CC_EINVAL;
   int r = visit_due(c, c->s);
-  if (r) return r;
-  return one_step(c, settle, kick, a);
+  if (!r) r = one_step(c, settle, kick, a);
+  if (!r) r = ff_join(c, c->s);
+  return r;
 }
 
 // What a captured cycle's launches bake in: the uniforms (params and the
@@ -754,6 +786,7 @@ int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nB
     for (int k = 0; k < chunk; k++) {
       c->box = c->boxHost[k];
       if ((r = visit_due(c, c->s))) return r;
+      if ((r = ff_join(c, c->s))) return r;   // outside any capture
       CycleGraph* g = nullptr;
       if ((r = cycle_graph(c, a, &g))) return r;
       HIPCHK(c, hipGraphLaunch(g->exec, c->s));
@@ -766,6 +799,7 @@ int cc_step_n(cc_ctx* c, int n, const cc_sim_args* a, const double* boxL, int nB
     cycles -= chunk;
   }
   while (i < n) { if ((r = eager())) return r; i++; }
+  if ((r = ff_join(c, c->s))) return r;
   HIPCHK(c, hipEventRecord(c->ev1, c->s));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
@@ -787,6 +821,7 @@ int cc_time_pass(cc_ctx* c, int pass, int reps, double* ms) {
     } else {
       r = pass == 0 ? sim_pass(c, &a) : pass == 1 ? cc_slots(c) : pass == 2 ? mesh_all(c) : pass == 3 ? cc_pp(c)
         : mesh_all(c, pass == 4 ? c->p.meshEvery : 0);
+      if (!r) r = ff_join(c, c->s);   // a judge pass is timed whole
     }
     if (r) return r;
   }
@@ -885,13 +920,17 @@ int cc_set_far_field(cc_ctx* c, cc_ff* f, int judgeEvery, double tol) {
     std::string err;
     c->ffWork = ffwork_create(err);
     if (!c->ffWork) FAIL(c, CC_EDEVICE, "learned far field workspace: %s", err.c_str());
-    ALLOC(c->ffGate, 2 * 4); ALLOC(c->ffStats, 8 * 4); ALLOC(c->ffSums, 4 * 8);
-    ALLOC(c->ffFar, 3 * 4096 * 16); ALLOC(c->ffForce, (size_t)FINE_TEXELS * 16);
+    ALLOC(c->ffGate, 2 * 4); ALLOC(c->ffStats, 2 * FF_STATS * 4); ALLOC(c->ffSums, 4 * 8);
+    ALLOC(c->ffFar, 3 * 4096 * 16);
+    HIPCHK(c, hipEventCreateWithFlags(&c->ffFork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ffJoin, hipEventDisableTiming));
   }
   if (f) {
     HIPCHK(c, hipMemsetAsync(c->ffGate, 0, 8, c->s));
-    HIPCHK(c, hipMemsetAsync(c->ffStats, 0, 32, c->s));
+    HIPCHK(c, hipMemsetAsync(c->ffStats, 0, 2 * FF_STATS * 4, c->s));
     HIPCHK(c, hipMemsetAsync(c->ffSums, 0, 32, c->s));
+    ff_forget_bases(c->ffWork, c->s);
+    LAUNCHCHK(c);
     HIPCHK(c, hipStreamSynchronize(c->s));
   }
   c->ff = f ? ff_net(f) : nullptr;
@@ -902,20 +941,20 @@ int cc_set_far_field(cc_ctx* c, cc_ff* f, int judgeEvery, double tol) {
 
 int cc_ff_stats(cc_ctx* c, double* out) {
   if (!c || !out) return CC_EINVAL;
-  float st[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+  float st[2 * FF_STATS] = {};
   if (c->ffStats) {
-    HIPCHK(c, hipMemcpyAsync(st, c->ffStats, 32, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(st, c->ffStats, sizeof(st), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
   }
-  for (int k = 0; k < 8; k++) out[k] = st[k];
+  for (int k = 0; k < 2 * FF_STATS; k++) out[k] = st[k];
   return 0;
 }
 
 int cc_ff_record(cc_ff* f, cc_ctx* c, int box) {
   if (!f || !c || box < 0 || box > 1) return CC_EINVAL;
   MeshBuf& m = c->mesh[box];
-  if (ff_record(ff_net(f), m.coarse, m.mono, c->boxDev, box, m.farA, m.farT0, m.farT1, c->s) < 0)
-    FAIL(c, CC_ESTATE, "the learned far field's sample record is full");
+  if (ff_record_pass(ff_net(f), m.fine, m.coarse, m.mono, m.farA, m.farT0, m.farT1, c->boxDev, box, c->s) < 0)
+    FAIL(c, CC_ESTATE, "the learned far field's pass record is full");
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->s));
   return 0;
@@ -1272,6 +1311,8 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
         b->ctx[i]->box = b->ctx[i]->boxHost[k];
         if ((r = visit_due(b->ctx[i], b->s))) { b->err = b->ctx[i]->err; return r; }
       }
+      for (size_t i = 0; i < m; i++)   // outside any capture
+        if ((r = ff_join(b->ctx[i], b->s))) { b->err = b->ctx[i]->err; return r; }
       if ((r = batch_graph(b, args))) return r;
       BCHK(b, hipGraphLaunch(b->exec, b->s));
       for (size_t i = 0; i < m; i++) {
@@ -1290,6 +1331,8 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
       b->ctx[i]->s = own;
       if (r) return r;
     }
+  for (size_t i = 0; i < m; i++)
+    if ((r = ff_join(b->ctx[i], b->s))) { b->err = b->ctx[i]->err; return r; }
   BCHK(b, hipEventRecord(b->ev1, b->s));
   BCHK(b, hipEventSynchronize(b->ev1));
   float ms = 0;

@@ -1,7 +1,10 @@
 // The learned far field (no reference counterpart): a 3D convolutional
 // operator on the 16^3 coarse mesh that stands in for the block-to-block far
-// field (FAR_FS, src/glsl-sim.js:263-295): coarse masses and centres of mass in,
-// each block's far-field acceleration, potential and tidal tensor out.
+// field (FAR_FS, src/glsl-sim.js:263-295) on the mesh steps between judges.
+// Its input is the coarse mesh now and its change since the last exact pass
+// (block masses and centre-of-mass offsets); its output, each block's change
+// of far-field acceleration, potential and tidal tensor since that pass, is
+// added to the exact far field recorded there (the mesh interface below).
 //
 // Network (weights FP32 masters, BF16 on the matrix cores, FP32 accumulation):
 //   4 levels 16^3 / 8^3 / 4^3 / 2^3, 32 channels, 3x3x3 convolutions, leaky
@@ -29,6 +32,7 @@
 #include <vector>
 
 #include "../../include/cc_hip.h"
+#include "cc_fpmath.h"
 #include "cc_internal.h"
 
 namespace cc {
@@ -42,17 +46,33 @@ __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// a wavefront's sum, in lane 0
+template <typename T>
+__device__ __forceinline__ T ff_wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
 
 // ------------------------------------------------------------ convolution --
 // in  [M voxels][CIN] bf16 (M = B*S^3), wt [27][CIN/16][COUT][16] bf16,
 // bias [COUT] f32 or null; slope: the leaky ReLU's (1: none).  Writes out (bf16, [M][COUT])
 // and/or outf (f32), and pre (f32 pre-activation) when asked.
+// The work-group stages the layer's weights in LDS once; each wavefront issues
+// all 27 taps' activation loads before its matrix-core chain, so a launch pays
+// about one memory latency instead of one per tap (what bounds a single
+// instance's 16^3 mesh, whose widest layer is 256 wavefronts).  The chain adds
+// taps and channel steps in the same order either way.
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_conv(const uint16_t* __restrict__ in, int B, int S,
                                               const uint16_t* __restrict__ wt, const float* __restrict__ bias,
                                               float slope, uint16_t* out, float* outf, float* pre,
                                               const int* __restrict__ gate) {
-  if (gate && *gate == 0) return;
+  if (gate && *gate == 0) return;   // the same for every work-group: before the barrier
+  constexpr int WN = 27 * (CIN / 16) * COUT * 16;
+  __shared__ __align__(16) uint16_t wl[WN];
+  for (int i = threadIdx.x; i < WN / 8; i += blockDim.x)
+    reinterpret_cast<uint4*>(wl)[i] = reinterpret_cast<const uint4*>(wt)[i];
+  __syncthreads();
   int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   int lane = threadIdx.x & 63;
   int S3 = S * S * S, M = B * S3;
@@ -63,22 +83,31 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t* __restrict__ in, i
   bool vin = v < M;
   int b = v / S3, r = v - b * S3;
   int z = r / (S * S), y = (r / S) % S, x = r % S;
-  f32x4 acc[COUT / 16];
-  for (int n = 0; n < COUT / 16; n++) acc[n] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
   const bf16x4 zero = bf16x4{ 0, 0, 0, 0 };
+  bf16x4 av[27][CIN / 16];
+#pragma unroll
   for (int t = 0; t < 27; t++) {
     int zz = z + t / 9 - 1, yy = y + (t / 3) % 3 - 1, xx = x + t % 3 - 1;
     bool ok = vin && zz >= 0 && zz < S && yy >= 0 && yy < S && xx >= 0 && xx < S;
     size_t src = ok ? ((size_t)b * S3 + ((size_t)zz * S + yy) * S + xx) * CIN : 0;
+#pragma unroll
+    for (int kc = 0; kc < CIN / 16; kc++)
+      av[t][kc] = ok ? *reinterpret_cast<const bf16x4*>(in + src + kc * 16 + kq) : zero;
+  }
+  f32x4 acc[COUT / 16];
+#pragma unroll
+  for (int n = 0; n < COUT / 16; n++) acc[n] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+  for (int t = 0; t < 27; t++)
+#pragma unroll
     for (int kc = 0; kc < CIN / 16; kc++) {
-      bf16x4 a = ok ? *reinterpret_cast<const bf16x4*>(in + src + kc * 16 + kq) : zero;
-      const uint16_t* wp = wt + ((size_t)(t * (CIN / 16) + kc) * COUT + row) * 16 + kq;
+      const uint16_t* wp = wl + ((t * (CIN / 16) + kc) * COUT + row) * 16 + kq;
+#pragma unroll
       for (int n = 0; n < COUT / 16; n++) {
-        bf16x4 w = *reinterpret_cast<const bf16x4*>(wp + (size_t)n * 16 * 16);
-        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc[n], 0, 0, 0);
+        bf16x4 w = *reinterpret_cast<const bf16x4*>(wp + n * 16 * 16);
+        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[t][kc], w, acc[n], 0, 0, 0);
       }
     }
-  }
   for (int n = 0; n < COUT / 16; n++) {
     int co = n * 16 + row;
     float bb = bias ? bias[co] : 0.0f;
@@ -235,16 +264,34 @@ __global__ void k_zero_f32(float* a, size_t n) {
 }
 
 // ------------------------------------------------------ the mesh interface --
-// Network input from a coarse mesh (64x64 atlas of 4x4 slices of 16^2; xyz
-// mass-weighted world positions): each block's mass in units of the box's
-// total mass times 64, and its dipole about the block's centre (that mass
-// times the centre of mass less the centre, in block lengths, 4 FH), which
-// is what the far field of the block depends on to first order; channels
-// 4..15 zero.  The network's output is the far field at the block's centre;
-// the centre of mass, where the per-cell pass wants it, is reached by the
-// tidal tensor (acceleration) and the acceleration (potential).
+// The network predicts how each block's far field has changed since the last
+// exact pass (the base), from how the coarse mesh has changed since then.
+// Every judge step (the exact far field is computed there anyway) makes that
+// pass the new base; the mesh steps in between take the base plus the
+// network's change.  Over the 8 or 16 steps a prediction spans, particles
+// move a few hundredths of a block, so the change is a small part of the far
+// field and the network has to get only that part right.
+//
+// Voxel v = (z*16 + y)*16 + x indexes the 16^3 blocks as the network's
+// activations do; block (x, y, z) is coarse texel coarse_texel(x, y, z).
+//
+// Features of a pass, per block: its mass in units of the box's total mass
+// times 64, and that times its centre of mass less its centre in block
+// lengths (4 FH); meta: the box's corner offset L (xyz) and total mass.
+//
+// Centre record of a pass, FF_R floats per block: acceleration (3),
+// potential, tidal tensor (xx yy zz xy xz yz), the block's occupied fine
+// texels, one spare.  An occupied block's record is its far field, which the
+// exact pass gives at its centre of mass, carried to the block's centre to
+// first order by the tidal tensor -- the inverse of what k_ff_out does, so
+// that a record carried back by the same tensor returns the far field it came
+// from.  Empty blocks: zero.  A block that fills after the base has no record
+// to start from: k_far computes its far field exactly on the mesh steps the
+// network stands in (there are few), and the network's output there is unused.
 __device__ __forceinline__ int coarse_texel(int x, int y, int z) { return ((z >> 2) * 16 + y) * 64 + (z & 3) * 16 + x; }
 constexpr float BLOCK_LEN = (float)(4.0 * FH);
+constexpr int FF_R = 12;
+constexpr int FF_IN = 10;   // input channels in use (of 16)
 // the block's centre of mass less its centre, world units
 __device__ __forceinline__ void block_offset(float4 c, int x, int y, int z, const float* L, float& dx, float& dy,
                                              float& dz) {
@@ -252,10 +299,8 @@ __device__ __forceinline__ void block_offset(float4 c, int x, int y, int z, cons
   dy = c.y / c.w - ((4 * y + 2) * (float)FH - L[1]);
   dz = c.z / c.w - ((4 * z + 2) * (float)FH - L[2]);
 }
-__global__ void k_ff_in(const float4* __restrict__ coarse, const float4* __restrict__ mono,
-                        const Box* __restrict__ bxp, int which, int inst, uint16_t* in, float* scale,
-                        const int* __restrict__ gate) {
-  if (gate && *gate == 0) return;
+__global__ void k_ff_feat(const float4* __restrict__ coarse, const float4* __restrict__ mono,
+                          const Box* __restrict__ bxp, int which, float4* feat, float* meta) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= 4096) return;
   int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
@@ -263,26 +308,150 @@ __global__ void k_ff_in(const float4* __restrict__ coarse, const float4* __restr
   const float* L = which ? bx.LB : bx.LA;
   float4 c = coarse[coarse_texel(x, y, z)];
   float mt = fmaxf(mono[0].w, 1e-20f);
-  uint16_t* o = in + ((size_t)inst * 4096 + v) * 16;
-  float f[4] = { 0, 0, 0, 0 };
+  float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (c.w > 0.0f) {
     float dx, dy, dz;
     block_offset(c, x, y, z, L, dx, dy, dz);
-    f[0] = c.w / mt * 64.0f;
-    f[1] = f[0] * dx / BLOCK_LEN;
-    f[2] = f[0] * dy / BLOCK_LEN;
-    f[3] = f[0] * dz / BLOCK_LEN;
+    f.x = c.w / mt * 64.0f;
+    f.y = f.x * dx / BLOCK_LEN;
+    f.z = f.x * dy / BLOCK_LEN;
+    f.w = f.x * dz / BLOCK_LEN;
   }
-  for (int k = 0; k < 16; k++) o[k] = f2bf(k < 4 ? f[k] : 0.0f);
-  if (v == 0) scale[inst] = mt;
+  feat[v] = f;
+  if (v == 0) { meta[0] = L[0]; meta[1] = L[1]; meta[2] = L[2]; meta[3] = mt; }
 }
-// Output scales: acceleration, potential and tidal tensor in units of the
-// total mass over the block length squared, to the first and cubed.
+
+// A pass's centre record (one wavefront per block: its 64 fine texels counted
+// by ballot).  With featT given the pass becomes a box's base: its features
+// and meta are copied to featB and metaB, and *validB set.
+__global__ void __launch_bounds__(256) k_ff_base(const float4* __restrict__ fine, const float4* __restrict__ coarse,
+                                                 const float4* __restrict__ farA, const float4* __restrict__ farT0,
+                                                 const float4* __restrict__ farT1, const Box* __restrict__ bxp,
+                                                 int which, float* R, const float4* __restrict__ featT,
+                                                 const float* __restrict__ metaT, float4* featB, float* metaB,
+                                                 int* validB) {
+  int lane = threadIdx.x & 63;
+  int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= 4096) return;
+  int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
+  if (featT && lane == 0) featB[v] = featT[v];
+  if (featT && v == 0 && lane == 0) {
+    for (int k = 0; k < 4; k++) metaB[k] = metaT[k];
+    *validB = 1;
+  }
+  float* o = R + (size_t)v * FF_R;
+  int t = coarse_texel(x, y, z);
+  float4 c = coarse[t];
+  if (c.w <= 0.0f) {
+    if (lane < FF_R) o[lane] = 0.0f;
+    return;
+  }
+  const int cx = 4 * x + (lane & 3), cy = 4 * y + ((lane >> 2) & 3), cz = 4 * z + (lane >> 4);
+  const unsigned long long filled = __ballot(fine[((cz >> 3) * 64 + cy) * 512 + (cz & 7) * 64 + cx].w > 0.0f);
+  if (lane == 0) {
+    const Box bx = *bxp;
+    float dx, dy, dz;
+    block_offset(c, x, y, z, which ? bx.LB : bx.LA, dx, dy, dz);
+    float4 A = farA[t], T0 = farT0[t], T1 = farT1[t];
+    float ax = A.x - (T0.x * dx + T0.w * dy + T1.x * dz), ay = A.y - (T0.w * dx + T0.y * dy + T1.y * dz),
+          az = A.z - (T1.x * dx + T1.y * dy + T0.z * dz);
+    float r[FF_R] = { ax, ay, az, A.w + (ax * dx + ay * dy + az * dz), T0.x, T0.y, T0.z, T0.w, T1.x, T1.y,
+                      (float)__popcll(filled), 0.0f };
+    for (int k = 0; k < FF_R; k++) o[k] = r[k];
+  }
+}
+
+// The network's input for a pair of passes, the present one (T) and the base
+// (B), by voxel: channels 0-3 the present features, 4-7 their change since the
+// base over s, 8 and 9 the block's occupancy at T and at B, the rest zero.
+// s (to scale[0]) is the root mean square of the change's four channels over
+// the blocks occupied at T or B, or 1 when nothing changed.  One work-group.
+// With ok given, whether the base can serve -- recorded, and the box's corner
+// within a quarter block of where it was -- goes to *ok, and a given gate is
+// cleared when it cannot.
+constexpr int PAIR_BLOCK = 1024;
+__global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(const float4* __restrict__ featT, const float* __restrict__ metaT,
+                                                           const float4* __restrict__ featB, const float* __restrict__ metaB,
+                                                           const int* __restrict__ validB, uint16_t* x0, float* scale,
+                                                           int* ok, int* gate) {
+  __shared__ float redS[PAIR_BLOCK / 64];
+  __shared__ int redN[PAIR_BLOCK / 64];
+  const int tid = threadIdx.x;
+  float4 fT[4096 / PAIR_BLOCK], fB[4096 / PAIR_BLOCK];
+  float ss = 0.0f;
+  int cnt = 0;
+  for (int j = 0; j < 4096 / PAIR_BLOCK; j++) {
+    int v = tid + PAIR_BLOCK * j;
+    fT[j] = featT[v];
+    fB[j] = featB[v];
+    if (fT[j].x > 0.0f || fB[j].x > 0.0f) {
+      float a = fT[j].x - fB[j].x, b = fT[j].y - fB[j].y, c = fT[j].z - fB[j].z, d = fT[j].w - fB[j].w;
+      ss += (a * a + b * b) + (c * c + d * d);
+      cnt++;
+    }
+  }
+  ss = ff_wave_sum(ss);
+  cnt = ff_wave_sum(cnt);
+  if ((tid & 63) == 0) { redS[tid >> 6] = ss; redN[tid >> 6] = cnt; }
+  __syncthreads();
+  ss = 0.0f;
+  cnt = 0;
+  for (int w = 0; w < PAIR_BLOCK / 64; w++) { ss += redS[w]; cnt += redN[w]; }
+  const float s = ss > 0.0f ? sqrtf(ss / (4.0f * (float)cnt)) : 1.0f;
+  for (int j = 0; j < 4096 / PAIR_BLOCK; j++) {
+    int v = tid + PAIR_BLOCK * j;
+    float ch[10] = { fT[j].x, fT[j].y, fT[j].z, fT[j].w, (fT[j].x - fB[j].x) / s, (fT[j].y - fB[j].y) / s,
+                     (fT[j].z - fB[j].z) / s, (fT[j].w - fB[j].w) / s, fT[j].x > 0.0f ? 1.0f : 0.0f,
+                     fB[j].x > 0.0f ? 1.0f : 0.0f };
+    uint16_t* o = x0 + (size_t)v * 16;
+    for (int k = 0; k < 16; k++) o[k] = k < 10 ? f2bf(ch[k]) : (uint16_t)0;
+  }
+  if (tid == 0) {
+    scale[0] = s;
+    if (ok) {
+      const float q = 0.25f * BLOCK_LEN;
+      int good = (!validB || *validB) && fabsf(metaT[0] - metaB[0]) <= q && fabsf(metaT[1] - metaB[1]) <= q &&
+                 fabsf(metaT[2] - metaB[2]) <= q;
+      *ok = good;
+      if (!good && gate) *gate = 0;
+    }
+  }
+}
+
+// Output and target units: the change of a block's record per unit of the
+// input's change is that of a mass s * (total mass) / 64; accelerations in
+// that mass over the block length squared, potentials over the block length,
+// tidal tensors over its cube.
 constexpr float FF_A = 1.0f / (BLOCK_LEN * BLOCK_LEN), FF_P = 1.0f / BLOCK_LEN,
                 FF_T = 1.0f / (BLOCK_LEN * BLOCK_LEN * BLOCK_LEN);
-__global__ void k_ff_out(const float* __restrict__ y, const float4* __restrict__ coarse, const float* __restrict__ scale,
-                         const Box* __restrict__ bxp, int which, int inst, float4* farA, float4* farT0, float4* farT1,
-                         const int* __restrict__ gate) {
+__device__ __forceinline__ float ff_unit(int k) { return k < 3 ? FF_A : (k == 3 ? FF_P : FF_T); }
+
+// The training target of a pair: the change of each block's record since the
+// base in the output's units, on the blocks occupied at T and at B (a block
+// that fills after the base gets the exact far field); the mask weighs each by
+// its occupied fine texels at T over 64, as the judge weighs the texels.
+__global__ void k_ff_pair_tgt(const float* __restrict__ RT, const float* __restrict__ RB,
+                              const float4* __restrict__ featT, const float4* __restrict__ featB,
+                              const float* __restrict__ metaT, const float* __restrict__ scale, float* tgt,
+                              float* mask) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 4096) return;
+  bool occ = featT[v].x > 0.0f && featB[v].x > 0.0f;
+  float u = scale[0] * metaT[3] / 64.0f;
+  float* o = tgt + (size_t)v * 16;
+  for (int k = 0; k < 16; k++)
+    o[k] = occ && k < 10 ? (RT[(size_t)v * FF_R + k] - RB[(size_t)v * FF_R + k]) / (u * ff_unit(k)) : 0.0f;
+  mask[v] = occ ? RT[(size_t)v * FF_R + 10] * (1.0f / 64.0f) : 0.0f;
+}
+
+// The network's far field: each occupied block's base record plus the
+// network's change, carried from the block's centre to its present centre of
+// mass by the tidal tensor; empty blocks zero; blocks empty at the base left
+// to k_far.  Skips unless *gate, when given.
+__global__ void k_ff_out(const float* __restrict__ y, const float* __restrict__ RB, const float4* __restrict__ featB,
+                         const float* __restrict__ scale, const float4* __restrict__ coarse,
+                         const float* __restrict__ metaT, const Box* __restrict__ bxp, int which, float4* farA,
+                         float4* farT0, float4* farT1, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= 4096) return;
@@ -293,61 +462,39 @@ __global__ void k_ff_out(const float* __restrict__ y, const float4* __restrict__
     farA[t] = make_float4(0, 0, 0, 0); farT0[t] = make_float4(0, 0, 0, 0); farT1[t] = make_float4(0, 0, 0, 0);
     return;
   }
+  if (featB[v].x <= 0.0f) return;   // filled after the base: k_far's
   const Box bx = *bxp;
   float dx, dy, dz;
   block_offset(c, x, yv, z, which ? bx.LB : bx.LA, dx, dy, dz);
-  const float* o = y + ((size_t)inst * 4096 + v) * 16;
-  float m = scale[inst];
-  float ax = o[0] * m * FF_A, ay = o[1] * m * FF_A, az = o[2] * m * FF_A, p = o[3] * m * FF_P;
-  float txx = o[4] * m * FF_T, tyy = o[5] * m * FF_T, tzz = o[6] * m * FF_T, txy = o[7] * m * FF_T;
-  float txz = o[8] * m * FF_T, tyz = o[9] * m * FF_T;
-  // from the block's centre to its centre of mass
+  const float* o = y + (size_t)v * 16;
+  const float* r = RB + (size_t)v * FF_R;
+  const float u = scale[0] * metaT[3] / 64.0f, uA = u * FF_A, uP = u * FF_P, uT = u * FF_T;
+  float ax = r[0] + o[0] * uA, ay = r[1] + o[1] * uA, az = r[2] + o[2] * uA, p = r[3] + o[3] * uP;
+  float txx = r[4] + o[4] * uT, tyy = r[5] + o[5] * uT, tzz = r[6] + o[6] * uT, txy = r[7] + o[7] * uT;
+  float txz = r[8] + o[8] * uT, tyz = r[9] + o[9] * uT;
   float sx = ax + txx * dx + txy * dy + txz * dz, sy = ay + txy * dx + tyy * dy + tyz * dz,
         sz = az + txz * dx + tyz * dy + tzz * dz;
   farA[t] = make_float4(sx, sy, sz, p - (ax * dx + ay * dy + az * dz));
   farT0[t] = make_float4(txx, tyy, tzz, txy);
   farT1[t] = make_float4(txz, tyz, 0.0f, 0.0f);
 }
-// Training target from the exact far field, in the output's units; the mask
-// marks occupied blocks.
-__global__ void k_ff_target(const float4* __restrict__ coarse, const float4* __restrict__ farA,
-                            const float4* __restrict__ farT0, const float4* __restrict__ farT1,
-                            const float4* __restrict__ mono, const Box* __restrict__ bxp, int which, int inst,
-                            float* tgt, float* mask) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= 4096) return;
-  int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
-  int t = coarse_texel(x, y, z);
-  float* o = tgt + ((size_t)inst * 4096 + v) * 16;
-  float m = fmaxf(mono[0].w, 1e-20f);
-  float4 c = coarse[t];
-  bool occ = c.w > 0.0f;
-  float4 A = farA[t], T0 = farT0[t], T1 = farT1[t];
-  // the exact far field at the centre of mass, taken back to the block's centre
-  float dx = 0, dy = 0, dz = 0;
-  if (occ) {
-    const Box bx = *bxp;
-    block_offset(c, x, y, z, which ? bx.LB : bx.LA, dx, dy, dz);
-  }
-  float ax = A.x - (T0.x * dx + T0.w * dy + T1.x * dz), ay = A.y - (T0.w * dx + T0.y * dy + T1.y * dz),
-        az = A.z - (T1.x * dx + T1.y * dy + T0.z * dz);
-  float p = A.w + (ax * dx + ay * dy + az * dz);
-  float vals[10] = { ax / (m * FF_A), ay / (m * FF_A), az / (m * FF_A), p / (m * FF_P), T0.x / (m * FF_T),
-                     T0.y / (m * FF_T), T0.z / (m * FF_T), T0.w / (m * FF_T), T1.x / (m * FF_T), T1.y / (m * FF_T) };
-  for (int k = 0; k < 16; k++) o[k] = occ && k < 10 ? vals[k] : 0.0f;
-  mask[(size_t)inst * 4096 + v] = occ ? 1.0f : 0.0f;
-}
-// Masked mean squared error over the 10 used channels and its gradient;
-// loss accumulated in double.
+
+// The loss's weight of output channel c: accelerations 1, the potential and
+// the tidal tensor 1/4 (the judge sees them only through the tidal term's
+// short lever), the unused channels 0.
+__device__ __forceinline__ float ff_chan_weight(int c) { return c < 3 ? 1.0f : (c < 10 ? 0.25f : 0.0f); }
+// Weighted mean squared error (each voxel by its mask, each channel by
+// ff_chan_weight) and its gradient; loss accumulated in double.
 __global__ void k_ff_loss(const float* __restrict__ y, const float* __restrict__ tgt, const float* __restrict__ mask,
                           int M, float invCount, float* gy, double* loss) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.0f;
   if (e < M * 16) {
     int v = e >> 4, c = e & 15;
-    float d = (c < 10 && mask[v] > 0.0f) ? y[e] - tgt[e] : 0.0f;
-    gy[e] = 2.0f * d * invCount;
-    l = d * d * invCount;
+    float w = mask[v] * ff_chan_weight(c);
+    float d = w > 0.0f ? y[e] - tgt[e] : 0.0f;
+    gy[e] = 2.0f * w * d * invCount;
+    l = w * d * d * invCount;
   }
   for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
   if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss, (double)l);
@@ -362,30 +509,68 @@ __global__ void k_adam(float* w, const float* __restrict__ g, float* m, float* v
   m[e] = mm; v[e] = vv;
   w[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
-// The judge: |F_learned - F_exact| / |F_exact| over the force mesh's
-// accelerations, accumulated in double.
-__global__ void k_judge_sum(const float4* __restrict__ a, const float4* __restrict__ b, int n, double* sums) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// The judge's sums over the force mesh's texels (scratch row left out):
+// [0] |learned - exact far-field part|^2, [1] |exact force|^2, in double.  A
+// texel's far-field part is its block's far acceleration carried by the
+// tidal tensor to the texel's centre of mass, times G m, as k_cell adds it;
+// the near part is the same in both, so the difference of the two force
+// meshes is the difference of their far parts.  Blocks empty at the base
+// count no difference: the mesh steps the network stands in give them k_far's.
+constexpr int JUDGE_BLOCKS = 128;   // work-groups of 256 striding over the texels: one atomic pair each
+__global__ void __launch_bounds__(256) k_ff_judge_sum(const float4* __restrict__ com, const float4* __restrict__ ccom,
+                                                      const float4* __restrict__ Al, const float4* __restrict__ T0l,
+                                                      const float4* __restrict__ T1l, const float4* __restrict__ Ae,
+                                                      const float4* __restrict__ T0e, const float4* __restrict__ T1e,
+                                                      const float4* __restrict__ force,
+                                                      const float4* __restrict__ featB, float gm, double* sums) {
+  __shared__ float redD[4], redR[4];
   float d2 = 0.0f, r2 = 0.0f;
-  if (e < n) {
-    float4 x = a[e], y = b[e];
-    float dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z;
-    d2 = dx * dx + dy * dy + dz * dz;
-    r2 = y.x * y.x + y.y * y.y + y.z * y.z;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 512 * 512; e += gridDim.x * blockDim.x) {
+    float4 F = force[e];
+    r2 += F.x * F.x + F.y * F.y + F.z * F.z;
+    float4 mc = com[e];
+    int mx = e & 511, my = e >> 9;
+    int c0 = mx & 63, c1 = my & 63, c2 = (my >> 6) * 8 + (mx >> 6);
+    if (mc.w > 0.0f && featB[((c2 >> 2) * 16 + (c1 >> 2)) * 16 + (c0 >> 2)].x > 0.0f) {
+      int ca = coarse_texel(c0 >> 2, c1 >> 2, c2 >> 2);
+      float4 mb = ccom[ca];
+      float q0 = mc.x - mb.x, q1 = mc.y - mb.y, q2 = mc.z - mb.z;
+      float4 a = Al[ca], b = Ae[ca], s0 = T0l[ca], t0 = T0e[ca], s1 = T1l[ca], t1 = T1e[ca];
+      float u0 = s0.x - t0.x, u1 = s0.y - t0.y, u2 = s0.z - t0.z, u3 = s0.w - t0.w, u4 = s1.x - t1.x, u5 = s1.y - t1.y;
+      float dx = (a.x - b.x) + u0 * q0 + u3 * q1 + u4 * q2;
+      float dy = (a.y - b.y) + u3 * q0 + u1 * q1 + u5 * q2;
+      float dz = (a.z - b.z) + u4 * q0 + u5 * q1 + u2 * q2;
+      d2 += (dx * dx + dy * dy + dz * dz) * (gm * gm);
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) { d2 += __shfl_xor(d2, o); r2 += __shfl_xor(r2, o); }
-  if ((threadIdx.x & 63) == 0) { atomicAdd(&sums[0], (double)d2); atomicAdd(&sums[1], (double)r2); }
+  d2 = ff_wave_sum(d2);
+  r2 = ff_wave_sum(r2);
+  if ((threadIdx.x & 63) == 0) { redD[threadIdx.x >> 6] = d2; redR[threadIdx.x >> 6] = r2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[0], (double)redD[0] + (double)redD[1] + (double)redD[2] + (double)redD[3]);
+    atomicAdd(&sums[1], (double)redR[0] + (double)redR[1] + (double)redR[2] + (double)redR[3]);
+  }
 }
-// stats: [0] last error, [1] judged, [2] fallbacks, [3] learned mesh steps;
-// flag: 1 -> the next mesh steps use the network
-__global__ void k_judge_decide(double* sums, float tol, float* stats, int* flag) {
+// The judge's verdict for the coming mesh steps (the sums put back to zero):
+// the network stands in while its relative error is within tol and it was
+// measured from a base that could serve.  stats: [0] last error, [1] judged,
+// [2] fallbacks, [3] learned mesh steps, [4] largest error measured from a
+// base that could serve, [5] their sum, [6] their count.
+__global__ void k_judge_decide(double* sums, float tol, const int* __restrict__ okBase, float* stats, int* gate) {
   if (threadIdx.x != 0) return;
   double err = sums[1] > 0.0 ? sqrt(sums[0] / sums[1]) : 0.0;
   stats[0] = (float)err;
   stats[1] += 1.0f;
-  int ok = err <= tol;
+  int usable = *okBase;
+  int ok = usable && err <= tol;
   if (!ok) stats[2] += 1.0f;
-  *flag = ok;
+  if (usable) {
+    stats[4] = fmaxf(stats[4], (float)err);
+    stats[5] += (float)err;
+    stats[6] += 1.0f;
+  }
+  *gate = ok;
   sums[0] = 0.0; sums[1] = 0.0;
 }
 __global__ void k_count_learned(const int* flag, float* stats) {
@@ -397,8 +582,9 @@ __global__ void k_zero_d(double* p, int n) {
 }
 
 
-// relative error sums of a forward pass: [0] |dA|^2 [1] |A|^2 over the
-// acceleration channels, [2] |dY|^2 [3] |Y|^2 over all ten
+// relative error sums of a forward pass, each voxel weighted by its mask:
+// [0] |dA|^2 [1] |A|^2 over the acceleration channels, [2] |dY|^2 [3] |Y|^2
+// over all ten
 __global__ void k_eval_sum(const float* __restrict__ y, const float* __restrict__ tgt, const float* __restrict__ mask,
                            int M, double* sums) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -406,8 +592,8 @@ __global__ void k_eval_sum(const float* __restrict__ y, const float* __restrict_
   if (e < M * 16) {
     int v = e >> 4, c = e & 15;
     if (c < 10 && mask[v] > 0.0f) {
-      float d = y[e] - tgt[e], t = tgt[e];
-      s2 = d * d; s3 = t * t;
+      float d = y[e] - tgt[e], t = tgt[e], w = mask[v];
+      s2 = w * d * d; s3 = w * t * t;
       if (c < 3) { s0 = s2; s1 = s3; }
     }
   }
@@ -462,7 +648,12 @@ struct FFNet {
   uint16_t* dataIn = nullptr;   // [cap][4096][16]
   float* dataTgt = nullptr;     // [cap][4096][16]
   float* dataMask = nullptr;    // [cap][4096]
-  float* dataScale = nullptr;   // [cap] total mass of each sample's box
+  float* dataScale = nullptr;   // [cap] the input's scale s of each sample
+  // recorded passes of the exact far field, from which samples are paired
+  int passes = 0;
+  float4* passFeat = nullptr;   // [cap][4096]
+  float* passMeta = nullptr;    // [cap][4]
+  float* passR = nullptr;       // [cap][4096][FF_R]
   Acts train;                   // batch activations for training
   int* idx = nullptr;
   int idxCap = 0;
@@ -473,8 +664,15 @@ struct FFNet {
   std::string err;
 };
 
+// A ctx's workspace: the inference activations, and per box the present
+// pass's features and the base (features, meta, centre record, whether one is
+// recorded, whether it could serve the last prediction).
 struct FFWork {
   Acts a;
+  float4 *featT[2], *featB[2];
+  float *metaT[2], *metaB[2], *RB[2];
+  int *validB[2], *okBase[2];
+  std::vector<void*> allocs;
 };
 
 static bool dev_alloc(std::vector<void*>& list, void** p, size_t bytes, std::string& err) {
@@ -658,6 +856,9 @@ static bool ff_init(FFNet* n, int cap, std::string& err) {
     DA(n->allocs, n->dataTgt, (size_t)cap * 4096 * 16 * 4);
     DA(n->allocs, n->dataMask, (size_t)cap * 4096 * 4);
     DA(n->allocs, n->dataScale, (size_t)cap * 4);
+    DA(n->allocs, n->passFeat, (size_t)cap * 4096 * 16);
+    DA(n->allocs, n->passMeta, (size_t)cap * 4 * 4);
+    DA(n->allocs, n->passR, (size_t)cap * 4096 * FF_R * 4);
   }
   hipError_t e = hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking);
   if (e != hipSuccess) { err = hipGetErrorString(e); return false; }
@@ -667,12 +868,12 @@ static bool ff_init(FFNet* n, int cap, std::string& err) {
   auto rnd = [&]() { st ^= st << 13; st ^= st >> 17; st ^= st << 5; return (st >> 8) * (1.0f / 16777216.0f); };
   for (int l = 0; l < 12; l++) {
     const Layer& L = n->L[l];
-    float fan = 27.0f * (l == 0 ? 4.0f : (float)L.cin);
+    float fan = 27.0f * (l == 0 ? (float)FF_IN : (float)L.cin);
     float a = sqrtf(3.0f * (L.act ? 2.0f : 1.0f) / fan);
     for (int t = 0; t < 27; t++)
       for (int ci = 0; ci < L.cin; ci++)
         for (int co = 0; co < L.cout; co++)
-          w[L.w + ((size_t)t * L.cin + ci) * L.cout + co] = (l == 0 && ci >= 4) ? 0.0f : a * (2.0f * rnd() - 1.0f);
+          w[L.w + ((size_t)t * L.cin + ci) * L.cout + co] = (l == 0 && ci >= FF_IN) ? 0.0f : a * (2.0f * rnd() - 1.0f);
   }
   e = hipMemcpyAsync(n->w, w.data(), off * 4, hipMemcpyHostToDevice, n->s);
   if (e == hipSuccess) e = hipMemsetAsync(n->m, 0, off * 4, n->s);
@@ -696,36 +897,107 @@ void ff_destroy(FFNet* n) {
   if (n->s) hipStreamDestroy(n->s);
   delete n;
 }
+static bool work_alloc(FFWork* w, std::string& err) {
+  for (int b = 0; b < 2; b++) {
+    DA(w->allocs, w->featT[b], 4096 * 16); DA(w->allocs, w->featB[b], 4096 * 16);
+    DA(w->allocs, w->metaT[b], 16); DA(w->allocs, w->metaB[b], 16);
+    DA(w->allocs, w->RB[b], (size_t)4096 * FF_R * 4);
+    DA(w->allocs, w->validB[b], 4); DA(w->allocs, w->okBase[b], 4);
+  }
+  for (int b = 0; b < 2; b++)   // no base yet: zero features and records, not whatever the allocation held
+    if (hipMemset(w->featB[b], 0, 4096 * 16) != hipSuccess || hipMemset(w->RB[b], 0, (size_t)4096 * FF_R * 4) != hipSuccess ||
+        hipMemset(w->featT[b], 0, 4096 * 16) != hipSuccess || hipMemset(w->metaB[b], 0, 16) != hipSuccess) {
+      err = "hipMemset";
+      return false;
+    }
+  return true;
+}
 FFWork* ffwork_create(std::string& err) {
   FFWork* w = new FFWork();
-  if (!acts_alloc(w->a, 1, false, err)) { ffwork_destroy(w); return nullptr; }
+  if (!acts_alloc(w->a, 1, false, err) || !work_alloc(w, err)) { ffwork_destroy(w); return nullptr; }
   return w;
 }
 void ffwork_destroy(FFWork* w) {
   if (!w) return;
   acts_free(w->a);
+  for (void* p : w->allocs) hipFree(p);
   delete w;
 }
-
-// inference for one box of one instance into farA/T0/T1; every kernel does
-// nothing unless *gate (when given) is nonzero
-void ff_infer(const FFNet* n, FFWork* wk, const float4* coarse, const float4* mono, const Box* bx, int which,
-              float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
-  Acts& a = wk->a;
-  hipLaunchKernelGGL(k_ff_in, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, 0, a.x0, a.scale, gate);
-  forward(n, a, 1, false, gate, s);
-  hipLaunchKernelGGL(k_ff_out, dim3(16), dim3(256), 0, s, a.y, coarse, a.scale, bx, which, 0, farA, farT0, farT1,
-                     gate);
+__global__ void k_fill_i32(int* p, int n, int v) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = v;
+}
+// both boxes' bases forgotten: the next judge of each records one, and no
+// mesh step stands in before it (a fill kernel, so a caller may capture it)
+const float4* ff_base_features(const FFWork* w, int which) { return w->featB[which]; }
+void ff_forget_bases(FFWork* w, hipStream_t s) {
+  for (int b = 0; b < 2; b++) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, w->validB[b], 1, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, w->okBase[b], 1, 0);
+  }
 }
 
-// one training sample from a box's coarse mesh and its exact far field
-int ff_record(FFNet* n, const float4* coarse, const float4* mono, const Box* bx, int which, const float4* farA,
-              const float4* farT0, const float4* farT1, hipStream_t s) {
+// The network's far field for box `which` at a mesh step: the present
+// pass's features, the input pair against the box's base, the forward pass and
+// the output into farA/T0/T1.  gate (the mesh steps between judges): cleared
+// when the base cannot serve, and the forward pass and the output skip unless
+// it is set.  Without a gate (a judge step) everything runs.
+void ff_predict(const FFNet* n, FFWork* wk, int which, const float4* coarse, const float4* mono, const Box* bx,
+                float4* farA, float4* farT0, float4* farT1, int* gate, hipStream_t s) {
+  Acts& a = wk->a;
+  hipLaunchKernelGGL(k_ff_feat, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, wk->featT[which], wk->metaT[which]);
+  hipLaunchKernelGGL(k_ff_pair_in, dim3(1), dim3(PAIR_BLOCK), 0, s, wk->featT[which], wk->metaT[which],
+                     wk->featB[which], wk->metaB[which], wk->validB[which], a.x0, a.scale, wk->okBase[which], gate);
+  forward(n, a, 1, false, gate, s);
+  hipLaunchKernelGGL(k_ff_out, dim3(16), dim3(256), 0, s, a.y, wk->RB[which], wk->featB[which], a.scale, coarse,
+                     wk->metaT[which], bx, which, farA, farT0, farT1, gate);
+}
+
+// A judge step, after ff_predict (into learned: farA, farT0, farT1 of 4096
+// each) and the exact far field and force mesh: the judge's sums and verdict
+// (gate, stats), then the exact pass becomes the box's base.
+void ff_judge_rebase(FFWork* wk, int which, const float4* fine, const float4* com, const float4* ccom,
+                     const float4* coarse, const float4* learned, const float4* farA, const float4* farT0,
+                     const float4* farT1, const float4* force, const Box* bx, float gm, double* sums, float tol,
+                     float* stats, int* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_ff_judge_sum, dim3(JUDGE_BLOCKS), dim3(256), 0, s, com, ccom, learned, learned + 4096,
+                     learned + 8192, farA, farT0, farT1, force, wk->featB[which], gm, sums);
+  hipLaunchKernelGGL(k_judge_decide, dim3(1), dim3(64), 0, s, sums, tol, wk->okBase[which], stats, gate);
+  hipLaunchKernelGGL(k_ff_base, dim3(1024), dim3(256), 0, s, fine, coarse, farA, farT0, farT1, bx, which,
+                     wk->RB[which], wk->featT[which], wk->metaT[which], wk->featB[which], wk->metaB[which],
+                     wk->validB[which]);
+}
+void launch_count_learned(const int* gate, float* stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_learned, dim3(1), dim3(64), 0, s, gate, stats);
+}
+
+// a recorded pass: box `which`'s features and the centre record of its exact
+// far field; -1 when the record is full
+int ff_record_pass(FFNet* n, const float4* fine, const float4* coarse, const float4* mono, const float4* farA, const float4* farT0, const float4* farT1, const Box* bx, int which,
+                   hipStream_t s) {
+  if (n->passes >= n->cap) return -1;
+  int k = n->passes++;
+  float4* feat = n->passFeat + (size_t)k * 4096;
+  float* meta = n->passMeta + (size_t)k * 4;
+  hipLaunchKernelGGL(k_ff_feat, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, feat, meta);
+  hipLaunchKernelGGL(k_ff_base, dim3(1024), dim3(256), 0, s, fine, coarse, farA, farT0, farT1, bx, which,
+                     n->passR + (size_t)k * 4096 * FF_R, nullptr, nullptr, nullptr, nullptr, nullptr);
+  return k;
+}
+
+// a training sample from recorded passes `base` and `cur`: the network's input
+// and target as a mesh step would form them; -1 when the record is full
+static int ff_pair(FFNet* n, int base, int cur) {
   if (n->count >= n->cap) return -1;
   int k = n->count++;
-  hipLaunchKernelGGL(k_ff_in, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, k, n->dataIn, n->dataScale, nullptr);
-  hipLaunchKernelGGL(k_ff_target, dim3(16), dim3(256), 0, s, coarse, farA, farT0, farT1, mono, bx, which, k,
-                     n->dataTgt, n->dataMask);
+  hipStream_t s = n->s;
+  const float4 *fT = n->passFeat + (size_t)cur * 4096, *fB = n->passFeat + (size_t)base * 4096;
+  const float *mT = n->passMeta + (size_t)cur * 4, *mB = n->passMeta + (size_t)base * 4;
+  hipLaunchKernelGGL(k_ff_pair_in, dim3(1), dim3(PAIR_BLOCK), 0, s, fT, mT, fB, mB, nullptr,
+                     n->dataIn + (size_t)k * 4096 * 16, n->dataScale + k, nullptr, nullptr);
+  hipLaunchKernelGGL(k_ff_pair_tgt, dim3(16), dim3(256), 0, s, n->passR + (size_t)cur * 4096 * FF_R,
+                     n->passR + (size_t)base * 4096 * FF_R, fT, fB, mT, n->dataScale + k, n->dataTgt + (size_t)k * 4096 * 16,
+                     n->dataMask + (size_t)k * 4096);
   return k;
 }
 
@@ -800,14 +1072,6 @@ int ff_train(FFNet* n, int iters, int B, float lr, uint32_t seed, double* losses
 }
 
 
-void launch_judge(const float4* force, const float4* ref, double* sums, float tol, float* stats, int* gate,
-                  hipStream_t s) {
-  hipLaunchKernelGGL(k_judge_sum, dim3(grid_of(FINE_TEXELS)), dim3(256), 0, s, force, ref, FINE_TEXELS, sums);
-  hipLaunchKernelGGL(k_judge_decide, dim3(1), dim3(64), 0, s, sums, tol, stats, gate);
-}
-void launch_count_learned(const int* gate, float* stats, hipStream_t s) {
-  hipLaunchKernelGGL(k_count_learned, dim3(1), dim3(64), 0, s, gate, stats);
-}
 
 // forward over recorded samples [first, first+count) in batches of up to 16;
 // optional outputs (count*4096*16 f32) and the error sums of k_eval_sum
@@ -931,6 +1195,43 @@ int cc_ff_samples(cc_ff* f, int* count) {
 int cc_ff_clear(cc_ff* f) {
   if (!f) return CC_EINVAL;
   f->n->count = 0;
+  return 0;
+}
+
+int cc_ff_passes(cc_ff* f, int* count) {
+  if (!f || !count) return CC_EINVAL;
+  *count = f->n->passes;
+  return 0;
+}
+
+int cc_ff_clear_passes(cc_ff* f) {
+  if (!f) return CC_EINVAL;
+  f->n->passes = 0;
+  return 0;
+}
+
+int cc_ff_pair(cc_ff* f, int base, int cur, int* sample) {
+  if (!f || base < 0 || cur < 0 || base >= f->n->passes || cur >= f->n->passes)
+    FFFAIL(f, CC_EINVAL, "cc_ff_pair: pass index outside the record");
+  int k = ff_pair(f->n, base, cur);
+  if (k < 0) FFFAIL(f, CC_ESTATE, "cc_ff_pair: the sample record is full");
+  hipError_t e = hipStreamSynchronize(f->n->s);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
+  if (sample) *sample = k;
+  return 0;
+}
+
+int cc_ff_read_pass(cc_ff* f, int k, float* feat, float* meta, float* record) {
+  if (!f || k < 0 || k >= f->n->passes) return CC_EINVAL;
+  FFNet* n = f->n;
+  hipError_t e = hipSuccess;
+  if (feat) e = hipMemcpyAsync(feat, n->passFeat + (size_t)k * 4096, 4096 * 16, hipMemcpyDeviceToHost, n->s);
+  if (e == hipSuccess && meta) e = hipMemcpyAsync(meta, n->passMeta + (size_t)k * 4, 16, hipMemcpyDeviceToHost, n->s);
+  if (e == hipSuccess && record)
+    e = hipMemcpyAsync(record, n->passR + (size_t)k * 4096 * FF_R, (size_t)4096 * FF_R * 4, hipMemcpyDeviceToHost, n->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->s);
+  if (e != hipSuccess) FFFAIL(f, CC_EDEVICE, hipGetErrorString(e));
   return 0;
 }
 

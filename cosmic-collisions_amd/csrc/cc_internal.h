@@ -143,10 +143,13 @@ void launch_coarse(MeshBuf& m, hipStream_t s);
 // the cell pass over the occupied fine texels, `blocks` work-groups of four wavefronts striding over them;
 // zeroed: the force mesh's empty texels already zero (the deposit's finish leaves them so)
 void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
-// the exact far field (skipped while *gate is nonzero) and the per-cell pass from given far-field buffers
-void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
+// the exact far field (k_far); with *gate set (a learned far field standing in) only the blocks occupied now
+// and empty in baseFeat, the network's base (none without baseFeat)
+void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s,
+                const float4* baseFeat = nullptr);
 // the centres of mass of a mesh written from outside: com[t] for mesh[t], t < n
 void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
+// the per-cell pass from given far-field buffers
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
 // the pairwise pass: the loose list's gather, its tiles' boxes (bounds: 2 x PP_TILES_MAX), then each seat's
@@ -186,6 +189,8 @@ void launch_books(int N, double mref, const float4* pos, const float4* vel, cons
                   double* partial, double* out, hipStream_t s);
 
 // ---- the learned far field (cc_ff.hip) ----
+// per box: last error, judged, fallbacks, learned mesh steps, largest / summed / counted error from a usable base, spare
+constexpr int FF_STATS = 8;
 struct FFNet;
 struct FFWork;
 size_t ff_weight_count();
@@ -193,15 +198,23 @@ FFNet* ff_create(int cap, std::string& err);
 void ff_destroy(FFNet* n);
 FFWork* ffwork_create(std::string& err);
 void ffwork_destroy(FFWork* w);
-// the network's far field for box `which` of one instance (kernels skipped unless *gate is nonzero, when given)
-void ff_infer(const FFNet* n, FFWork* wk, const float4* coarse, const float4* mono, const Box* bx, int which,
-              float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s);
-// a training sample from a box's coarse mesh and exact far field; -1 when the record is full
-int ff_record(FFNet* n, const float4* coarse, const float4* mono, const Box* bx, int which, const float4* farA,
-              const float4* farT0, const float4* farT1, hipStream_t s);
-// the judge: relative force error of `force` against `ref` -> stats and the gate for the coming mesh steps
-void launch_judge(const float4* force, const float4* ref, double* sums, float tol, float* stats, int* gate,
-                  hipStream_t s);
+// the network's far field for box `which` at a mesh step (the box's base plus the predicted change) into
+// farA/T0/T1; gate (between judges): cleared when the base cannot serve, the network skipped unless set
+void ff_predict(const FFNet* n, FFWork* wk, int which, const float4* coarse, const float4* mono, const Box* bx,
+                float4* farA, float4* farT0, float4* farT1, int* gate, hipStream_t s);
+// a judge step after ff_predict (into learned: 3 x 4096) and the exact far field and force mesh: the judge's
+// verdict (gate, stats) for the coming mesh steps, then the exact pass becomes the box's base
+void ff_judge_rebase(FFWork* wk, int which, const float4* fine, const float4* com, const float4* ccom, const float4* coarse,
+                     const float4* learned, const float4* farA, const float4* farT0, const float4* farT1,
+                     const float4* force, const Box* bx, float gm, double* sums, float tol, float* stats, int* gate,
+                     hipStream_t s);
+// both boxes' bases forgotten (a new state): no mesh step stands in before the next judge records one
+void ff_forget_bases(FFWork* w, hipStream_t s);
+// the features of box `which`'s base by block (what k_far reads to find the blocks the network has no base for)
+const float4* ff_base_features(const FFWork* w, int which);
+// a recorded pass of box `which`'s exact far field (features, centre record); -1 when the record is full
+int ff_record_pass(FFNet* n, const float4* fine, const float4* coarse, const float4* mono, const float4* farA, const float4* farT0, const float4* farT1, const Box* bx, int which,
+                   hipStream_t s);
 void launch_count_learned(const int* gate, float* stats, hipStream_t s);
 
 // ---- the analysis (cc_fof.hip) ----

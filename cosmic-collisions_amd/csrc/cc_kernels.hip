@@ -506,17 +506,24 @@ __global__ void __launch_bounds__(256) k_coarse(const float4* __restrict__ fine,
 
 // -------------------------------------------------------- far field (A6) --
 // One wavefront per coarse block; its lanes share the 4096 source blocks.
+// While a learned far field stands in (*gate set) only the blocks it has no
+// base for run: occupied now and empty in baseFeat (by block, (z*16 + y)*16 + x).
 __global__ void __launch_bounds__(256) k_far(const float4* __restrict__ ccom, float4* farA, float4* farT0,
-                                             float4* farT1, const int* __restrict__ gate) {
-  if (gate && *gate != 0) return;   // the learned far field is standing in
+                                             float4* farT1, const int* __restrict__ gate,
+                                             const float4* __restrict__ baseFeat) {
+  const bool learned = gate && *gate != 0;
+  if (learned && !baseFeat) return;
   int lane = threadIdx.x & 63;
   int me = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (me >= 4096) return;
   int mx = me & 63, my = me >> 6;
   int c0 = mx & 15, c1 = my & 15, c2 = (my >> 4) * 4 + (mx >> 4);
+  if (learned && baseFeat[(c2 * 16 + c1) * 16 + c0].x > 0.0f) return;
   float4 mc = ccom[me];
   if (mc.w <= 0.0f) {
-    if (lane == 0) { farA[me] = make_float4(0, 0, 0, 0); farT0[me] = make_float4(0, 0, 0, 0); farT1[me] = make_float4(0, 0, 0, 0); }
+    if (!learned && lane == 0) {
+      farA[me] = make_float4(0, 0, 0, 0); farT0[me] = make_float4(0, 0, 0, 0); farT1[me] = make_float4(0, 0, 0, 0);
+    }
     return;
   }
   float x0 = mc.x, x1 = mc.y, x2 = mc.z;
@@ -651,8 +658,9 @@ __global__ void __launch_bounds__(256) k_cell(const float4* __restrict__ fine, c
 void launch_coarse(MeshBuf& m, hipStream_t s) {
   hipLaunchKernelGGL(k_coarse, dim3(16), dim3(256), 0, s, m.fine, m.coarse, m.ccom, m.mono, m.coarseDone);
 }
-void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s) {
-  hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, m.ccom, farA, farT0, farT1, gate);
+void launch_far(const MeshBuf& m, float4* farA, float4* farT0, float4* farT1, const int* gate, hipStream_t s,
+                const float4* baseFeat) {
+  hipLaunchKernelGGL(k_far, dim3(1024), dim3(256), 0, s, m.ccom, farA, farT0, farT1, gate, baseFeat);
 }
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, int blocks, bool zeroed, hipStream_t s) {

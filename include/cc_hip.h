@@ -210,15 +210,19 @@ int cc_last_step_ms(cc_ctx* ctx, double* ms);
 /* The learned far field (no reference counterpart).  A 3D convolutional
  * operator on the 16^3 coarse mesh -- 4 levels, 32 channels, 3x3x3 kernels,
  * BF16 weights on the matrix cores with FP32 accumulation -- that stands in
- * for the block-to-block far field (FAR_FS, src/glsl-sim.js:263-295): coarse
- * masses and dipoles in, each block's far acceleration, potential and tidal
- * tensor out (at the block's centre, carried to its centre of mass by the
- * tidal tensor).  Trained on the GPU by the library's own forward and
- * backward kernels on samples recorded from the exact far field.
+ * for the block-to-block far field (FAR_FS, src/glsl-sim.js:263-295) on the
+ * mesh steps between judges.  Every judgeEvery steps the exact far field runs,
+ * the network's prediction is measured against it, and that exact pass becomes
+ * the base; in between the network predicts each block's change of far-field
+ * acceleration, potential and tidal tensor since the base from the change of
+ * the coarse mesh (block masses and centre-of-mass offsets), added to the
+ * base's far field and carried to each block's centre of mass by the tidal
+ * tensor.  Trained on the GPU by the library's own forward and backward
+ * kernels on pairs of passes recorded from the exact far field.
  * Weights: FP32, per layer [27 taps][cin][cout] then [cout] biases, layers in
  * order (cc_ff_weight_count floats). */
 typedef struct cc_ff cc_ff;
-int cc_ff_create(int capacity, cc_ff** out);     /* capacity: training samples held on the device */
+int cc_ff_create(int capacity, cc_ff** out);     /* capacity: recorded passes, and training samples, on the device */
 int cc_ff_destroy(cc_ff* f);
 const char* cc_ff_last_error(cc_ff* f);
 int cc_ff_weight_count(int64_t* n);
@@ -226,10 +230,22 @@ int cc_ff_get_weights(cc_ff* f, float* w, int64_t n);
 int cc_ff_set_weights(cc_ff* f, const float* w, int64_t n);
 /* the leaky ReLU's negative slope, 0.25 by default; 1 makes the network a linear operator of its input */
 int cc_ff_set_slope(cc_ff* f, double slope);
-/* a sample from box `box` of ctx's last mesh pass, whose far field must be the exact one (cc_mesh) */
+/* record a pass: box `box` of ctx's last mesh pass, whose far field must be the exact one (cc_mesh, or a
+ * cc_step_n without a network attached).  feat[4096][4] per block (voxel (z*16 + y)*16 + x): mass over the
+ * box's total mass times 64, and that times the centre-of-mass offset from the block's centre in block
+ * lengths; meta[4]: the box's corner offset L and total mass; record[4096][12]: the far field carried to
+ * each block's centre (acceleration, potential, tidal xx yy zz xy xz yz, 2 spare) on occupied blocks and
+ * their neighbours */
 int cc_ff_record(cc_ff* f, cc_ctx* ctx, int box);
-/* samples in the network's own units: in[4096][16] (channels 0-3 mass, centre-of-mass offset), target[4096][16]
- * (0-2 acceleration, 3 potential, 4-9 tidal xx yy zz xy xz yz), mask[4096] (1: occupied block) */
+int cc_ff_passes(cc_ff* f, int* count);
+int cc_ff_read_pass(cc_ff* f, int k, float* feat, float* meta, float* record);
+int cc_ff_clear_passes(cc_ff* f);
+/* a training sample from recorded passes: the prediction from `base` to `cur` as a mesh step forms it */
+int cc_ff_pair(cc_ff* f, int base, int cur, int* sample);
+/* samples in the network's own units: in[4096][16] (0-3 the present features, 4-7 their change since the
+ * base over s -- the change's root mean square over blocks occupied in either pass --, 8-9 occupancy now
+ * and at the base), target[4096][16] (the change of the record's first ten channels over s times the
+ * total mass / 64, per block length squared, to the first and cubed), mask[4096] (1: occupied now) */
 int cc_ff_push_sample(cc_ff* f, const float* in, const float* target, const float* mask);
 int cc_ff_read_sample(cc_ff* f, int k, float* in, float* target, float* mask);
 int cc_ff_samples(cc_ff* f, int* count);
@@ -242,10 +258,13 @@ int cc_ff_eval(cc_ff* f, int first, int count, float* out, double* relAcc, doubl
 /* the masked mean-squared loss and its gradient wrt every weight (no update), samples [first, first+count) */
 int cc_ff_gradient(cc_ff* f, int first, int count, float* grad, double* loss);
 /* attach (f) or detach (NULL) the network to a ctx's step: every judgeEvery steps (a multiple of meshEvery)
- * the exact far field makes the force mesh and the network's force mesh is measured against it; the
- * network stands in on the mesh steps in between while the last relative force error was <= tol */
+ * the exact far field makes the force mesh, the network's prediction is measured against it (the relative
+ * error of the force mesh's far-field part over the whole force mesh), and that pass becomes the base; the
+ * network stands in on the mesh steps in between while the last error was <= tol and measured from a base
+ * that could serve (recorded since the state was last uploaded, the box moved by under a quarter block) */
 int cc_set_far_field(cc_ctx* ctx, cc_ff* f, int judgeEvery, double tol);
-/* per box: last relative force error, judge passes, fallbacks, mesh steps run on the network (out[8]) */
+/* per box (out[16]): last relative force error, judge passes, fallbacks, mesh steps run on the network, the
+ * largest and the summed error and the count of judges measured from a base that could serve, spare */
 int cc_ff_stats(cc_ctx* ctx, double* out);
 
 /* Batched sweeps (no reference counterpart: the reference runs one instance

@@ -122,13 +122,18 @@ def forward(w, x0, rounding=True):
     return y, c
 
 
+CHAN_W = np.array([1.0] * 3 + [0.25] * 7 + [0.0] * 6)   # the loss's channel weights (cc_ff.hip: ff_chan_weight)
+
+
 def loss_grad(y, tgt, mask, rounding=True):
-    """Masked mean squared error over the ten used channels, per voxel mean."""
+    """Weighted mean squared error (each voxel by its mask, each channel by
+    CHAN_W), per voxel mean, and its gradient."""
     B = y.shape[0]
-    m = (mask.reshape(B, 16, 16, 16, 1) > 0) & (np.arange(16) < 10)
-    d = np.where(m, y - tgt, 0.0)
+    w = np.asarray(mask, np.float64).reshape(B, 16, 16, 16, 1) * CHAN_W
+    d = np.where(w > 0, y - tgt, 0.0)
     n = B * 4096.0
-    return float((d * d).sum() / n), (f32(2.0 * d / n) if rounding else 2.0 * d / n)
+    g = 2.0 * w * d / n
+    return float((w * d * d).sum() / n), (f32(g) if rounding else g)
 
 
 def backward(c, gy):
@@ -168,3 +173,175 @@ def backward(c, gy):
     ge0 = ge0 + poolb(gp1)
     G(0, G(1, ge0))
     return np.concatenate([np.concatenate([gw.ravel(), gb]) for gw, gb in grads])
+
+
+# ---------------------------------------------------------------- the mesh interface
+# Restatements of the library's interface between the mesh and the network
+# (cc_ff.hip: k_ff_feat, k_ff_base, k_ff_pair_in, k_ff_pair_tgt, k_ff_out,
+# k_ff_judge_sum), in FP32 where the device computes in FP32 and in the same
+# operation order, so that features, records and the carried far field can be
+# compared bit for bit.  Blocks by voxel v = (z*16 + y)*16 + x.
+FH = 2.0 * 6.0 / 64.0
+BLOCK_LEN = np.float32(4.0 * FH)
+FF_R = 12
+_V = np.arange(4096)
+VX, VY, VZ = _V & 15, (_V >> 4) & 15, _V >> 8
+
+
+def coarse_texel(x, y, z):
+    return ((z >> 2) * 16 + y) * 64 + (z & 3) * 16 + x
+
+
+def _by_voxel(mesh64):
+    """A 64x64 coarse atlas (float4 texels) -> [4096, 4] by voxel."""
+    return np.asarray(mesh64, np.float32).reshape(-1, 4)[coarse_texel(VX, VY, VZ)]
+
+
+def _centres(L):
+    """Each block's centre, FP32 as the device forms it: (4k + 2) FH - L."""
+    fh = np.float32(FH)
+    return np.stack([np.float32(4 * c + 2) * fh - np.float32(L[k]) for k, c in enumerate((VX, VY, VZ))], 1)
+
+
+def _offsets(c, L):
+    """Centre of mass less centre (occupied blocks), FP32."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return c[:, :3] / c[:, 3:4] - _centres(L)
+
+
+def features(coarse, mono_w, L):
+    """k_ff_feat: ([4096, 4] features, meta[4])."""
+    c = _by_voxel(coarse)
+    mt = np.float32(max(np.float32(mono_w), np.float32(1e-20)))
+    f = np.zeros((4096, 4), np.float32)
+    occ = c[:, 3] > 0
+    d = _offsets(c, L)
+    f[occ, 0] = c[occ, 3] / mt * np.float32(64.0)
+    for k in range(3):
+        f[occ, 1 + k] = f[occ, 0] * d[occ, k] / BLOCK_LEN
+    meta = np.array([L[0], L[1], L[2], mt], np.float32)
+    return f, meta
+
+
+def _centre_occupied(c, L, A, T0, T1):
+    """An occupied block's record: the far field carried to its centre to first order."""
+    d = _offsets(c, L)
+    dx, dy, dz = d[:, 0], d[:, 1], d[:, 2]
+    ax = A[:, 0] - (T0[:, 0] * dx + T0[:, 3] * dy + T1[:, 0] * dz)
+    ay = A[:, 1] - (T0[:, 3] * dx + T0[:, 1] * dy + T1[:, 1] * dz)
+    az = A[:, 2] - (T1[:, 0] * dx + T1[:, 1] * dy + T0[:, 2] * dz)
+    p = A[:, 3] + (ax * dx + ay * dy + az * dz)
+    z = np.zeros_like(ax)
+    return np.stack([ax, ay, az, p, T0[:, 0], T0[:, 1], T0[:, 2], T0[:, 3], T1[:, 0], T1[:, 1], z, z], 1)
+
+
+def texel_counts(fine):
+    """Occupied fine texels of each block, by voxel (a 513x512 fine atlas)."""
+    F = np.asarray(fine, np.float32).reshape(-1, 4)[:512 * 512]
+    e = np.arange(512 * 512)
+    mx, my = e & 511, e >> 9
+    c0, c1, c2 = mx & 63, my & 63, (my >> 6) * 8 + (mx >> 6)
+    vox = ((c2 >> 2) * 16 + (c1 >> 2)) * 16 + (c0 >> 2)
+    return np.bincount(vox[F[:, 3] > 0], minlength=4096)
+
+
+def centre_record(fine, coarse, L, farA, farT0, farT1):
+    """k_ff_base: [4096, 12] record, FP32 as the device computes it: occupied
+    blocks' far field carried to their centres, their occupied fine texels in
+    channel 10; empty blocks zero."""
+    c = _by_voxel(coarse)
+    A, T0, T1 = _by_voxel(farA), _by_voxel(farT0), _by_voxel(farT1)
+    occ = c[:, 3] > 0
+    R = np.zeros((4096, FF_R), np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        R[occ] = _centre_occupied(c, L, A, T0, T1)[occ]
+    R[occ, 10] = texel_counts(fine)[occ]
+    return R
+
+
+def pair_input(fT, fB):
+    """k_ff_pair_in: (x0 [4096, 16] before the BF16 rounding, s); s in FP64."""
+    d = fT.astype(np.float64) - fB
+    use = (fT[:, 0] > 0) | (fB[:, 0] > 0)
+    ss = (d[use] ** 2).sum()
+    s = np.sqrt(ss / (4.0 * use.sum())) if ss > 0 else 1.0
+    x = np.zeros((4096, 16))
+    x[:, 0:4] = fT
+    x[:, 4:8] = d / s
+    x[:, 8] = fT[:, 0] > 0
+    x[:, 9] = fB[:, 0] > 0
+    return x, s
+
+
+def units(s, mt):
+    """Per-channel units of the output: s * mt / 64 over BLOCK_LEN^2, ^1, ^3."""
+    u = np.float64(s) * np.float64(mt) / 64.0
+    bl = np.float64(BLOCK_LEN)
+    return np.array([u / bl ** 2] * 3 + [u / bl] + [u / bl ** 3] * 6)
+
+
+def pair_target(RT, RB, fT, fB, mt, s):
+    """k_ff_pair_tgt: (target [4096, 16], mask [4096]): the change on blocks
+    occupied at T and B, weighted by their occupied fine texels at T / 64."""
+    occ = (fT[:, 0] > 0) & (fB[:, 0] > 0)
+    t = np.zeros((4096, 16))
+    t[occ, :10] = (RT[occ, :10].astype(np.float64) - RB[occ, :10]) / units(s, mt)
+    m = np.zeros(4096, np.float32)
+    m[occ] = RT[occ, 10] / np.float32(64.0)
+    return t, m
+
+
+def carry(y, RB, s, mt, coarse, L):
+    """k_ff_out: the network's far field (farA, farT0, farT1, each [4096, 4] by
+    voxel) from outputs y [4096, 16], FP32 as the device computes it (blocks
+    empty at the base, which k_ff_out leaves to k_far, included)."""
+    c = _by_voxel(coarse)
+    occ = c[:, 3] > 0
+    y = np.asarray(y, np.float32)
+    r = np.asarray(RB, np.float32)
+    u = np.float32(s) * np.float32(mt) / np.float32(64.0)
+    bl = BLOCK_LEN
+    uA, uP, uT = u * (np.float32(1.0) / (bl * bl)), u * (np.float32(1.0) / bl), u * (np.float32(1.0) / (bl * bl * bl))
+    a = [r[:, k] + y[:, k] * uA for k in range(3)]
+    p = r[:, 3] + y[:, 3] * uP
+    t = [r[:, 4 + k] + y[:, 4 + k] * uT for k in range(6)]
+    txx, tyy, tzz, txy, txz, tyz = t
+    d = _offsets(c, L)
+    dx, dy, dz = d[:, 0], d[:, 1], d[:, 2]
+    sx = a[0] + txx * dx + txy * dy + txz * dz
+    sy = a[1] + txy * dx + tyy * dy + tyz * dz
+    sz = a[2] + txz * dx + tyz * dy + tzz * dz
+    A = np.stack([sx, sy, sz, p - (a[0] * dx + a[1] * dy + a[2] * dz)], 1)
+    T0 = np.stack([txx, tyy, tzz, txy], 1)
+    T1 = np.stack([txz, tyz, np.zeros_like(txz), np.zeros_like(txz)], 1)
+    for X in (A, T0, T1):
+        X[~occ] = 0
+    return A.astype(np.float32), T0.astype(np.float32), T1.astype(np.float32)
+
+
+def judge_error(fine, coarse, learned, exact, force, gm, fB):
+    """k_ff_judge_sum + k_judge_decide: the relative error, over the force mesh
+    (its 512 x 512 texels), of the far-field part the learned far field gives
+    each occupied texel of a block occupied at the base (fB: the base's
+    features).  learned, exact: (farA, farT0, farT1) by voxel."""
+    F = np.asarray(fine, np.float64).reshape(-1, 4)[:512 * 512]
+    e = np.arange(512 * 512)
+    mx, my = e & 511, e >> 9
+    c0, c1, c2 = mx & 63, my & 63, (my >> 6) * 8 + (mx >> 6)
+    vox = ((c2 >> 2) * 16 + (c1 >> 2)) * 16 + (c0 >> 2)
+    c = _by_voxel(coarse).astype(np.float64)
+    occ = (F[:, 3] > 0) & (np.asarray(fB)[vox, 0] > 0)
+    com = np.zeros((len(F), 3))
+    com[occ] = F[occ, :3] / F[occ, 3:4]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        bcom = c[:, :3] / c[:, 3:4]
+    q = com - bcom[vox]
+    dA = (np.asarray(learned[0], np.float64) - exact[0])[vox]
+    d0 = (np.asarray(learned[1], np.float64) - exact[1])[vox]
+    d1 = (np.asarray(learned[2], np.float64) - exact[2])[vox]
+    dx = dA[:, 0] + d0[:, 0] * q[:, 0] + d0[:, 3] * q[:, 1] + d1[:, 0] * q[:, 2]
+    dy = dA[:, 1] + d0[:, 3] * q[:, 0] + d0[:, 1] * q[:, 1] + d1[:, 1] * q[:, 2]
+    dz = dA[:, 2] + d1[:, 0] * q[:, 0] + d1[:, 1] * q[:, 1] + d0[:, 2] * q[:, 2]
+    num = ((dx * dx + dy * dy + dz * dz)[occ]).sum() * gm * gm
+    Fm = np.asarray(force, np.float64).reshape(-1, 4)[:512 * 512, :3]
+    return float(np.sqrt(num / (Fm * Fm).sum()))
