@@ -129,7 +129,7 @@ def _books(ctx):
     return np.array([b["Jx"], b["Jy"], b["Jz"]]), b["KE"] + b["PE"] + b["Q"]
 
 
-def learned_leg(state_root, batch_dir, steps, warmup, reuse, cycles, sweep, sweep_cycles, iters):
+def learned_leg(state_root, batch_dir, steps, warmup, reuse, cycles, sweep, sweep_cycles, iters, batch_steps):
     """The learned far field on the shatter impact, a configuration it was not
     trained on: trained on the GPU from the exact far field of the theia-like and
     twins runs and the batched sweep's instances, then judged in the shatter run
@@ -171,6 +171,35 @@ def learned_leg(state_root, batch_dir, steps, warmup, reuse, cycles, sweep, swee
                       "mean_force_err": st_["sum_err"] / max(st_["usable"], 1.0)})
         out[mode] = o
         R.close()
+    # the batched form: a sweep the network never saw (the next seed), 64 instances stepped as one batch with
+    # the exact far field and with the network attached to every instance
+    if batch_steps <= 0:
+        insts = []
+    else:
+        insts = runs.prepare_batch(batch_dir + "_unseen", 64, seed=runs.BATCH_SEED + 1, reuse=True)
+    bo = {"workload": "64 giant impacts of the sweep's next seed (none in the training data), one batch, %d steps "
+                      "after 24" % batch_steps}
+    for mode in ("exact", "learned") if insts else ():
+        B = runs.BatchRunner(insts)
+        if mode == "learned":
+            for r in B.rs:
+                r.ctx.set_far_field(ff, 16, 1e-3)
+        B.steps(24)
+        t0 = time.perf_counter()
+        B.steps(batch_steps)
+        wall = time.perf_counter() - t0
+        o = {"value": B.N * batch_steps / wall, "ms_per_step": wall * 1e3 / batch_steps}
+        if mode == "learned":
+            st_ = np.array([r.ctx.ff_stats() for r in B.rs])   # instance, box, FF_STATS
+            tot = st_.sum(axis=(0, 1))
+            o.update({"judged": int(tot[1]), "fallbacks": int(tot[2]), "learned_mesh_steps": int(tot[3]),
+                      "mean_force_err": float(tot[5] / max(tot[6], 1.0)), "max_force_err": float(st_[:, :, 4].max()),
+                      "instances_above_1e-3": int((st_[:, :, 4] > 1e-3).any(axis=1).sum())})
+        bo[mode] = o
+        bo["particles"] = B.N
+        B.close()
+    if insts:
+        out["batched"] = bo
     ff.close()
     zero.close()
     for mode in ("learned", "zero_network"):   # energy (KE + PE + heat, no contact springs) against the exact run's
@@ -270,6 +299,7 @@ def main():
     ap.add_argument("--ff-sweep", type=int, default=64, help="sweep instances recorded for training (0: none)")
     ap.add_argument("--ff-sweep-cycles", type=int, default=40)
     ap.add_argument("--ff-iters", type=int, default=6000)
+    ap.add_argument("--ff-batch-steps", type=int, default=96, help="steps of the unseen sweep with the network (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)
@@ -344,7 +374,7 @@ def main():
     if args.learned > 0:
         out["learned"] = learned_leg(tempfile.gettempdir(), args.batch_dir, args.learned_steps, 200,
                                      True, args.ff_cycles, args.ff_sweep,
-                                     args.ff_sweep_cycles, args.ff_iters)
+                                     args.ff_sweep_cycles, args.ff_iters, args.ff_batch_steps)
     out["hip_runtime"] = hip_runtime()
     print(json.dumps(out))
     return 0
