@@ -57,7 +57,7 @@ __device__ __forceinline__ T ff_wave_sum(T v) {
 // in  [M voxels][CIN] bf16 (M = B*S^3), wt [27][CIN/16][COUT][16] bf16,
 // bias [COUT] f32 or null; slope: the leaky ReLU's (1: none).  Writes out (bf16, [M][COUT])
 // and/or outf (f32), and pre (f32 pre-activation) when asked.
-// The work-group stages the layer's weights in LDS once; each wavefront issues
+// Work-groups of 256.  The work-group stages the layer's weights in LDS once; each wavefront issues
 // all 27 taps' activation loads before its matrix-core chain, so a launch pays
 // about one memory latency instead of one per tap (what bounds a single
 // instance's 16^3 mesh, whose widest layer is 256 wavefronts).  The chain adds
@@ -68,10 +68,21 @@ __global__ void __launch_bounds__(256) k_conv(const uint16_t* __restrict__ in, i
                                               float slope, uint16_t* out, float* outf, float* pre,
                                               const int* __restrict__ gate) {
   if (gate && *gate == 0) return;   // the same for every work-group: before the barrier
-  constexpr int WN = 27 * (CIN / 16) * COUT * 16;
+  constexpr int WN = 27 * (CIN / 16) * COUT * 16, NV = WN / 8, PER = (NV + 255) / 256;
   __shared__ __align__(16) uint16_t wl[WN];
-  for (int i = threadIdx.x; i < WN / 8; i += blockDim.x)
-    reinterpret_cast<uint4*>(wl)[i] = reinterpret_cast<const uint4*>(wt)[i];
+  {   // every 16-byte chunk's load in flight before the stores (a load-store loop waits out each in turn)
+    uint4 tmp[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = threadIdx.x + 256 * k;
+      tmp[k] = i < NV ? reinterpret_cast<const uint4*>(wt)[i] : uint4{ 0u, 0u, 0u, 0u };
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = threadIdx.x + 256 * k;
+      if (i < NV) reinterpret_cast<uint4*>(wl)[i] = tmp[k];
+    }
+  }
   __syncthreads();
   int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   int lane = threadIdx.x & 63;
@@ -299,15 +310,9 @@ __device__ __forceinline__ void block_offset(float4 c, int x, int y, int z, cons
   dy = c.y / c.w - ((4 * y + 2) * (float)FH - L[1]);
   dz = c.z / c.w - ((4 * z + 2) * (float)FH - L[2]);
 }
-__global__ void k_ff_feat(const float4* __restrict__ coarse, const float4* __restrict__ mono,
-                          const Box* __restrict__ bxp, int which, float4* feat, float* meta) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= 4096) return;
+__device__ __forceinline__ float4 ff_feature(const float4* __restrict__ coarse, float mt, const float* L, int v) {
   int x = v & 15, y = (v >> 4) & 15, z = v >> 8;
-  const Box bx = *bxp;
-  const float* L = which ? bx.LB : bx.LA;
   float4 c = coarse[coarse_texel(x, y, z)];
-  float mt = fmaxf(mono[0].w, 1e-20f);
   float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (c.w > 0.0f) {
     float dx, dy, dz;
@@ -317,7 +322,16 @@ __global__ void k_ff_feat(const float4* __restrict__ coarse, const float4* __res
     f.z = f.x * dy / BLOCK_LEN;
     f.w = f.x * dz / BLOCK_LEN;
   }
-  feat[v] = f;
+  return f;
+}
+__global__ void k_ff_feat(const float4* __restrict__ coarse, const float4* __restrict__ mono,
+                          const Box* __restrict__ bxp, int which, float4* feat, float* meta) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 4096) return;
+  const Box bx = *bxp;
+  const float* L = which ? bx.LB : bx.LA;
+  float mt = fmaxf(mono[0].w, 1e-20f);
+  feat[v] = ff_feature(coarse, mt, L, v);
   if (v == 0) { meta[0] = L[0]; meta[1] = L[1]; meta[2] = L[2]; meta[3] = mt; }
 }
 
@@ -368,21 +382,39 @@ __global__ void __launch_bounds__(256) k_ff_base(const float4* __restrict__ fine
 // the blocks occupied at T or B, or 1 when nothing changed.  One work-group.
 // With ok given, whether the base can serve -- recorded, and the box's corner
 // within a quarter block of where it was -- goes to *ok, and a given gate is
-// cleared when it cannot.
+// cleared when it cannot.  MESH: the present features are taken here from box
+// `which`'s coarse mesh (k_ff_feat's) and stored to featT and metaT; else read
+// from featT and metaT.
 constexpr int PAIR_BLOCK = 1024;
-__global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(const float4* __restrict__ featT, const float* __restrict__ metaT,
+template <bool MESH>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(float4* featT, float* metaT,
                                                            const float4* __restrict__ featB, const float* __restrict__ metaB,
                                                            const int* __restrict__ validB, uint16_t* x0, float* scale,
-                                                           int* ok, int* gate) {
+                                                           int* ok, int* gate, const float4* __restrict__ coarse,
+                                                           const float4* __restrict__ mono, const Box* __restrict__ bxp,
+                                                           int which) {
   __shared__ float redS[PAIR_BLOCK / 64];
   __shared__ int redN[PAIR_BLOCK / 64];
   const int tid = threadIdx.x;
   float4 fT[4096 / PAIR_BLOCK], fB[4096 / PAIR_BLOCK];
   float ss = 0.0f;
   int cnt = 0;
+  float mt = 0.0f;
+  const float* L = nullptr;
+  Box bx;
+  if (MESH) {
+    bx = *bxp;
+    L = which ? bx.LB : bx.LA;
+    mt = fmaxf(mono[0].w, 1e-20f);
+  }
   for (int j = 0; j < 4096 / PAIR_BLOCK; j++) {
     int v = tid + PAIR_BLOCK * j;
-    fT[j] = featT[v];
+    if (MESH) {
+      fT[j] = ff_feature(coarse, mt, L, v);
+      featT[v] = fT[j];
+    } else {
+      fT[j] = featT[v];
+    }
     fB[j] = featB[v];
     if (fT[j].x > 0.0f || fB[j].x > 0.0f) {
       float a = fT[j].x - fB[j].x, b = fT[j].y - fB[j].y, c = fT[j].z - fB[j].z, d = fT[j].w - fB[j].w;
@@ -408,6 +440,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(const float4* __restr
   }
   if (tid == 0) {
     scale[0] = s;
+    if (MESH) { metaT[0] = L[0]; metaT[1] = L[1]; metaT[2] = L[2]; metaT[3] = mt; }
     if (ok) {
       const float q = 0.25f * BLOCK_LEN;
       int good = (!validB || *validB) && fabsf(metaT[0] - metaB[0]) <= q && fabsf(metaT[1] - metaB[1]) <= q &&
@@ -945,9 +978,9 @@ void ff_forget_bases(FFWork* w, hipStream_t s) {
 void ff_predict(const FFNet* n, FFWork* wk, int which, const float4* coarse, const float4* mono, const Box* bx,
                 float4* farA, float4* farT0, float4* farT1, int* gate, hipStream_t s) {
   Acts& a = wk->a;
-  hipLaunchKernelGGL(k_ff_feat, dim3(16), dim3(256), 0, s, coarse, mono, bx, which, wk->featT[which], wk->metaT[which]);
-  hipLaunchKernelGGL(k_ff_pair_in, dim3(1), dim3(PAIR_BLOCK), 0, s, wk->featT[which], wk->metaT[which],
-                     wk->featB[which], wk->metaB[which], wk->validB[which], a.x0, a.scale, wk->okBase[which], gate);
+  hipLaunchKernelGGL(k_ff_pair_in<true>, dim3(1), dim3(PAIR_BLOCK), 0, s, wk->featT[which], wk->metaT[which],
+                     wk->featB[which], wk->metaB[which], wk->validB[which], a.x0, a.scale, wk->okBase[which], gate,
+                     coarse, mono, bx, which);
   forward(n, a, 1, false, gate, s);
   hipLaunchKernelGGL(k_ff_out, dim3(16), dim3(256), 0, s, a.y, wk->RB[which], wk->featB[which], a.scale, coarse,
                      wk->metaT[which], bx, which, farA, farT0, farT1, gate);
@@ -993,8 +1026,9 @@ static int ff_pair(FFNet* n, int base, int cur) {
   hipStream_t s = n->s;
   const float4 *fT = n->passFeat + (size_t)cur * 4096, *fB = n->passFeat + (size_t)base * 4096;
   const float *mT = n->passMeta + (size_t)cur * 4, *mB = n->passMeta + (size_t)base * 4;
-  hipLaunchKernelGGL(k_ff_pair_in, dim3(1), dim3(PAIR_BLOCK), 0, s, fT, mT, fB, mB, nullptr,
-                     n->dataIn + (size_t)k * 4096 * 16, n->dataScale + k, nullptr, nullptr);
+  hipLaunchKernelGGL(k_ff_pair_in<false>, dim3(1), dim3(PAIR_BLOCK), 0, s, const_cast<float4*>(fT),
+                     const_cast<float*>(mT), fB, mB, nullptr, n->dataIn + (size_t)k * 4096 * 16, n->dataScale + k,
+                     nullptr, nullptr, nullptr, nullptr, nullptr, 0);
   hipLaunchKernelGGL(k_ff_pair_tgt, dim3(16), dim3(256), 0, s, n->passR + (size_t)cur * 4096 * FF_R,
                      n->passR + (size_t)base * 4096 * FF_R, fT, fB, mT, n->dataScale + k, n->dataTgt + (size_t)k * 4096 * 16,
                      n->dataMask + (size_t)k * 4096);
