@@ -1125,14 +1125,22 @@ int cc_last_step_ms(cc_ctx* c, double* ms) {
 // Independent instances stepped together: each instance's step cycle is one
 // branch of a single captured graph (forked from and joined to the batch's
 // stream), so a cycle of meshEvery steps of every instance is one launch.
+// A captured batch cycle: what it bakes in (each instance's cycle key), and each instance's
+// buffer parities and count state after it.
+struct BatchGraph {
+  std::vector<double> key;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> curAfter, gridCurAfter, countedAfter;
+};
 struct cc_batch {
   std::vector<cc_ctx*> ctx;
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> join;
-  std::vector<double> key;
-  hipGraphExec_t exec = nullptr;
-  std::vector<int> curAfter, gridCurAfter, countedAfter;
+  // the last few captured cycles, replaced in turn: cycles alternate between keys (a learned far
+  // field's judge and the steps between, the step parity with an odd meshEvery)
+  BatchGraph graphs[4];
+  int nextGraph = 0;
   double lastMs = 0.0;
   std::string err;
 };
@@ -1150,17 +1158,20 @@ struct cc_batch {
     if (_e != hipSuccess) BFAIL(b, CC_EDEVICE, "%s: %s", #call, hipGetErrorString(_e));       \
   } while (0)
 
-static int batch_graph(cc_batch* b, const cc_sim_args* args) {
+static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   std::vector<double> key;
   for (size_t i = 0; i < b->ctx.size(); i++) {
     std::vector<double> k = cycle_key(b->ctx[i], &args[i]);
     key.insert(key.end(), k.begin(), k.end());
   }
-  if (b->exec && key == b->key) return 0;
-  if (b->exec) {   // its last launch may still run: let it end first
+  for (BatchGraph& g : b->graphs)
+    if (g.exec && g.key == key) { *out = &g; return 0; }
+  BatchGraph& g = b->graphs[b->nextGraph];
+  b->nextGraph = (b->nextGraph + 1) % 4;
+  if (g.exec) {   // its last launch may still run: let it end first
     BCHK(b, hipStreamSynchronize(b->s));
-    hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
+    hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
   }
   size_t n = b->ctx.size();
   std::vector<int> stepNo(n), cur(n), gcur(n), counted(n);
@@ -1182,12 +1193,12 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args) {
   }
   hipGraph_t graph = nullptr;
   hipError_t ee = hipStreamEndCapture(b->s, &graph);
-  b->curAfter.assign(n, 0);
-  b->gridCurAfter.assign(n, 0);
-  b->countedAfter.assign(n, 0);
+  g.curAfter.assign(n, 0);
+  g.gridCurAfter.assign(n, 0);
+  g.countedAfter.assign(n, 0);
   for (size_t i = 0; i < n; i++) {
     cc_ctx* c = b->ctx[i];
-    b->curAfter[i] = c->cur; b->gridCurAfter[i] = c->gb.cur; b->countedAfter[i] = c->counted;
+    g.curAfter[i] = c->cur; g.gridCurAfter[i] = c->gb.cur; g.countedAfter[i] = c->counted;
     c->stepNo = stepNo[i]; c->cur = cur[i]; c->gb.cur = gcur[i]; c->counted = counted[i];
   }
   if (r || e != hipSuccess || ee != hipSuccess) {
@@ -1195,10 +1206,11 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args) {
     if (r) return r;
     BFAIL(b, CC_EDEVICE, "batch capture: %s", hipGetErrorString(e != hipSuccess ? e : ee));
   }
-  e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
-  if (e != hipSuccess) { b->exec = nullptr; BFAIL(b, CC_EDEVICE, "batch instantiate: %s", hipGetErrorString(e)); }
-  b->key = key;
+  if (e != hipSuccess) { g.exec = nullptr; BFAIL(b, CC_EDEVICE, "batch instantiate: %s", hipGetErrorString(e)); }
+  g.key = key;
+  *out = &g;
   return 0;
 }
 
@@ -1230,7 +1242,7 @@ int cc_batch_create(cc_ctx* const* ctxs, int n, cc_batch** out) {
 int cc_batch_destroy(cc_batch* b) {
   if (!b) return 0;
   if (b->s) hipStreamSynchronize(b->s);
-  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (BatchGraph& g : b->graphs) if (g.exec) hipGraphExecDestroy(g.exec);
   for (hipEvent_t e : b->join) if (e) hipEventDestroy(e);
   if (b->fork) hipEventDestroy(b->fork);
   if (b->ev0) hipEventDestroy(b->ev0);
@@ -1313,11 +1325,12 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
       }
       for (size_t i = 0; i < m; i++)   // outside any capture
         if ((r = ff_join(b->ctx[i], b->s))) { b->err = b->ctx[i]->err; return r; }
-      if ((r = batch_graph(b, args))) return r;
-      BCHK(b, hipGraphLaunch(b->exec, b->s));
+      BatchGraph* g = nullptr;
+      if ((r = batch_graph(b, args, &g))) return r;
+      BCHK(b, hipGraphLaunch(g->exec, b->s));
       for (size_t i = 0; i < m; i++) {
         cc_ctx* c = b->ctx[i];
-        c->cur = b->curAfter[i]; c->gb.cur = b->gridCurAfter[i]; c->counted = b->countedAfter[i]; c->stepNo += M;
+        c->cur = g->curAfter[i]; c->gb.cur = g->gridCurAfter[i]; c->counted = g->countedAfter[i]; c->stepNo += M;
       }
     }
     i0 += chunk * M;

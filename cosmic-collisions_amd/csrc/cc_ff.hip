@@ -338,12 +338,12 @@ __global__ void k_ff_feat(const float4* __restrict__ coarse, const float4* __res
 // A pass's centre record (one wavefront per block: its 64 fine texels counted
 // by ballot).  With featT given the pass becomes a box's base: its features
 // and meta are copied to featB and metaB, and *validB set.
-__global__ void __launch_bounds__(256) k_ff_base(const float4* __restrict__ fine, const float4* __restrict__ coarse,
-                                                 const float4* __restrict__ farA, const float4* __restrict__ farT0,
-                                                 const float4* __restrict__ farT1, const Box* __restrict__ bxp,
-                                                 int which, float* R, const float4* __restrict__ featT,
-                                                 const float* __restrict__ metaT, float4* featB, float* metaB,
-                                                 int* validB) {
+__device__ __forceinline__ void ff_base_body(const float4* __restrict__ fine, const float4* __restrict__ coarse,
+                                             const float4* __restrict__ farA, const float4* __restrict__ farT0,
+                                             const float4* __restrict__ farT1, const Box* __restrict__ bxp,
+                                             int which, float* R, const float4* __restrict__ featT,
+                                             const float* __restrict__ metaT, float4* featB, float* metaB,
+                                             int* validB) {
   int lane = threadIdx.x & 63;
   int v = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (v >= 4096) return;
@@ -375,6 +375,15 @@ __global__ void __launch_bounds__(256) k_ff_base(const float4* __restrict__ fine
   }
 }
 
+__global__ void __launch_bounds__(256) k_ff_base(const float4* __restrict__ fine, const float4* __restrict__ coarse,
+                                                 const float4* __restrict__ farA, const float4* __restrict__ farT0,
+                                                 const float4* __restrict__ farT1, const Box* __restrict__ bxp,
+                                                 int which, float* R, const float4* __restrict__ featT,
+                                                 const float* __restrict__ metaT, float4* featB, float* metaB,
+                                                 int* validB) {
+  ff_base_body(fine, coarse, farA, farT0, farT1, bxp, which, R, featT, metaT, featB, metaB, validB);
+}
+
 // The network's input for a pair of passes, the present one (T) and the base
 // (B), by voxel: channels 0-3 the present features, 4-7 their change since the
 // base over s, 8 and 9 the block's occupancy at T and at B, the rest zero.
@@ -387,12 +396,11 @@ __global__ void __launch_bounds__(256) k_ff_base(const float4* __restrict__ fine
 // from featT and metaT.
 constexpr int PAIR_BLOCK = 1024;
 template <bool MESH>
-__global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(float4* featT, float* metaT,
-                                                           const float4* __restrict__ featB, const float* __restrict__ metaB,
-                                                           const int* __restrict__ validB, uint16_t* x0, float* scale,
-                                                           int* ok, int* gate, const float4* __restrict__ coarse,
-                                                           const float4* __restrict__ mono, const Box* __restrict__ bxp,
-                                                           int which) {
+__device__ __forceinline__ void ff_pair_in_body(float4* featT, float* metaT, const float4* __restrict__ featB,
+                                                const float* __restrict__ metaB, const int* __restrict__ validB,
+                                                uint16_t* x0, float* scale, int* ok, int* gate,
+                                                const float4* __restrict__ coarse, const float4* __restrict__ mono,
+                                                const Box* __restrict__ bxp, int which) {
   __shared__ float redS[PAIR_BLOCK / 64];
   __shared__ int redN[PAIR_BLOCK / 64];
   const int tid = threadIdx.x;
@@ -451,6 +459,16 @@ __global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(float4* featT, float*
   }
 }
 
+template <bool MESH>
+__global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in(float4* featT, float* metaT,
+                                                           const float4* __restrict__ featB, const float* __restrict__ metaB,
+                                                           const int* __restrict__ validB, uint16_t* x0, float* scale,
+                                                           int* ok, int* gate, const float4* __restrict__ coarse,
+                                                           const float4* __restrict__ mono, const Box* __restrict__ bxp,
+                                                           int which) {
+  ff_pair_in_body<MESH>(featT, metaT, featB, metaB, validB, x0, scale, ok, gate, coarse, mono, bxp, which);
+}
+
 // Output and target units: the change of a block's record per unit of the
 // input's change is that of a mass s * (total mass) / 64; accelerations in
 // that mass over the block length squared, potentials over the block length,
@@ -481,10 +499,11 @@ __global__ void k_ff_pair_tgt(const float* __restrict__ RT, const float* __restr
 // network's change, carried from the block's centre to its present centre of
 // mass by the tidal tensor; empty blocks zero; blocks empty at the base left
 // to k_far.  Skips unless *gate, when given.
-__global__ void k_ff_out(const float* __restrict__ y, const float* __restrict__ RB, const float4* __restrict__ featB,
-                         const float* __restrict__ scale, const float4* __restrict__ coarse,
-                         const float* __restrict__ metaT, const Box* __restrict__ bxp, int which, float4* farA,
-                         float4* farT0, float4* farT1, const int* __restrict__ gate) {
+__device__ __forceinline__ void ff_out_body(const float* __restrict__ y, const float* __restrict__ RB,
+                                            const float4* __restrict__ featB, const float* __restrict__ scale,
+                                            const float4* __restrict__ coarse, const float* __restrict__ metaT,
+                                            const Box* __restrict__ bxp, int which, float4* farA, float4* farT0,
+                                            float4* farT1, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= 4096) return;
@@ -510,6 +529,12 @@ __global__ void k_ff_out(const float* __restrict__ y, const float* __restrict__ 
   farA[t] = make_float4(sx, sy, sz, p - (ax * dx + ay * dy + az * dz));
   farT0[t] = make_float4(txx, tyy, tzz, txy);
   farT1[t] = make_float4(txz, tyz, 0.0f, 0.0f);
+}
+__global__ void k_ff_out(const float* __restrict__ y, const float* __restrict__ RB, const float4* __restrict__ featB,
+                         const float* __restrict__ scale, const float4* __restrict__ coarse,
+                         const float* __restrict__ metaT, const Box* __restrict__ bxp, int which, float4* farA,
+                         float4* farT0, float4* farT1, const int* __restrict__ gate) {
+  ff_out_body(y, RB, featB, scale, coarse, metaT, bxp, which, farA, farT0, farT1, gate);
 }
 
 // The loss's weight of output channel c: accelerations 1, the potential and
@@ -550,12 +575,12 @@ __global__ void k_adam(float* w, const float* __restrict__ g, float* m, float* v
 // meshes is the difference of their far parts.  Blocks empty at the base
 // count no difference: the mesh steps the network stands in give them k_far's.
 constexpr int JUDGE_BLOCKS = 128;   // work-groups of 256 striding over the texels: one atomic pair each
-__global__ void __launch_bounds__(256) k_ff_judge_sum(const float4* __restrict__ com, const float4* __restrict__ ccom,
-                                                      const float4* __restrict__ Al, const float4* __restrict__ T0l,
-                                                      const float4* __restrict__ T1l, const float4* __restrict__ Ae,
-                                                      const float4* __restrict__ T0e, const float4* __restrict__ T1e,
-                                                      const float4* __restrict__ force,
-                                                      const float4* __restrict__ featB, float gm, double* sums) {
+__device__ __forceinline__ void ff_judge_sum_body(const float4* __restrict__ com, const float4* __restrict__ ccom,
+                                                  const float4* __restrict__ Al, const float4* __restrict__ T0l,
+                                                  const float4* __restrict__ T1l, const float4* __restrict__ Ae,
+                                                  const float4* __restrict__ T0e, const float4* __restrict__ T1e,
+                                                  const float4* __restrict__ force,
+                                                  const float4* __restrict__ featB, float gm, double* sums) {
   __shared__ float redD[4], redR[4];
   float d2 = 0.0f, r2 = 0.0f;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 512 * 512; e += gridDim.x * blockDim.x) {
@@ -585,12 +610,21 @@ __global__ void __launch_bounds__(256) k_ff_judge_sum(const float4* __restrict__
     atomicAdd(&sums[1], (double)redR[0] + (double)redR[1] + (double)redR[2] + (double)redR[3]);
   }
 }
+__global__ void __launch_bounds__(256) k_ff_judge_sum(const float4* __restrict__ com, const float4* __restrict__ ccom,
+                                                      const float4* __restrict__ Al, const float4* __restrict__ T0l,
+                                                      const float4* __restrict__ T1l, const float4* __restrict__ Ae,
+                                                      const float4* __restrict__ T0e, const float4* __restrict__ T1e,
+                                                      const float4* __restrict__ force,
+                                                      const float4* __restrict__ featB, float gm, double* sums) {
+  ff_judge_sum_body(com, ccom, Al, T0l, T1l, Ae, T0e, T1e, force, featB, gm, sums);
+}
 // The judge's verdict for the coming mesh steps (the sums put back to zero):
 // the network stands in while its relative error is within tol and it was
 // measured from a base that could serve.  stats: [0] last error, [1] judged,
 // [2] fallbacks, [3] learned mesh steps, [4] largest error measured from a
 // base that could serve, [5] their sum, [6] their count.
-__global__ void k_judge_decide(double* sums, float tol, const int* __restrict__ okBase, float* stats, int* gate) {
+__device__ __forceinline__ void judge_decide_body(double* sums, float tol, const int* __restrict__ okBase,
+                                                  float* stats, int* gate) {
   if (threadIdx.x != 0) return;
   double err = sums[1] > 0.0 ? sqrt(sums[0] / sums[1]) : 0.0;
   stats[0] = (float)err;
@@ -606,8 +640,44 @@ __global__ void k_judge_decide(double* sums, float tol, const int* __restrict__ 
   *gate = ok;
   sums[0] = 0.0; sums[1] = 0.0;
 }
+__global__ void k_judge_decide(double* sums, float tol, const int* __restrict__ okBase, float* stats, int* gate) {
+  judge_decide_body(sums, tol, okBase, stats, gate);
+}
 __global__ void k_count_learned(const int* flag, float* stats) {
   if (threadIdx.x == 0 && *flag) stats[3] += 1.0f;
+}
+
+// ------------------------------------------- batched: one pass over a batch --
+// Every (instance, box) of a batch at its mesh step is a slot of one forward
+// pass: the slot's input pair from its own mesh and base (work-group y = slot),
+// the forward pass over all slots, and each slot's output to its destination
+// (the mesh's far field on a step the network may stand in for, gated; the
+// judge's buffer on a judge step).  The judged slots are judged together at
+// the end of the batch's cycle, when every instance's exact pass is done.
+__global__ void __launch_bounds__(PAIR_BLOCK) k_ff_pair_in_b(const FFSlot* __restrict__ slots, uint16_t* x0,
+                                                             float* scale) {
+  const FFSlot& S = slots[blockIdx.x];
+  ff_pair_in_body<true>(S.featT, S.metaT, S.featB, S.metaB, S.validB, x0 + (size_t)blockIdx.x * 4096 * 16,
+                        scale + blockIdx.x, S.okBase, S.gate, S.coarse, S.mono, S.box, S.which);
+}
+__global__ void k_ff_out_b(const FFSlot* __restrict__ slots, const float* __restrict__ y, const float* __restrict__ scale) {
+  const FFSlot& S = slots[blockIdx.y];
+  ff_out_body(y + (size_t)blockIdx.y * 4096 * 16, S.RB, S.featB, scale + blockIdx.y, S.coarse, S.metaT, S.box,
+              S.which, S.A, S.T0, S.T1, S.gate);
+}
+__global__ void __launch_bounds__(256) k_ff_judge_sum_b(const FFJudgeSlot* __restrict__ slots) {
+  const FFJudgeSlot& J = slots[blockIdx.y];
+  ff_judge_sum_body(J.com, J.ccom, J.learned, J.learned + 4096, J.learned + 8192, J.farA, J.farT0, J.farT1, J.force,
+                    J.featB, J.gm, J.sums);
+}
+__global__ void k_judge_decide_b(const FFJudgeSlot* __restrict__ slots) {
+  const FFJudgeSlot& J = slots[blockIdx.x];
+  judge_decide_body(J.sums, J.tol, J.okBase, J.stats, J.gate);
+}
+__global__ void __launch_bounds__(256) k_ff_base_b(const FFJudgeSlot* __restrict__ slots) {
+  const FFJudgeSlot& J = slots[blockIdx.y];
+  ff_base_body(J.fine, J.coarse, J.farA, J.farT0, J.farT1, J.box, J.which, J.RB, J.featT, J.metaT, J.featB, J.metaB,
+               J.validB);
 }
 __global__ void k_zero_d(double* p, int n) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;

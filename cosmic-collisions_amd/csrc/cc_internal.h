@@ -193,6 +193,43 @@ void launch_books(int N, double mref, const float4* pos, const float4* vel, cons
 constexpr int FF_STATS = 8;
 struct FFNet;
 struct FFWork;
+// One (instance, box) of a batched inference: its mesh and box, its present features and base (from its
+// FFWork), its gate (a mesh step the network may stand in for) or none (a judge step), and where the
+// prediction goes.
+struct FFSlot {
+  const float4* coarse;
+  const float4* mono;
+  const Box* box;
+  int which;
+  float4* featT;
+  float* metaT;
+  const float4* featB;
+  const float* metaB;
+  const int* validB;
+  int* okBase;
+  int* gate;
+  const float* RB;
+  float4 *A, *T0, *T1;
+};
+// One (instance, box) judged in a batch: its exact pass (mesh, far field, force mesh), the prediction
+// (learned: farA, farT0, farT1 of 4096 each), the judge's sums, bound, statistics and gate, and the base
+// the exact pass becomes.
+struct FFJudgeSlot {
+  const float4 *fine, *com, *ccom, *coarse, *learned, *farA, *farT0, *farT1, *force;
+  const Box* box;
+  int which;
+  float gm, tol;
+  double* sums;
+  float* stats;
+  int* gate;
+  int* okBase;
+  float* RB;
+  const float4* featT;
+  const float* metaT;
+  float4* featB;
+  float* metaB;
+  int* validB;
+};
 size_t ff_weight_count();
 FFNet* ff_create(int cap, std::string& err);
 void ff_destroy(FFNet* n);
