@@ -78,7 +78,9 @@ struct cc_ctx {
   int* ffGate = nullptr;
   float* ffStats = nullptr;
   double* ffSums = nullptr;
-  float4* ffFar = nullptr;     // 3 x 4096: the network's farA, farT0, farT1 (judge steps)
+  float4* ffFar = nullptr;     // per box 3 x 4096 (FF_FAR): the network's farA, farT0, farT1 (judge steps)
+  bool ffBatched = false;      // while a batch captures a cycle whose inference and judge it runs itself
+  int headN = 0;               // the step whose head (one_step part 1) ran last
   // a judge runs on the second stream beside the steps after it: forked after the exact pass, joined by the end
   // of its cycle (or of the call that forked it), ffPending while it is not joined
   hipEvent_t ffFork = nullptr, ffJoin = nullptr;
@@ -490,6 +492,8 @@ int cc_cell(cc_ctx* c, int box) {
   LAUNCHCHK(c);
   return 0;
 }
+constexpr int FF_FAR = 3 * 4096;   // float4 per box of ffFar
+
 // A judge still running on the second stream joined into stream s (work on s
 // after this waits for it).
 static int ff_join(cc_ctx* c, hipStream_t s) {
@@ -499,6 +503,54 @@ static int ff_join(cc_ctx* c, hipStream_t s) {
   return 0;
 }
 
+// A box's deposit and coarse mesh (the deposit also zeroes the force mesh's
+// empty texels, which the cell pass leaves).
+static int box_head(cc_ctx* c, int b) {
+  MeshBuf& m = c->mesh[b];
+  launch_deposit(c->pos[c->cur], c->mat, c->N, c->visitEvery > 0 ? c->visit : nullptr, c->boxDev, b,
+                 (float)(FH * FH), m, true, c->s);
+  LAUNCHCHK(c);
+  return cc_coarse(c, b);
+}
+// A box's far field and per-cell pass, with a learned far field attached
+// decided by stepIdx (see mesh_all).  ffBatched: the batch predicts and judges
+// every instance's boxes in passes of its own, so only the exact far field (as
+// the gate leaves it) and the cell pass run here.
+static int box_tail(cc_ctx* c, int b, int stepIdx) {
+  MeshBuf& m = c->mesh[b];
+  float eps2 = (float)(0.5 * FH * FH), gm = (float)c->p.m;
+  if (!c->ff || stepIdx < 0) {
+    launch_cell(m, eps2, gm, c->cellBlocks, true, c->s);
+  } else if (stepIdx % c->ffJudge == 0) {
+    launch_far(m, m.farA, m.farT0, m.farT1, nullptr, c->s);
+    launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
+    if (!c->ffBatched) {
+      // The judge: the network's prediction from the box's base, its error against this exact pass and the
+      // verdict, then this pass becomes the base.  Nothing of this step waits for it, so it runs on the second
+      // stream (after the pairwise pass there) beside the steps that follow, joined by the end of the cycle;
+      // it reads this pass's mesh and box, which only the next cycle overwrites.
+      float4 *A = c->ffFar + FF_FAR * b, *T0 = A + 4096, *T1 = A + 8192;
+      HIPCHK(c, hipEventRecord(c->ffFork, c->s));
+      HIPCHK(c, hipStreamWaitEvent(c->s2, c->ffFork, 0));
+      ff_predict(c->ff, c->ffWork, b, m.coarse, m.mono, c->boxDev, A, T0, T1, nullptr, c->s2);
+      ff_judge_rebase(c->ffWork, b, m.fine, m.com, m.ccom, m.coarse, A, m.farA, m.farT0, m.farT1, m.force,
+                      c->boxDev, gm, c->ffSums + 2 * b, c->ffTol, c->ffStats + FF_STATS * b, c->ffGate + b, c->s2);
+      HIPCHK(c, hipEventRecord(c->ffJoin, c->s2));
+      c->ffPending = true;
+    }
+  } else {
+    // the prediction first: it clears the gate when the base cannot serve; then the exact far field, for
+    // every block when the network does not stand in, else for the blocks it has no base for
+    if (!c->ffBatched)
+      ff_predict(c->ff, c->ffWork, b, m.coarse, m.mono, c->boxDev, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
+    launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s, ff_base_features(c->ffWork, b));
+    launch_count_learned(c->ffGate + b, c->ffStats + FF_STATS * b, c->s);
+    launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
+  }
+  LAUNCHCHK(c);
+  return 0;
+}
+static int boxes_up(const cc_ctx* c) { return c->box.boxB > 0.5f ? 2 : 1; }
 // Deposit, coarse and per-cell passes of each box that is up.  With a
 // learned far field attached, stepIdx (the step of this mesh pass) decides:
 // on a judge step the exact far field makes the force mesh, the network's
@@ -507,44 +559,11 @@ static int ff_join(cc_ctx* c, hipStream_t s) {
 // field or the network's.  stepIdx < 0: exact only.
 static int mesh_all(cc_ctx* c, int stepIdx = -1) {
   int r;
-  for (int b = 0; b < 2; b++) {
-    if (b == 1 && !(c->box.boxB > 0.5f)) break;
-    MeshBuf& m = c->mesh[b];
-    // the deposit also zeroes the force mesh's empty texels, which the cell pass below leaves
-    launch_deposit(c->pos[c->cur], c->mat, c->N, c->visitEvery > 0 ? c->visit : nullptr, c->boxDev, b,
-                   (float)(FH * FH), m, true, c->s);
-    LAUNCHCHK(c);
-    if ((r = cc_coarse(c, b))) return r;
-    float eps2 = (float)(0.5 * FH * FH), gm = (float)c->p.m;
-    if (!c->ff || stepIdx < 0) {
-      launch_cell(m, eps2, gm, c->cellBlocks, true, c->s);
-    } else if (stepIdx % c->ffJudge == 0) {
-      launch_far(m, m.farA, m.farT0, m.farT1, nullptr, c->s);
-      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
-      // The judge: the network's prediction from the box's base, its error against this exact pass and the
-      // verdict, then this pass becomes the base.  Nothing of this step waits for it, so it runs on the second
-      // stream (after the pairwise pass there) beside the steps that follow, joined by the end of the cycle;
-      // it reads this pass's mesh and box, which only the next cycle overwrites.
-      float4 *A = c->ffFar, *T0 = c->ffFar + 4096, *T1 = c->ffFar + 8192;
-      HIPCHK(c, hipEventRecord(c->ffFork, c->s));
-      HIPCHK(c, hipStreamWaitEvent(c->s2, c->ffFork, 0));
-      ff_predict(c->ff, c->ffWork, b, m.coarse, m.mono, c->boxDev, A, T0, T1, nullptr, c->s2);
-      ff_judge_rebase(c->ffWork, b, m.fine, m.com, m.ccom, m.coarse, c->ffFar, m.farA, m.farT0, m.farT1, m.force,
-                      c->boxDev, gm, c->ffSums + 2 * b, c->ffTol, c->ffStats + FF_STATS * b, c->ffGate + b, c->s2);
-      HIPCHK(c, hipEventRecord(c->ffJoin, c->s2));
-      c->ffPending = true;
-    } else {
-      // the prediction first: it clears the gate when the base cannot serve; then the exact far field, for
-      // every block when the network does not stand in, else for the blocks it has no base for
-      ff_predict(c->ff, c->ffWork, b, m.coarse, m.mono, c->boxDev, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s);
-      launch_far(m, m.farA, m.farT0, m.farT1, c->ffGate + b, c->s, ff_base_features(c->ffWork, b));
-      launch_count_learned(c->ffGate + b, c->ffStats + FF_STATS * b, c->s);
-      launch_cell_only(m, m.farA, m.farT0, m.farT1, m.force, eps2, gm, c->cellBlocks, true, c->s);
-    }
-    LAUNCHCHK(c);
-  }
+  for (int b = 0; b < boxes_up(c); b++)
+    if ((r = box_head(c, b)) || (r = box_tail(c, b, stepIdx))) return r;
   return 0;
 }
+
 int cc_mesh(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   return mesh_all(c);
@@ -621,31 +640,36 @@ static int visit_due(cc_ctx* c, hipStream_t s) {
   return 0;
 }
 
-static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
-  int n = c->stepNo++;
+// One step.  part 0: the whole step; 1: a mesh step up to its boxes' coarse
+// meshes; 2: the rest of the mesh step whose head ran last (a batch predicts
+// every instance's far field in one pass between the two).
+static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a, int part = 0) {
+  const int n = part == 2 ? c->headN : c->stepNo++;
   int r;
   bool mesh = !c->p.field && n % c->p.meshEvery == 0;
-  if (mesh && (r = ff_join(c, c->s))) return r;
   // The pairwise pass reads the positions, masses and boxes and writes only its
   // own buffers, as the seats and the mesh do theirs: on a mesh step it runs on
   // the second stream beside them (a fork and a join, which a captured cycle
   // keeps as graph edges), and the step pass waits for all of them.
   bool side = mesh && c->ppCount > 0;
-  if (side) {
-    HIPCHK(c, hipEventRecord(c->ppFork, c->s));
-    HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
-    launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
-              (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppBounds, c->ppForce, c->s2);
-    LAUNCHCHK(c);
-    HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
-  }
-  if (n % 2 == 0) {   // a rebuild: seated from the count the last step pass took, or counted and seated here
-    if (c->counted) {
-      launch_slots_seat(c->N, c->gb, c->s);
+  if (part != 2) {
+    if (mesh && (r = ff_join(c, c->s))) return r;
+    if (side) {
+      HIPCHK(c, hipEventRecord(c->ppFork, c->s));
+      HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
+      launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+                (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppBounds, c->ppForce, c->s2);
       LAUNCHCHK(c);
-      c->counted = 0;
-    } else if ((r = cc_slots(c))) {
-      return r;
+      HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
+    }
+    if (n % 2 == 0) {   // a rebuild: seated from the count the last step pass took, or counted and seated here
+      if (c->counted) {
+        launch_slots_seat(c->N, c->gb, c->s);
+        LAUNCHCHK(c);
+        c->counted = 0;
+      } else if ((r = cc_slots(c))) {
+        return r;
+      }
     }
   }
   const bool count = c->fuseCount && n % 2 == 1;   // the next step rebuilds: this step pass takes its count
@@ -656,7 +680,20 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a) {
     b.gdt = c->p.dt;
     return sim_pass(c, &b, count);
   }
-  if (mesh && (r = mesh_all(c, n))) return r;
+  if (mesh) {
+    if (part == 1) {
+      for (int bx = 0; bx < boxes_up(c); bx++)
+        if ((r = box_head(c, bx))) return r;
+      c->headN = n;
+      return 0;
+    }
+    if (part == 2) {
+      for (int bx = 0; bx < boxes_up(c); bx++)
+        if ((r = box_tail(c, bx, n))) return r;
+    } else if ((r = mesh_all(c, n))) {
+      return r;
+    }
+  }
   if (side) HIPCHK(c, hipStreamWaitEvent(c->s, c->ppJoin, 0));
   cc_sim_args b = *a;
   b.settle = settle;
@@ -921,7 +958,7 @@ int cc_set_far_field(cc_ctx* c, cc_ff* f, int judgeEvery, double tol) {
     c->ffWork = ffwork_create(err);
     if (!c->ffWork) FAIL(c, CC_EDEVICE, "learned far field workspace: %s", err.c_str());
     ALLOC(c->ffGate, 2 * 4); ALLOC(c->ffStats, 2 * FF_STATS * 4); ALLOC(c->ffSums, 4 * 8);
-    ALLOC(c->ffFar, 3 * 4096 * 16);
+    ALLOC(c->ffFar, 2 * FF_FAR * 16);
     HIPCHK(c, hipEventCreateWithFlags(&c->ffFork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ffJoin, hipEventDisableTiming));
   }
@@ -1136,10 +1173,14 @@ struct cc_batch {
   std::vector<cc_ctx*> ctx;
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> join;
+  std::vector<hipEvent_t> join, mid;
+  hipEvent_t fork2 = nullptr;
   // the last few captured cycles, replaced in turn: cycles alternate between keys (a learned far
   // field's judge and the steps between, the step parity with an odd meshEvery)
-  BatchGraph graphs[4];
+  BatchGraph graphs[FF_BATCH_TABLES];
+  // with one learned far field attached to every instance: one inference over all instances' boxes per
+  // mesh step and one judge of the judged ones (the graph cache's slot tables, one per entry)
+  FFBatchWork* ffw = nullptr;
   int nextGraph = 0;
   double lastMs = 0.0;
   std::string err;
@@ -1166,14 +1207,48 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   }
   for (BatchGraph& g : b->graphs)
     if (g.exec && g.key == key) { *out = &g; return 0; }
-  BatchGraph& g = b->graphs[b->nextGraph];
-  b->nextGraph = (b->nextGraph + 1) % 4;
+  const int t = b->nextGraph;
+  BatchGraph& g = b->graphs[t];
+  b->nextGraph = (b->nextGraph + 1) % FF_BATCH_TABLES;
   if (g.exec) {   // its last launch may still run: let it end first
     BCHK(b, hipStreamSynchronize(b->s));
     hipGraphExecDestroy(g.exec);
     g.exec = nullptr;
   }
   size_t n = b->ctx.size();
+  // One learned far field on every instance: its inference runs once for the whole batch at the mesh step
+  // (every instance's boxes after their coarse meshes) and its judge once after the cycle.
+  FFNet* net = b->ctx[0]->ff;
+  bool batched = net != nullptr;
+  for (size_t i = 0; i < n; i++) batched = batched && b->ctx[i]->ff == net && !b->ctx[i]->p.field;
+  std::vector<FFSlot> slots;
+  std::vector<FFJudgeSlot> judged;
+  if (batched) {
+    if (!b->ffw) {
+      std::string err;
+      b->ffw = ffbatch_create((int)(2 * n), err);
+      if (!b->ffw) BFAIL(b, CC_EDEVICE, "batched far field workspace: %s", err.c_str());
+    }
+    for (size_t i = 0; i < n; i++) {
+      cc_ctx* c = b->ctx[i];
+      const bool judge = c->stepNo % c->ffJudge == 0;
+      for (int bx = 0; bx < boxes_up(c); bx++) {
+        MeshBuf& m = c->mesh[bx];
+        float4* A = c->ffFar + FF_FAR * bx;
+        if (judge) {
+          slots.push_back(ff_slot(c->ffWork, bx, m.coarse, m.mono, c->boxDev, nullptr, A, A + 4096, A + 8192));
+          judged.push_back(ff_judge_slot(c->ffWork, bx, m.fine, m.com, m.ccom, m.coarse, A, m.farA, m.farT0, m.farT1,
+                                         m.force, c->boxDev, (float)c->p.m, c->ffSums + 2 * bx, c->ffTol,
+                                         c->ffStats + FF_STATS * bx, c->ffGate + bx));
+        } else {
+          slots.push_back(ff_slot(c->ffWork, bx, m.coarse, m.mono, c->boxDev, c->ffGate + bx, m.farA, m.farT0,
+                                  m.farT1));
+        }
+      }
+    }
+    std::string err;
+    if (ffbatch_set(b->ffw, t, slots, judged, err)) BFAIL(b, CC_EDEVICE, "batched far field: %s", err.c_str());
+  }
   std::vector<int> stepNo(n), cur(n), gcur(n), counted(n);
   for (size_t i = 0; i < n; i++) {
     stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; counted[i] = b->ctx[i]->counted;
@@ -1181,15 +1256,49 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipEventRecord(b->fork, b->s);
   int r = 0;
-  for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
-    cc_ctx* c = b->ctx[i];
-    e = hipStreamWaitEvent(c->s, b->fork, 0);
-    if (e != hipSuccess) break;
-    launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
-    for (int k = 0; k < c->p.meshEvery && !r; k++) r = one_step(c, args[i].settle, args[i].kick, &args[i]);
-    if (r) { b->err = c->err; break; }
-    e = hipEventRecord(b->join[i], c->s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(b->s, b->join[i], 0);
+  if (!batched) {
+    for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
+      cc_ctx* c = b->ctx[i];
+      e = hipStreamWaitEvent(c->s, b->fork, 0);
+      if (e != hipSuccess) break;
+      launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
+      for (int k = 0; k < c->p.meshEvery && !r; k++) r = one_step(c, args[i].settle, args[i].kick, &args[i]);
+      if (r) { b->err = c->err; break; }
+      e = hipEventRecord(b->join[i], c->s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->s, b->join[i], 0);
+    }
+  } else {
+    for (size_t i = 0; i < n; i++) b->ctx[i]->ffBatched = true;
+    // every instance's mesh step up to its coarse meshes
+    for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
+      cc_ctx* c = b->ctx[i];
+      e = hipStreamWaitEvent(c->s, b->fork, 0);
+      if (e != hipSuccess) break;
+      launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
+      r = one_step(c, args[i].settle, args[i].kick, &args[i], 1);
+      if (r) { b->err = c->err; break; }
+      e = hipEventRecord(b->mid[i], c->s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->s, b->mid[i], 0);
+    }
+    // one inference over every box of every instance
+    if (!r && e == hipSuccess) {
+      ff_predict_batch(net, b->ffw, t, (int)slots.size(), b->s);
+      e = hipEventRecord(b->fork2, b->s);
+    }
+    // the rest of each instance's cycle
+    for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
+      cc_ctx* c = b->ctx[i];
+      e = hipStreamWaitEvent(c->s, b->fork2, 0);
+      if (e != hipSuccess) break;
+      r = one_step(c, args[i].settle, args[i].kick, &args[i], 2);
+      for (int k = 1; k < c->p.meshEvery && !r; k++) r = one_step(c, args[i].settle, args[i].kick, &args[i]);
+      if (r) { b->err = c->err; break; }
+      e = hipEventRecord(b->join[i], c->s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->s, b->join[i], 0);
+    }
+    // the judged boxes, their exact passes done
+    if (!r && e == hipSuccess) ff_judge_batch(b->ffw, t, (int)judged.size(), b->s);
+    for (size_t i = 0; i < n; i++) b->ctx[i]->ffBatched = false;
   }
   hipGraph_t graph = nullptr;
   hipError_t ee = hipStreamEndCapture(b->s, &graph);
@@ -1229,11 +1338,14 @@ int cc_batch_create(cc_ctx* const* ctxs, int n, cc_batch** out) {
   cc_batch* b = new cc_batch();
   b->ctx.assign(ctxs, ctxs + n);
   b->join.assign(n, nullptr);
+  b->mid.assign(n, nullptr);
   hipError_t e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   for (int i = 0; i < n && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+  for (int i = 0; i < n && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->mid[i], hipEventDisableTiming);
   if (e != hipSuccess) { g_err = std::string("batch create: ") + hipGetErrorString(e); cc_batch_destroy(b); return CC_EDEVICE; }
   *out = b;
   return 0;
@@ -1244,6 +1356,9 @@ int cc_batch_destroy(cc_batch* b) {
   if (b->s) hipStreamSynchronize(b->s);
   for (BatchGraph& g : b->graphs) if (g.exec) hipGraphExecDestroy(g.exec);
   for (hipEvent_t e : b->join) if (e) hipEventDestroy(e);
+  for (hipEvent_t e : b->mid) if (e) hipEventDestroy(e);
+  if (b->fork2) hipEventDestroy(b->fork2);
+  if (b->ffw) ffbatch_destroy(b->ffw);
   if (b->fork) hipEventDestroy(b->fork);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);

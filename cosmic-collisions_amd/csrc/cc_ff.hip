@@ -1074,6 +1074,81 @@ void launch_count_learned(const int* gate, float* stats, hipStream_t s) {
   hipLaunchKernelGGL(k_count_learned, dim3(1), dim3(64), 0, s, gate, stats);
 }
 
+FFSlot ff_slot(FFWork* wk, int which, const float4* coarse, const float4* mono, const Box* box, int* gate, float4* A,
+               float4* T0, float4* T1) {
+  FFSlot o;
+  o.coarse = coarse; o.mono = mono; o.box = box; o.which = which;
+  o.featT = wk->featT[which]; o.metaT = wk->metaT[which]; o.featB = wk->featB[which]; o.metaB = wk->metaB[which];
+  o.validB = wk->validB[which]; o.okBase = wk->okBase[which]; o.gate = gate; o.RB = wk->RB[which];
+  o.A = A; o.T0 = T0; o.T1 = T1;
+  return o;
+}
+FFJudgeSlot ff_judge_slot(FFWork* wk, int which, const float4* fine, const float4* com, const float4* ccom,
+                          const float4* coarse, const float4* learned, const float4* farA, const float4* farT0,
+                          const float4* farT1, const float4* force, const Box* box, float gm, double* sums, float tol,
+                          float* stats, int* gate) {
+  FFJudgeSlot o;
+  o.fine = fine; o.com = com; o.ccom = ccom; o.coarse = coarse; o.learned = learned;
+  o.farA = farA; o.farT0 = farT0; o.farT1 = farT1; o.force = force; o.box = box; o.which = which;
+  o.gm = gm; o.tol = tol; o.sums = sums; o.stats = stats; o.gate = gate; o.okBase = wk->okBase[which];
+  o.RB = wk->RB[which]; o.featT = wk->featT[which]; o.metaT = wk->metaT[which]; o.featB = wk->featB[which];
+  o.metaB = wk->metaB[which]; o.validB = wk->validB[which];
+  return o;
+}
+
+// A batch's workspace: activations for up to cap slots and, per captured cycle
+// of the batch's graph cache, the slot tables that cycle's launches read.
+struct FFBatchWork {
+  int cap = 0;
+  Acts a;
+  FFSlot* slots[FF_BATCH_TABLES] = {};
+  FFJudgeSlot* judge[FF_BATCH_TABLES] = {};
+  std::vector<void*> allocs;
+};
+FFBatchWork* ffbatch_create(int cap, std::string& err) {
+  FFBatchWork* w = new FFBatchWork();
+  w->cap = cap;
+  bool ok = acts_alloc(w->a, cap, false, err);
+  for (int t = 0; ok && t < FF_BATCH_TABLES; t++)
+    ok = dev_alloc(w->allocs, (void**)&w->slots[t], sizeof(FFSlot) * cap, err) &&
+         dev_alloc(w->allocs, (void**)&w->judge[t], sizeof(FFJudgeSlot) * cap, err);
+  if (!ok) { ffbatch_destroy(w); return nullptr; }
+  return w;
+}
+void ffbatch_destroy(FFBatchWork* w) {
+  if (!w) return;
+  acts_free(w->a);
+  for (void* p : w->allocs) hipFree(p);
+  delete w;
+}
+int ffbatch_cap(const FFBatchWork* w) { return w->cap; }
+int ffbatch_set(FFBatchWork* w, int t, const std::vector<FFSlot>& slots, const std::vector<FFJudgeSlot>& judge,
+                std::string& err) {
+  if (t < 0 || t >= FF_BATCH_TABLES || (int)slots.size() > w->cap || (int)judge.size() > w->cap) {
+    err = "batched far field: slot table out of range";
+    return -1;
+  }
+  hipError_t e = hipSuccess;
+  if (!slots.empty()) e = hipMemcpy(w->slots[t], slots.data(), sizeof(FFSlot) * slots.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !judge.empty())
+    e = hipMemcpy(w->judge[t], judge.data(), sizeof(FFJudgeSlot) * judge.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { err = hipGetErrorString(e); return -1; }
+  return 0;
+}
+void ff_predict_batch(const FFNet* n, FFBatchWork* w, int t, int nSlots, hipStream_t s) {
+  if (nSlots <= 0) return;
+  Acts& a = w->a;
+  hipLaunchKernelGGL(k_ff_pair_in_b, dim3(nSlots), dim3(PAIR_BLOCK), 0, s, w->slots[t], a.x0, a.scale);
+  forward(n, a, nSlots, false, nullptr, s);
+  hipLaunchKernelGGL(k_ff_out_b, dim3(16, nSlots), dim3(256), 0, s, w->slots[t], a.y, a.scale);
+}
+void ff_judge_batch(FFBatchWork* w, int t, int nJudge, hipStream_t s) {
+  if (nJudge <= 0) return;
+  hipLaunchKernelGGL(k_ff_judge_sum_b, dim3(JUDGE_BLOCKS, nJudge), dim3(256), 0, s, w->judge[t]);
+  hipLaunchKernelGGL(k_judge_decide_b, dim3(nJudge), dim3(64), 0, s, w->judge[t]);
+  hipLaunchKernelGGL(k_ff_base_b, dim3(1024, nJudge), dim3(256), 0, s, w->judge[t]);
+}
+
 // a recorded pass: box `which`'s features and the centre record of its exact
 // far field; -1 when the record is full
 int ff_record_pass(FFNet* n, const float4* fine, const float4* coarse, const float4* mono, const float4* farA, const float4* farT0, const float4* farT1, const Box* bx, int which,

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 struct cc_ff;
 struct cc_report;
@@ -249,6 +250,26 @@ void ff_judge_rebase(FFWork* wk, int which, const float4* fine, const float4* co
 void ff_forget_bases(FFWork* w, hipStream_t s);
 // the features of box `which`'s base by block (what k_far reads to find the blocks the network has no base for)
 const float4* ff_base_features(const FFWork* w, int which);
+// a ctx's slot of a batched inference for box `which` (gate: a mesh step between judges; null: a judge step),
+// and its judged slot
+FFSlot ff_slot(FFWork* wk, int which, const float4* coarse, const float4* mono, const Box* box, int* gate, float4* A,
+               float4* T0, float4* T1);
+FFJudgeSlot ff_judge_slot(FFWork* wk, int which, const float4* fine, const float4* com, const float4* ccom,
+                          const float4* coarse, const float4* learned, const float4* farA, const float4* farT0,
+                          const float4* farT1, const float4* force, const Box* box, float gm, double* sums, float tol,
+                          float* stats, int* gate);
+// a batch's workspace for batched inference: activations for up to cap slots, a slot table per captured cycle
+constexpr int FF_BATCH_TABLES = 4;
+struct FFBatchWork;
+FFBatchWork* ffbatch_create(int cap, std::string& err);
+void ffbatch_destroy(FFBatchWork* w);
+int ffbatch_cap(const FFBatchWork* w);
+// table t's slots (synchronous upload; no launch of a cycle using table t may be running)
+int ffbatch_set(FFBatchWork* w, int t, const std::vector<FFSlot>& slots, const std::vector<FFJudgeSlot>& judge,
+                std::string& err);
+// one forward pass over table t's nSlots slots; the judge of table t's nJudge judged slots
+void ff_predict_batch(const FFNet* n, FFBatchWork* w, int t, int nSlots, hipStream_t s);
+void ff_judge_batch(FFBatchWork* w, int t, int nJudge, hipStream_t s);
 // a recorded pass of box `which`'s exact far field (features, centre record); -1 when the record is full
 int ff_record_pass(FFNet* n, const float4* fine, const float4* coarse, const float4* mono, const float4* farA, const float4* farT0, const float4* farT1, const Box* bx, int which,
                    hipStream_t s);

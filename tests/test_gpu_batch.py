@@ -77,6 +77,47 @@ def test_batch_equals_instances_alone():
             c.close()
 
 
+def test_batch_with_learned_far_field_equals_instances_alone():
+    """One learned far field on every member: the batch predicts every member's
+    boxes in one forward pass per mesh step and judges them together after the
+    cycle; each member alone predicts and judges its own.  The states are bit
+    for bit the same after every call (a loose bound, so the network stands in
+    between judges from the second judge on), and so are the judges' verdicts."""
+    ms = members()
+    alone = [make(m) for m in ms]
+    together = [make(m) for m in ms]
+    ff = ccamd.FarField(capacity=0)   # its seeded initial weights: any prediction will do
+    batch = None
+    try:
+        for ctx in alone + together:
+            ctx.set_far_field(ff, 16, 1e9)
+        batch = ccamd.Batch(together)
+        args = ccamd.sim_args()
+        for call, n in enumerate((24, 21, 35)):
+            start = alone[0].step_no
+            print("batch with the network, call", call, "steps", n, "from", start, flush=True)
+            bl = [boxes(m, start, n) for m in ms]
+            for ctx, b in zip(alone, bl):
+                ctx.step_n(n, args, b)
+            batch.step_n(n, args, bl)
+            for i, (a, t) in enumerate(zip(alone, together)):
+                for x, y, what in zip(a.download(), t.download(), ("pos", "vel", "aux", "q", "mat")):
+                    assert np.array_equal(x, y), (call, i, what)
+                sa, st = a.ff_stats(), t.ff_stats()
+                assert np.array_equal(sa[:, [1, 2, 3, 6]], st[:, [1, 2, 3, 6]]), (call, i, sa, st)
+                assert np.allclose(sa[:, 0], st[:, 0], rtol=1e-9, atol=0), (call, i)
+            for ctx in alone + together:   # eager work between calls
+                ctx.step(0, 0, args)
+                ctx.sync()
+        assert all(t.ff_stats()[0, 3] > 0 for t in together)
+    finally:
+        if batch is not None:
+            batch.close()
+        for c in alone + together:
+            c.close()
+        ff.close()
+
+
 def test_batch_rejects_misaligned_members():
     ms = members()[:2]
     ctxs = [make(m) for m in ms]
