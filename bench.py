@@ -130,16 +130,17 @@ def _books(ctx):
 
 
 def learned_leg(state_root, batch_dir, steps, warmup, reuse, cycles, sweep, sweep_cycles, iters, batch_steps):
-    """The learned far field on the shatter impact, a configuration it was not
-    trained on: trained on the GPU from the exact far field of the theia-like and
-    twins runs and the batched sweep's instances, then judged in the shatter run
-    (every 16 steps, fallback above a 1e-3 relative force error) beside the same
-    run with the exact far field and with a zero network (each mesh step between
-    judges keeps the base's far field: what the network's prediction adds to)."""
+    """The learned far field, trained on the GPU from the exact far field of the
+    theia-like and twins runs and a sweep of the generator's next seed, on what it
+    was not trained on: the shatter impact, judged every 16 steps with fallback
+    above a 1e-3 relative force error, beside the same run with the exact far
+    field and with a zero network (each mesh step between judges keeps the base's
+    far field: what the network's prediction adds to); and the batched sweep
+    (C4) with the network on every instance, one inference per mesh step for
+    the whole batch, beside the same batch with the exact far field."""
     import ff_train
     ff, rep = ff_train.build(["THEIA", "TWINS"], state_root, cycles, iters, reuse=reuse, sweep=sweep,
-                             sweep_cycles=sweep_cycles, batch_dir=batch_dir,
-                             log=lambda m: print(m, file=sys.stderr, flush=True))
+                             sweep_cycles=sweep_cycles, log=lambda m: print(m, file=sys.stderr, flush=True))
     zero = ccamd.FarField(capacity=0)
     zero.set_weights(np.zeros(zero.nw, np.float32))
     run, st = runs.prepare(runs.SHATTER, os.path.join(state_root, "cc_shatter_state"), reuse=True)
@@ -171,14 +172,12 @@ def learned_leg(state_root, batch_dir, steps, warmup, reuse, cycles, sweep, swee
                       "mean_force_err": st_["sum_err"] / max(st_["usable"], 1.0)})
         out[mode] = o
         R.close()
-    # the batched form: a sweep the network never saw (the next seed), 64 instances stepped as one batch with
-    # the exact far field and with the network attached to every instance
-    if batch_steps <= 0:
-        insts = []
-    else:
-        insts = runs.prepare_batch(batch_dir + "_unseen", 64, seed=runs.BATCH_SEED + 1, reuse=True)
-    bo = {"workload": "64 giant impacts of the sweep's next seed (none in the training data), one batch, %d steps "
-                      "after 24" % batch_steps}
+    # the batched form: the C4 sweep (none of its instances in the training data), 64 instances stepped as one
+    # batch with the exact far field and with the network attached to every instance
+    insts = runs.prepare_batch(batch_dir, 64, reuse=True) if batch_steps > 0 else []
+    bo = {"workload": "C4: the batched sweep's 64 giant impacts (none in the training data), one batch, %d steps "
+                      "after 24; one inference over every instance's boxes per mesh step, one judge of the judged "
+                      "ones after each 8-step cycle" % batch_steps}
     for mode in ("exact", "learned") if insts else ():
         B = runs.BatchRunner(insts)
         if mode == "learned":
@@ -299,7 +298,7 @@ def main():
     ap.add_argument("--ff-sweep", type=int, default=64, help="sweep instances recorded for training (0: none)")
     ap.add_argument("--ff-sweep-cycles", type=int, default=40)
     ap.add_argument("--ff-iters", type=int, default=6000)
-    ap.add_argument("--ff-batch-steps", type=int, default=96, help="steps of the unseen sweep with the network (0: skip)")
+    ap.add_argument("--ff-batch-steps", type=int, default=240, help="steps of the C4 sweep with the network (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X)", file=sys.stderr)

@@ -6,8 +6,10 @@ and the exact far field carried to the block centres).  A training sample
 pairs two passes of one box as a mesh step would: the prediction from the
 earlier pass (the base) to the later one, 8 steps on (a mesh step between
 judges) or 16 (a judge step).  The runs are the named configurations (the
-theia-like and twins impacts by default) and the instances of the batched
-what-if sweep (giant impacts of uneven size, mass ratio, angle and speed).
+theia-like and twins impacts by default) and the instances of a what-if sweep
+of their own (giant impacts of uneven size, mass ratio, angle and speed; the
+sweep generator's next seed, so that the batched sweep a network is measured
+on is none of them).
 The network is trained by the library's own BF16 matrix-core forward and
 backward kernels with Adam (cc_ff_train).  The last quarter of each run's
 passes is held out.
@@ -68,10 +70,13 @@ def capacity(configs, cycles, sweep, sweep_cycles):
     return 2 * 2 * (len(configs) * cycles + sweep * sweep_cycles) + 16
 
 
-def collect_sweep(ff, batch_dir, count, cycles, reuse=True, log=print):
-    """The batched sweep's instances stepped together (runs.BatchRunner), a
-    recorded pass per instance and box after each cycle's mesh pass."""
-    B = runs.BatchRunner(runs.prepare_batch(batch_dir, count, reuse=reuse))
+SWEEP_SEED = runs.BATCH_SEED + 1   # the training sweep: not the batched sweep's own seed
+
+
+def collect_sweep(ff, batch_dir, count, cycles, seed=SWEEP_SEED, reuse=True, log=print):
+    """A sweep's instances stepped together (runs.BatchRunner), a recorded
+    pass per instance and box after each cycle's mesh pass."""
+    B = runs.BatchRunner(runs.prepare_batch(batch_dir, count, seed=seed, reuse=reuse))
     seqs = [[[], []] for _ in B.rs]
     for _ in range(cycles):
         B.steps(8)
@@ -86,9 +91,9 @@ def collect_sweep(ff, batch_dir, count, cycles, reuse=True, log=print):
 
 
 def build(configs, state_root, cycles, iters, reuse=True, slope=1.0, sweep=0, sweep_cycles=40, batch_dir=None,
-          lr=2e-3, log=print):
+          lr=2e-3, sweep_seed=SWEEP_SEED, log=print):
     """A FarField trained on the named configurations (runs.py) and, with
-    sweep > 0, the batched sweep's first `sweep` instances: (ff, report).
+    sweep > 0, `sweep` instances of the sweep of seed `sweep_seed`: (ff, report).
     Training samples come first in the record, held-out ones after them."""
     ff = ccamd.FarField(capacity=capacity(configs, cycles, sweep, sweep_cycles))
     ff.set_slope(slope)
@@ -102,8 +107,8 @@ def build(configs, state_root, cycles, iters, reuse=True, slope=1.0, sweep=0, sw
         log("ff passes: %s %s" % (name, [len(q) for q in got]))
         seqs += got
     if sweep > 0:
-        seqs += collect_sweep(ff, batch_dir or os.path.join(state_root, "cc_batch_state"), sweep, sweep_cycles,
-                              reuse=reuse, log=log)
+        seqs += collect_sweep(ff, batch_dir or os.path.join(state_root, "cc_train_sweep_state"), sweep, sweep_cycles,
+                              seed=sweep_seed, reuse=reuse, log=log)
     # each box's passes: the first three quarters train, the last quarter is held out
     cut = [len(q) - len(q) // 4 for q in seqs]
     first = ff.count
@@ -119,7 +124,8 @@ def build(configs, state_root, cycles, iters, reuse=True, slope=1.0, sweep=0, sw
     t2 = time.perf_counter()
     train_err = ff.eval(first, n_train)
     held_err = ff.eval(first + n_train, n_held) if n_held else (float("nan"), float("nan"))
-    rep = {"configs": list(configs), "sweep_instances": sweep, "slope": slope, "passes": ff.passes,
+    rep = {"configs": list(configs), "sweep_instances": sweep, "sweep_seed": sweep_seed, "slope": slope,
+           "passes": ff.passes,
            "samples": n_train, "held_out": n_held, "iters": iters, "collect_s": t1 - t0, "train_s": t2 - t1,
            "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-20:])),
            "what": "relative L2 error of the predicted change of the blocks' far-field acceleration since the "

@@ -3,7 +3,7 @@ configuration: 262,144 particles, a 0.5 M_earth impactor head-on at 2.6
 v_esc), against the same schedule without the network.
 
 The network is trained on the GPU (ff_train.build: passes of the theia-like and
-twins runs and the batched sweep's 64 instances, 6000 Adam steps) and attached
+twins runs and 64 instances of a sweep of its own, 6000 Adam steps) and attached
 to the shatter run from first touch for 1000 steps: judged against the exact
 far field every 16 steps, standing in on the mesh steps between judges while
 its last error was within 1e-3.  The ablation is a zero network: every mesh
@@ -16,7 +16,11 @@ mean error of 5e-4; the zero network on 6, its mean error 1.15e-3.  The bars
 below leave room for the run-to-run spread of training (its weight gradients
 are summed with atomics).  Drift: the angular momentum after the run within
 1e-4 of the start (the reference's books allow a fraction of a percent) and
-the final energy within 1e-3 of the exact run's.
+the final energy within 1e-3 of the exact run's.  Then the batched sweep (C4,
+64 impacts of uneven size, none in the training data) with the network on every
+instance, one inference over the whole batch per mesh step: measured, 960 of
+the 1024 judged steps' successors on the network (the 64 fallbacks each
+instance's first judge, before it has a base), the largest error 5.3e-4.
 """
 import os
 import sys
@@ -55,7 +59,7 @@ def run_shatter(run, st, net):
 
 def test_learned_far_field_stands_in_on_an_unseen_impact():
     root = tempfile.gettempdir()
-    print("training: theia-like, twins, 64 sweep instances", flush=True)
+    print("training: theia-like, twins, 64 instances of the training sweep", flush=True)
     ff, rep = ff_train.build(["THEIA", "TWINS"], root, 250, 6000, sweep=64, sweep_cycles=40, reuse=True,
                              log=lambda m: print(m, flush=True))
     zero = ccamd.FarField(capacity=0)
@@ -85,6 +89,21 @@ def test_learned_far_field_stands_in_on_an_unseen_impact():
         print("drift: |dJ|/|J| exact %.2e learned %.2e; energy against the exact run %.2e" %
               (np.linalg.norm(J(be) - J(b0)) / np.linalg.norm(J(b0)), dJ, dE))
         assert dJ < 1e-4 and dE < 1e-3
+        # the batched sweep (C4: 64 impacts, none in the training data) with the network on every instance,
+        # one inference per mesh step for the whole batch: every instance within the bound
+        print("C4 batch with the network", flush=True)
+        B = runs.BatchRunner(runs.prepare_batch(os.path.join(root, "cc_batch_state"), 64, reuse=True))
+        try:
+            for r in B.rs:
+                r.ctx.set_far_field(ff, 16, 1e-3)
+            B.steps(160)
+            st = np.array([r.ctx.ff_stats() for r in B.rs])   # instance, box, stats
+        finally:
+            B.close()
+        tot = st.sum(axis=(0, 1))
+        print("C4: judged %d fallbacks %d learned %d mean %.3e max %.3e" %
+              (tot[1], tot[2], tot[3], tot[5] / max(tot[6], 1), st[:, :, 4].max()))
+        assert tot[3] >= 0.8 * tot[1] and st[:, :, 4].max() <= 1e-3
     finally:
         ff.close()
         zero.close()
