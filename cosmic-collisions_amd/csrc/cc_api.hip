@@ -1199,6 +1199,50 @@ struct cc_batch {
     if (_e != hipSuccess) BFAIL(b, CC_EDEVICE, "%s: %s", #call, hipGetErrorString(_e));       \
   } while (0)
 
+// One learned far field on every instance (none under constant gravity): its inference then runs once for the
+// whole batch at the mesh step (every instance's boxes after their coarse meshes) and its judge once after it.
+static FFNet* batch_net(const cc_batch* b) {
+  FFNet* net = b->ctx[0]->ff;
+  for (const cc_ctx* c : b->ctx)
+    if (c->ff != net || c->p.field) return nullptr;
+  return net;
+}
+
+// Slot table t for the mesh step every instance is at: a slot per box that is up (a judge step's prediction into
+// the instance's own buffer, else into its far field behind the gate), a judged slot per box on a judge step.
+static int batch_ff_table(cc_batch* b, int t, int* nSlots, int* nJudged) {
+  size_t n = b->ctx.size();
+  if (!b->ffw) {
+    std::string err;
+    b->ffw = ffbatch_create((int)(2 * n), err);
+    if (!b->ffw) BFAIL(b, CC_EDEVICE, "batched far field workspace: %s", err.c_str());
+  }
+  std::vector<FFSlot> slots;
+  std::vector<FFJudgeSlot> judged;
+  for (size_t i = 0; i < n; i++) {
+    cc_ctx* c = b->ctx[i];
+    const bool judge = c->stepNo % c->ffJudge == 0;
+    for (int bx = 0; bx < boxes_up(c); bx++) {
+      MeshBuf& m = c->mesh[bx];
+      float4* A = c->ffFar + FF_FAR * bx;
+      if (judge) {
+        slots.push_back(ff_slot(c->ffWork, bx, m.coarse, m.mono, c->boxDev, nullptr, A, A + 4096, A + 8192));
+        judged.push_back(ff_judge_slot(c->ffWork, bx, m.fine, m.com, m.ccom, m.coarse, A, m.farA, m.farT0, m.farT1,
+                                       m.force, c->boxDev, (float)c->p.m, c->ffSums + 2 * bx, c->ffTol,
+                                       c->ffStats + FF_STATS * bx, c->ffGate + bx));
+      } else {
+        slots.push_back(ff_slot(c->ffWork, bx, m.coarse, m.mono, c->boxDev, c->ffGate + bx, m.farA, m.farT0,
+                                m.farT1));
+      }
+    }
+  }
+  std::string err;
+  if (ffbatch_set(b->ffw, t, slots, judged, err)) BFAIL(b, CC_EDEVICE, "batched far field: %s", err.c_str());
+  *nSlots = (int)slots.size();
+  *nJudged = (int)judged.size();
+  return 0;
+}
+
 static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   std::vector<double> key;
   for (size_t i = 0; i < b->ctx.size(); i++) {
@@ -1216,46 +1260,17 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
     g.exec = nullptr;
   }
   size_t n = b->ctx.size();
-  // One learned far field on every instance: its inference runs once for the whole batch at the mesh step
-  // (every instance's boxes after their coarse meshes) and its judge once after the cycle.
-  FFNet* net = b->ctx[0]->ff;
-  bool batched = net != nullptr;
-  for (size_t i = 0; i < n; i++) batched = batched && b->ctx[i]->ff == net && !b->ctx[i]->p.field;
-  std::vector<FFSlot> slots;
-  std::vector<FFJudgeSlot> judged;
-  if (batched) {
-    if (!b->ffw) {
-      std::string err;
-      b->ffw = ffbatch_create((int)(2 * n), err);
-      if (!b->ffw) BFAIL(b, CC_EDEVICE, "batched far field workspace: %s", err.c_str());
-    }
-    for (size_t i = 0; i < n; i++) {
-      cc_ctx* c = b->ctx[i];
-      const bool judge = c->stepNo % c->ffJudge == 0;
-      for (int bx = 0; bx < boxes_up(c); bx++) {
-        MeshBuf& m = c->mesh[bx];
-        float4* A = c->ffFar + FF_FAR * bx;
-        if (judge) {
-          slots.push_back(ff_slot(c->ffWork, bx, m.coarse, m.mono, c->boxDev, nullptr, A, A + 4096, A + 8192));
-          judged.push_back(ff_judge_slot(c->ffWork, bx, m.fine, m.com, m.ccom, m.coarse, A, m.farA, m.farT0, m.farT1,
-                                         m.force, c->boxDev, (float)c->p.m, c->ffSums + 2 * bx, c->ffTol,
-                                         c->ffStats + FF_STATS * bx, c->ffGate + bx));
-        } else {
-          slots.push_back(ff_slot(c->ffWork, bx, m.coarse, m.mono, c->boxDev, c->ffGate + bx, m.farA, m.farT0,
-                                  m.farT1));
-        }
-      }
-    }
-    std::string err;
-    if (ffbatch_set(b->ffw, t, slots, judged, err)) BFAIL(b, CC_EDEVICE, "batched far field: %s", err.c_str());
-  }
+  FFNet* net = batch_net(b);
+  const bool batched = net != nullptr;
+  int nSlots = 0, nJudged = 0, r;
+  if (batched && (r = batch_ff_table(b, t, &nSlots, &nJudged))) return r;
   std::vector<int> stepNo(n), cur(n), gcur(n), counted(n);
   for (size_t i = 0; i < n; i++) {
     stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; counted[i] = b->ctx[i]->counted;
   }
   BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipEventRecord(b->fork, b->s);
-  int r = 0;
+  r = 0;
   if (!batched) {
     for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
       cc_ctx* c = b->ctx[i];
@@ -1282,7 +1297,7 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
     }
     // one inference over every box of every instance
     if (!r && e == hipSuccess) {
-      ff_predict_batch(net, b->ffw, t, (int)slots.size(), b->s);
+      ff_predict_batch(net, b->ffw, t, nSlots, b->s);
       e = hipEventRecord(b->fork2, b->s);
     }
     // the rest of each instance's cycle
@@ -1297,7 +1312,7 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
       if (e == hipSuccess) e = hipStreamWaitEvent(b->s, b->join[i], 0);
     }
     // the judged boxes, their exact passes done
-    if (!r && e == hipSuccess) ff_judge_batch(b->ffw, t, (int)judged.size(), b->s);
+    if (!r && e == hipSuccess) ff_judge_batch(b->ffw, t, nJudged, b->s);
     for (size_t i = 0; i < n; i++) b->ctx[i]->ffBatched = false;
   }
   hipGraph_t graph = nullptr;
@@ -1320,6 +1335,56 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   if (e != hipSuccess) { g.exec = nullptr; BFAIL(b, CC_EDEVICE, "batch instantiate: %s", hipGetErrorString(e)); }
   g.key = key;
   *out = &g;
+  return 0;
+}
+
+// One step of every instance outside a whole cycle (a call that starts or ends mid-cycle).  Each instance's
+// launches go to its own stream, forked from the batch stream and joined back, so the instances' passes overlap
+// as in a captured cycle.  A mesh step of a batch with one learned far field runs as a captured cycle's does:
+// every instance up to its coarse meshes, one inference over all their boxes (the eager slot table), the rest of
+// the step, then the judge of the judged boxes.
+static int batch_step_eager(cc_batch* b, const cc_sim_args* args, const std::vector<const double*>& L,
+                            const std::vector<int>& nL, std::vector<int>& used) {
+  const size_t n = b->ctx.size();
+  const int M = b->ctx[0]->p.meshEvery;
+  const bool mesh = b->ctx[0]->stepNo % M == 0;
+  FFNet* net = mesh ? batch_net(b) : nullptr;
+  int nSlots = 0, nJudged = 0, r;
+  if (net) {
+    BCHK(b, hipStreamSynchronize(b->s));   // the eager table's last inference and judge have ended
+    if ((r = batch_ff_table(b, FF_EAGER_TABLE, &nSlots, &nJudged))) return r;
+  }
+  BCHK(b, hipEventRecord(b->fork, b->s));
+  for (size_t i = 0; i < n; i++) {
+    cc_ctx* c = b->ctx[i];
+    BCHK(b, hipStreamWaitEvent(c->s, b->fork, 0));
+    if (mesh && L[i] && used[i] < nL[i]) {
+      const double* q = L[i] + 7 * used[i]++;
+      if ((r = cc_set_boxes(c, q, q + 3, q[6] > 0.5))) { b->err = c->err; return r; }
+    }
+    if ((r = visit_due(c, c->s))) { b->err = c->err; return r; }
+    c->ffBatched = net != nullptr;
+    r = one_step(c, args[i].settle, args[i].kick, &args[i], net ? 1 : 0);
+    c->ffBatched = false;
+    if (r) { b->err = c->err; return r; }
+    hipEvent_t ev = net ? b->mid[i] : b->join[i];
+    BCHK(b, hipEventRecord(ev, c->s));
+    BCHK(b, hipStreamWaitEvent(b->s, ev, 0));
+  }
+  if (!net) return 0;
+  ff_predict_batch(net, b->ffw, FF_EAGER_TABLE, nSlots, b->s);
+  BCHK(b, hipEventRecord(b->fork2, b->s));
+  for (size_t i = 0; i < n; i++) {
+    cc_ctx* c = b->ctx[i];
+    BCHK(b, hipStreamWaitEvent(c->s, b->fork2, 0));
+    c->ffBatched = true;
+    r = one_step(c, args[i].settle, args[i].kick, &args[i], 2);
+    c->ffBatched = false;
+    if (r) { b->err = c->err; return r; }
+    BCHK(b, hipEventRecord(b->join[i], c->s));
+    BCHK(b, hipStreamWaitEvent(b->s, b->join[i], 0));
+  }
+  ff_judge_batch(b->ffw, FF_EAGER_TABLE, nJudged, b->s);
   return 0;
 }
 
@@ -1390,28 +1455,10 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
     BCHK(b, hipEventRecord(b->join[i], b->ctx[i]->s));
     BCHK(b, hipStreamWaitEvent(b->s, b->join[i], 0));
   }
-  auto eager = [&](size_t i) -> int {
-    cc_ctx* c = b->ctx[i];
-    if (c->stepNo % M == 0 && L[i] && used[i] < nL[i]) {
-      const double* q = L[i] + 7 * used[i]++;
-      int e = cc_set_boxes(c, q, q + 3, q[6] > 0.5);
-      if (e) { b->err = c->err; return e; }
-    }
-    int e = visit_due(c, c->s);
-    if (!e) e = one_step(c, args[i].settle, args[i].kick, &args[i]);
-    if (e) b->err = c->err;
-    return e;
-  };
-  // eager on the batch stream (ordered after every instance's earlier work) up to the cycle start
+  // steps outside a whole cycle: one at a time, every instance on its own stream
   int i0 = 0, r;
   while (i0 < n && b->ctx[0]->stepNo % M != 0) {
-    for (size_t i = 0; i < m; i++) {
-      hipStream_t own = b->ctx[i]->s;
-      b->ctx[i]->s = b->s;
-      r = eager(i);
-      b->ctx[i]->s = own;
-      if (r) return r;
-    }
+    if ((r = batch_step_eager(b, args, L, nL, used))) return r;
     i0++;
   }
   int cycles = (n - i0) / M;
@@ -1452,13 +1499,7 @@ int cc_batch_step_n(cc_batch* b, int n, const cc_sim_args* args, const double* b
     cycles -= chunk;
   }
   for (; i0 < n; i0++)
-    for (size_t i = 0; i < m; i++) {
-      hipStream_t own = b->ctx[i]->s;
-      b->ctx[i]->s = b->s;
-      r = eager(i);
-      b->ctx[i]->s = own;
-      if (r) return r;
-    }
+    if ((r = batch_step_eager(b, args, L, nL, used))) return r;
   for (size_t i = 0; i < m; i++)
     if ((r = ff_join(b->ctx[i], b->s))) { b->err = b->ctx[i]->err; return r; }
   BCHK(b, hipEventRecord(b->ev1, b->s));

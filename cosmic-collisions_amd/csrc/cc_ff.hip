@@ -1101,15 +1101,15 @@ FFJudgeSlot ff_judge_slot(FFWork* wk, int which, const float4* fine, const float
 struct FFBatchWork {
   int cap = 0;
   Acts a;
-  FFSlot* slots[FF_BATCH_TABLES] = {};
-  FFJudgeSlot* judge[FF_BATCH_TABLES] = {};
+  FFSlot* slots[FF_BATCH_TABLES + 1] = {};
+  FFJudgeSlot* judge[FF_BATCH_TABLES + 1] = {};
   std::vector<void*> allocs;
 };
 FFBatchWork* ffbatch_create(int cap, std::string& err) {
   FFBatchWork* w = new FFBatchWork();
   w->cap = cap;
   bool ok = acts_alloc(w->a, cap, false, err);
-  for (int t = 0; ok && t < FF_BATCH_TABLES; t++)
+  for (int t = 0; ok && t <= FF_BATCH_TABLES; t++)
     ok = dev_alloc(w->allocs, (void**)&w->slots[t], sizeof(FFSlot) * cap, err) &&
          dev_alloc(w->allocs, (void**)&w->judge[t], sizeof(FFJudgeSlot) * cap, err);
   if (!ok) { ffbatch_destroy(w); return nullptr; }
@@ -1124,7 +1124,7 @@ void ffbatch_destroy(FFBatchWork* w) {
 int ffbatch_cap(const FFBatchWork* w) { return w->cap; }
 int ffbatch_set(FFBatchWork* w, int t, const std::vector<FFSlot>& slots, const std::vector<FFJudgeSlot>& judge,
                 std::string& err) {
-  if (t < 0 || t >= FF_BATCH_TABLES || (int)slots.size() > w->cap || (int)judge.size() > w->cap) {
+  if (t < 0 || t > FF_EAGER_TABLE || (int)slots.size() > w->cap || (int)judge.size() > w->cap) {
     err = "batched far field: slot table out of range";
     return -1;
   }

@@ -259,7 +259,9 @@ FFJudgeSlot ff_judge_slot(FFWork* wk, int which, const float4* fine, const float
                           const float4* farT1, const float4* force, const Box* box, float gm, double* sums, float tol,
                           float* stats, int* gate);
 // a batch's workspace for batched inference: activations for up to cap slots, a slot table per captured cycle
+// (FF_BATCH_TABLES of them) and one more (FF_EAGER_TABLE) for a mesh step outside a whole cycle
 constexpr int FF_BATCH_TABLES = 4;
+constexpr int FF_EAGER_TABLE = FF_BATCH_TABLES;
 struct FFBatchWork;
 FFBatchWork* ffbatch_create(int cap, std::string& err);
 void ffbatch_destroy(FFBatchWork* w);
