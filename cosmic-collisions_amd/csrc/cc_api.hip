@@ -81,6 +81,12 @@ struct cc_ctx {
   float4* ffFar = nullptr;     // per box 3 x 4096 (FF_FAR): the network's farA, farT0, farT1 (judge steps)
   bool ffBatched = false;      // while a batch captures a cycle whose inference and judge it runs itself
   int headN = 0;               // the step whose head (one_step part 1) ran last
+  // while a batch that fuses its instances' passes captures a cycle: the step's step pass and seat pass
+  // recorded here instead of launched (seatWaits: the recorded seat pass seats a count launched on this
+  // ctx's stream in the same step)
+  std::vector<SimJob>* simJobs = nullptr;
+  std::vector<SeatJob>* seatJobs = nullptr;
+  bool seatWaits = false;
   // a judge runs on the second stream beside the steps after it: forked after the exact pass, joined by the end
   // of its cycle (or of the call that forked it), ffPending while it is not joined
   hipEvent_t ffFork = nullptr, ffJoin = nullptr;
@@ -617,9 +623,15 @@ static int sim_pass(cc_ctx* c, const cc_sim_args* a, bool count = false) {
   SimU u = sim_uniforms(c, a);
   int s = c->cur, d = c->cur ^ 1;
   SlotsOut o = slots_out(c->gb, c->N, c->gb.cur ^ 1);
-  launch_sim(u, c->boxDev, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot,
-             c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], count ? &o : nullptr, c->s);
-  LAUNCHCHK(c);
+  if (c->simJobs) {
+    c->simJobs->push_back(sim_job(u, c->boxDev, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0],
+                                  c->mesh[1], c->ppSlot, c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d],
+                                  count ? &o : nullptr));
+  } else {
+    launch_sim(u, c->boxDev, c->pos[s], c->vel[s], c->mat, c->q[s], c->gb.seats, c->mesh[0], c->mesh[1], c->ppSlot,
+               c->ppForce, c->pos[d], c->vel[d], c->aux[d], c->q[d], count ? &o : nullptr, c->s);
+    LAUNCHCHK(c);
+  }
   c->cur = d;
   c->counted = count ? 1 : 0;
   return 0;
@@ -664,9 +676,18 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a, int p
     }
     if (n % 2 == 0) {   // a rebuild: seated from the count the last step pass took, or counted and seated here
       if (c->counted) {
-        launch_slots_seat(c->N, c->gb, c->s);
-        LAUNCHCHK(c);
+        if (c->seatJobs) {
+          c->seatJobs->push_back(seat_job(c->N, c->gb));
+        } else {
+          launch_slots_seat(c->N, c->gb, c->s);
+          LAUNCHCHK(c);
+        }
         c->counted = 0;
+      } else if (c->seatJobs) {
+        launch_slots_count(c->pos[c->cur], c->N, (float)(1.0 / c->p.cell), c->grid, c->gb, c->s);
+        LAUNCHCHK(c);
+        c->seatJobs->push_back(seat_job(c->N, c->gb));
+        c->seatWaits = true;
       } else if ((r = cc_slots(c))) {
         return r;
       }
@@ -1168,6 +1189,9 @@ struct BatchGraph {
   std::vector<double> key;
   hipGraphExec_t exec = nullptr;
   std::vector<int> curAfter, gridCurAfter, countedAfter;
+  // a fused cycle's jobs: per step of the cycle, every instance's step pass and seat pass (instance order)
+  SimJob* simTab = nullptr;
+  SeatJob* seatTab = nullptr;
 };
 struct cc_batch {
   std::vector<cc_ctx*> ctx;
@@ -1181,6 +1205,12 @@ struct cc_batch {
   // with one learned far field attached to every instance: one inference over all instances' boxes per
   // mesh step and one judge of the judged ones (the graph cache's slot tables, one per entry)
   FFBatchWork* ffw = nullptr;
+  // the fused passes' work-group maps (work-group -> instance) and each instance's first work-group, for the
+  // step pass (SIM_JOB_BLOCK particles a work-group) and the seat pass (SEAT_JOB_BLOCK)
+  uint16_t* simMap = nullptr;
+  uint16_t* seatMap = nullptr;
+  int simBlocks = 0, seatBlocks = 0;
+  std::vector<int> simBlock0, seatBlock0;
   int nextGraph = 0;
   double lastMs = 0.0;
   std::string err;
@@ -1243,6 +1273,121 @@ static int batch_ff_table(cc_batch* b, int t, int* nSlots, int* nJudged) {
   return 0;
 }
 
+// The instances' step and seat passes fuse into one launch each per step when every instance takes the next
+// rebuild's count the same way (the launch is one kernel, with the count or without) and the instances fit the
+// maps' 16-bit entries.
+static bool batch_fusable(const cc_batch* b) {
+  if (b->ctx.size() > 65535) return false;
+  for (const cc_ctx* c : b->ctx)
+    if (c->fuseCount != b->ctx[0]->fuseCount) return false;
+  return true;
+}
+
+// The work-group maps of the fused passes, made once (the instances' sizes are fixed for the batch's life).
+static int batch_maps(cc_batch* b) {
+  if (b->simMap) return 0;
+  const size_t n = b->ctx.size();
+  std::vector<uint16_t> sm, tm;
+  b->simBlock0.assign(n, 0);
+  b->seatBlock0.assign(n, 0);
+  for (size_t i = 0; i < n; i++) {
+    const int N = b->ctx[i]->N;
+    b->simBlock0[i] = (int)sm.size();
+    b->seatBlock0[i] = (int)tm.size();
+    sm.insert(sm.end(), (size_t)((N + SIM_JOB_BLOCK - 1) / SIM_JOB_BLOCK), (uint16_t)i);
+    tm.insert(tm.end(), (size_t)((N + SEAT_JOB_BLOCK - 1) / SEAT_JOB_BLOCK), (uint16_t)i);
+  }
+  if (sm.size() > (size_t)(1 << 30)) BFAIL(b, CC_EINVAL, "batch too large to fuse (%zu work-groups)", sm.size());
+  BCHK(b, hipMalloc(&b->simMap, sm.size() * sizeof(uint16_t)));
+  BCHK(b, hipMalloc(&b->seatMap, tm.size() * sizeof(uint16_t)));
+  BCHK(b, hipMemcpy(b->simMap, sm.data(), sm.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->seatMap, tm.data(), tm.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  b->simBlocks = (int)sm.size();
+  b->seatBlocks = (int)tm.size();
+  return 0;
+}
+
+// A fused cycle, issued into the batch stream's capture after its first fork.  Per step: each instance's own
+// work of the step (a mesh step's passes with the pairwise pass beside them, the cycle's box, a count the last
+// step pass did not take, the judge's join at the cycle's end) on its own stream, forked from the batch stream
+// after the step before; the step's seat pass of every instance in one launch (beside that work, which does not
+// read the seats, unless it counted for them); then, after every instance's work, the step pass of every
+// instance in one launch.  With one learned far field on every instance the mesh step is split as in the
+// unfused batched cycle (heads, one inference over every box, tails) and the judged boxes are judged after the
+// cycle.  The jobs are collected into simAll / seatAll (step-major, instance order) for the graph's tables.
+static int batch_capture_fused(cc_batch* b, const cc_sim_args* args, BatchGraph& g, FFNet* net, int t, int nSlots,
+                               int nJudged, std::vector<SimJob>& simAll, std::vector<SeatJob>& seatAll,
+                               hipError_t& e) {
+  const size_t n = b->ctx.size();
+  const int M = b->ctx[0]->p.meshEvery;
+  simAll.assign((size_t)M * n, SimJob{});
+  seatAll.assign((size_t)M * n, SeatJob{});
+  std::vector<SimJob> sim;
+  std::vector<SeatJob> seat;
+  int r = 0;
+  for (size_t i = 0; i < n; i++) b->ctx[i]->ffBatched = net != nullptr;
+  for (int k = 0; k < M && !r && e == hipSuccess; k++) {
+    sim.clear();
+    seat.clear();
+    bool waits = false;
+    const bool heads = net && k == 0;
+    if (k > 0) e = hipEventRecord(b->fork, b->s);
+    for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
+      cc_ctx* c = b->ctx[i];
+      e = hipStreamWaitEvent(c->s, b->fork, 0);
+      if (e != hipSuccess) break;
+      if (k == 0) launch_box_next(c->boxSched, c->boxIdx, c->boxDev, c->s);
+      c->simJobs = &sim; c->seatJobs = &seat; c->seatWaits = false;
+      r = one_step(c, args[i].settle, args[i].kick, &args[i], heads ? 1 : 0);
+      waits = waits || c->seatWaits;
+      c->simJobs = nullptr; c->seatJobs = nullptr;
+      if (r) { b->err = c->err; break; }
+      e = hipEventRecord(heads ? b->mid[i] : b->join[i], c->s);
+    }
+    if (r || e != hipSuccess) break;
+    if (!seat.empty() && seat.size() != n) { b->err = "fused batch: the instances rebuild on different steps"; r = CC_ESTATE; break; }
+    SimJob* simTab = g.simTab + (size_t)k * n;
+    SeatJob* seatTab = g.seatTab + (size_t)k * n;
+    if (!seat.empty() && !waits) launch_slots_seat_jobs(seatTab, b->seatMap, b->seatBlocks, b->s);
+    if (heads) {
+      for (size_t i = 0; i < n && e == hipSuccess; i++) e = hipStreamWaitEvent(b->s, b->mid[i], 0);
+      if (e != hipSuccess) break;
+      ff_predict_batch(net, b->ffw, t, nSlots, b->s);
+      e = hipEventRecord(b->fork2, b->s);
+      for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
+        cc_ctx* c = b->ctx[i];
+        e = hipStreamWaitEvent(c->s, b->fork2, 0);
+        if (e != hipSuccess) break;
+        c->simJobs = &sim;
+        r = one_step(c, args[i].settle, args[i].kick, &args[i], 2);
+        c->simJobs = nullptr;
+        if (r) { b->err = c->err; break; }
+        e = hipEventRecord(b->join[i], c->s);
+      }
+      if (r || e != hipSuccess) break;
+    }
+    for (size_t i = 0; i < n && e == hipSuccess; i++) e = hipStreamWaitEvent(b->s, b->join[i], 0);
+    if (e != hipSuccess) break;
+    if (!seat.empty() && waits) launch_slots_seat_jobs(seatTab, b->seatMap, b->seatBlocks, b->s);
+    if (sim.size() != n) { b->err = "fused batch: an instance took no step pass"; r = CC_ESTATE; break; }
+    const bool count = sim[0].cnt.cnt != nullptr;
+    for (size_t i = 0; i < n; i++) {
+      if ((sim[i].cnt.cnt != nullptr) != count) { b->err = "fused batch: the instances count on different steps"; r = CC_ESTATE; break; }
+      sim[i].block0 = b->simBlock0[i];
+      simAll[(size_t)k * n + i] = sim[i];
+      if (!seat.empty()) {
+        seat[i].block0 = b->seatBlock0[i];
+        seatAll[(size_t)k * n + i] = seat[i];
+      }
+    }
+    if (r) break;
+    launch_sim_jobs(simTab, b->simMap, b->simBlocks, count, b->s);
+  }
+  if (!r && e == hipSuccess && net) ff_judge_batch(b->ffw, t, nJudged, b->s);
+  for (size_t i = 0; i < n; i++) b->ctx[i]->ffBatched = false;
+  return r;
+}
+
 static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   std::vector<double> key;
   for (size_t i = 0; i < b->ctx.size(); i++) {
@@ -1264,6 +1409,15 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   const bool batched = net != nullptr;
   int nSlots = 0, nJudged = 0, r;
   if (batched && (r = batch_ff_table(b, t, &nSlots, &nJudged))) return r;
+  const bool fused = batch_fusable(b);
+  if (fused) {
+    if ((r = batch_maps(b))) return r;
+    const size_t jobs = (size_t)b->ctx[0]->p.meshEvery * n;
+    if (!g.simTab) BCHK(b, hipMalloc(&g.simTab, jobs * sizeof(SimJob)));
+    if (!g.seatTab) BCHK(b, hipMalloc(&g.seatTab, jobs * sizeof(SeatJob)));
+  }
+  std::vector<SimJob> simAll;
+  std::vector<SeatJob> seatAll;
   std::vector<int> stepNo(n), cur(n), gcur(n), counted(n);
   for (size_t i = 0; i < n; i++) {
     stepNo[i] = b->ctx[i]->stepNo; cur[i] = b->ctx[i]->cur; gcur[i] = b->ctx[i]->gb.cur; counted[i] = b->ctx[i]->counted;
@@ -1271,7 +1425,9 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
   BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipEventRecord(b->fork, b->s);
   r = 0;
-  if (!batched) {
+  if (fused) {
+    if (e == hipSuccess) r = batch_capture_fused(b, args, g, net, t, nSlots, nJudged, simAll, seatAll, e);
+  } else if (!batched) {
     for (size_t i = 0; i < n && !r && e == hipSuccess; i++) {
       cc_ctx* c = b->ctx[i];
       e = hipStreamWaitEvent(c->s, b->fork, 0);
@@ -1329,6 +1485,22 @@ static int batch_graph(cc_batch* b, const cc_sim_args* args, BatchGraph** out) {
     if (graph) hipGraphDestroy(graph);
     if (r) return r;
     BFAIL(b, CC_EDEVICE, "batch capture: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+  }
+  if (fused) {   // the jobs the captured launches read (no replay of this entry's last graph is running)
+    e = hipMemcpy(g.simTab, simAll.data(), simAll.size() * sizeof(SimJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g.seatTab, seatAll.data(), seatAll.size() * sizeof(SeatJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipGraphDestroy(graph);
+      BFAIL(b, CC_EDEVICE, "fused batch jobs: %s", hipGetErrorString(e));
+    }
+  }
+  if (fused) {   // the jobs the captured launches read (no replay of this entry's last graph is running)
+    e = hipMemcpy(g.simTab, simAll.data(), simAll.size() * sizeof(SimJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g.seatTab, seatAll.data(), seatAll.size() * sizeof(SeatJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipGraphDestroy(graph);
+      BFAIL(b, CC_EDEVICE, "fused batch jobs: %s", hipGetErrorString(e));
+    }
   }
   e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
@@ -1419,7 +1591,13 @@ int cc_batch_create(cc_ctx* const* ctxs, int n, cc_batch** out) {
 int cc_batch_destroy(cc_batch* b) {
   if (!b) return 0;
   if (b->s) hipStreamSynchronize(b->s);
-  for (BatchGraph& g : b->graphs) if (g.exec) hipGraphExecDestroy(g.exec);
+  for (BatchGraph& g : b->graphs) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.simTab) hipFree(g.simTab);
+    if (g.seatTab) hipFree(g.seatTab);
+  }
+  if (b->simMap) hipFree(b->simMap);
+  if (b->seatMap) hipFree(b->seatMap);
   for (hipEvent_t e : b->join) if (e) hipEventDestroy(e);
   for (hipEvent_t e : b->mid) if (e) hipEventDestroy(e);
   if (b->fork2) hipEventDestroy(b->fork2);

@@ -123,6 +123,39 @@ struct SlotsOut {
   int* ovfN;
 };
 
+// One instance's step pass or seat pass as a job of a launch that serves several instances (a batch's
+// fused passes): the arguments its own launch would take, and block0, its first work-group in the launch.
+struct SimJob {
+  SimU u;
+  const Box* bx;
+  const float4 *pos, *vel;
+  const float *mat, *q;
+  const uint32_t* seats;
+  const float4 *force0, *mono0, *force1, *mono1;
+  const float* ppSlot;
+  const float4* ppForce;
+  float4 *oPos, *oVel, *oAux;
+  float* oQ;
+  SlotsOut cnt;
+  float touch2r;
+  int block0;
+};
+struct SeatJob {
+  const int *occ, *segN, *occPrev, *segPrev;
+  int N;
+  const int* cnt;
+  int* cntPrev;
+  const uint32_t* bucket;
+  const int2* ovf;
+  const int* ovfN;
+  int* ovfNOther;
+  uint32_t idxMask;
+  uint32_t* seats;
+  int block0;
+};
+constexpr int SIM_JOB_BLOCK = 64;    // a step-pass job's work-group: particles
+constexpr int SEAT_JOB_BLOCK = 256;  // a seat-pass job's work-group: threads
+
 // ---- launches (cc_kernels.hip) ----
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
@@ -130,6 +163,10 @@ void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_slots(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
 void launch_slots_count(const float4* pos, int N, float invCell, const Grid& g, GridBuf& gb, hipStream_t s);
 void launch_slots_seat(int N, GridBuf& gb, hipStream_t s);
+// the seat pass as a job (and the parity flipped as launch_slots_seat flips it); jobs[blockJob[b]] for
+// work-groups b < blocks of SEAT_JOB_BLOCK threads
+SeatJob seat_job(int N, GridBuf& gb);
+void launch_slots_seat_jobs(const SeatJob* jobs, const uint16_t* blockJob, int blocks, hipStream_t s);
 // a count of parity cur ^ 1 that will not be seated: its counters back to zero
 void launch_slots_discard(int N, GridBuf& gb, hipStream_t s);
 // the arrays a count of that parity writes
@@ -168,6 +205,12 @@ void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* v
                 const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
                 const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut* count,
                 hipStream_t s);
+// the step pass as a job (the same arguments); jobs[blockJob[L]] for logical work-groups L < blocks of
+// SIM_JOB_BLOCK particles, every job of one launch with the count or every one without
+SimJob sim_job(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
+               const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
+               const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut* count);
+void launch_sim_jobs(const SimJob* jobs, const uint16_t* blockJob, int blocks, bool count, hipStream_t s);
 void launch_rigid(int N, const float4* pos, const float4* vel, const float4* aux, const float* q,
                   const float* bulk0, const float* bulk1, const float* cen0, const float* cen1,
                   const float* spin0, const float* spin1, const float* rot0, const float* rot1, float dt,

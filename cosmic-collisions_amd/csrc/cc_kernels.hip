@@ -164,13 +164,13 @@ __global__ void __launch_bounds__(256) k_slots_count(const float4* __restrict__ 
 // time (a cell with arrivals has all eight seats written), and its counter of
 // that rebuild back to zero for the rebuild after this one.  Neither job
 // writes what the other reads.
-__global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN,
-                                                    const int* __restrict__ occPrev, const int* __restrict__ segPrev,
-                                                    int N, const int* __restrict__ cnt, int* __restrict__ cntPrev,
-                                                    const uint32_t* __restrict__ bucket, const int2* __restrict__ ovf,
-                                                    const int* __restrict__ ovfN, int* __restrict__ ovfNOther,
-                                                    uint32_t idxMask, uint32_t* __restrict__ seats) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+// (t: the thread's place among the instance's N)
+__device__ __forceinline__ void seat_body(const int* __restrict__ occ, const int* __restrict__ segN,
+                                          const int* __restrict__ occPrev, const int* __restrict__ segPrev, int N,
+                                          const int* __restrict__ cnt, int* __restrict__ cntPrev,
+                                          const uint32_t* __restrict__ bucket, const int2* __restrict__ ovf,
+                                          const int* __restrict__ ovfN, int* __restrict__ ovfNOther, uint32_t idxMask,
+                                          uint32_t* __restrict__ seats, int t) {
   if (t == 0) *ovfNOther = 0;   // the overflow counter of the next rebuild
   if (t >= N) return;
   // The loads that wait on nothing but each other, issued before any store: the segment lengths, the
@@ -226,6 +226,27 @@ __global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ,
   sv[1] = make_uint4(out[1], out[3], out[5], out[7]);
 }
 
+__global__ void __launch_bounds__(256) k_slots_seat(const int* __restrict__ occ, const int* __restrict__ segN,
+                                                    const int* __restrict__ occPrev, const int* __restrict__ segPrev,
+                                                    int N, const int* __restrict__ cnt, int* __restrict__ cntPrev,
+                                                    const uint32_t* __restrict__ bucket, const int2* __restrict__ ovf,
+                                                    const int* __restrict__ ovfN, int* __restrict__ ovfNOther,
+                                                    uint32_t idxMask, uint32_t* __restrict__ seats) {
+  seat_body(occ, segN, occPrev, segPrev, N, cnt, cntPrev, bucket, ovf, ovfN, ovfNOther, idxMask, seats,
+            blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The seat passes of several instances in one launch: work-group b serves job blockJob[b], its local block
+// b - block0 (a batch whose instances step together launches one per step instead of one per instance).
+__global__ void __launch_bounds__(SEAT_JOB_BLOCK) k_slots_seat_jobs(const SeatJob* __restrict__ jobs,
+                                                         const uint16_t* __restrict__ blockJob, int blocks) {
+  const int b = (int)blockIdx.x;
+  if (b >= blocks) return;
+  const SeatJob& J = jobs[blockJob[b]];
+  seat_body(J.occ, J.segN, J.occPrev, J.segPrev, J.N, J.cnt, J.cntPrev, J.bucket, J.ovf, J.ovfN, J.ovfNOther,
+            J.idxMask, J.seats, (b - J.block0) * SEAT_JOB_BLOCK + (int)threadIdx.x);
+}
+
 // A count that will not be seated (the particles moved since): its counters
 // and overflow counter back to zero.
 __global__ void __launch_bounds__(256) k_slots_discard(const int* __restrict__ occ, const int* __restrict__ segN, int N,
@@ -258,6 +279,22 @@ void launch_slots_seat(int N, GridBuf& gb, hipStream_t s) {
   hipLaunchKernelGGL(k_slots_seat, dim3(blocks), dim3(256), 0, s, n.occ, n.segN, p.occ, p.segN, N, n.cnt, p.cnt,
                      gb.bucket, gb.ovf, n.ovfN, p.ovfN, gb.idxMask, gb.seats);
   gb.cur = next;
+}
+
+SeatJob seat_job(int N, GridBuf& gb) {
+  int prev = gb.cur, next = gb.cur ^ 1;
+  SlotsOut n = slots_out(gb, N, next), p = slots_out(gb, N, prev);
+  SeatJob j;
+  j.occ = n.occ; j.segN = n.segN; j.occPrev = p.occ; j.segPrev = p.segN; j.N = N; j.cnt = n.cnt; j.cntPrev = p.cnt;
+  j.bucket = gb.bucket; j.ovf = gb.ovf; j.ovfN = n.ovfN; j.ovfNOther = p.ovfN; j.idxMask = gb.idxMask;
+  j.seats = gb.seats; j.block0 = 0;
+  gb.cur = next;
+  return j;
+}
+
+void launch_slots_seat_jobs(const SeatJob* jobs, const uint16_t* blockJob, int blocks, hipStream_t s) {
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_slots_seat_jobs, dim3(blocks), dim3(SEAT_JOB_BLOCK), 0, s, jobs, blockJob, blocks);
 }
 
 void launch_slots_discard(int N, GridBuf& gb, hipStream_t s) {
@@ -933,6 +970,7 @@ __device__ __forceinline__ void cross_at(float4 mo, float4 mm, float gm, float e
 //   C  the touching pairs' velocities and masses, and the sums.
 // A lane whose list would overflow finishes alone on the direct path.
 constexpr int SIM_BLOCK = 64;
+static_assert(SIM_BLOCK == SIM_JOB_BLOCK, "a step-pass job's work-group is the pass's own");
 constexpr int CAND_CAP = 31;                 // a list longer than this takes the direct path
 constexpr int CAND_ROWS = CAND_CAP + 9;      // A's writes run up to a cell's eight seats past the row after the
                                              // cap; B reads eight rows at a time from below the cap
@@ -1016,17 +1054,20 @@ __device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int b
   return d2;
 }
 
-// (at most 128 registers: four wavefronts a SIMD, as the 10 KiB of LDS allow)
+// The pass for the particle at place `at` of the visit order, by lane threadIdx.x of a one-wavefront
+// work-group (cand: the work-group's candidate lists, entry-major: a lane's list is a column, no bank
+// conflicts).
 template <bool COUNT>
-__global__ void __launch_bounds__(SIM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restrict__ pos,
-      const float4* __restrict__ vel, const float* __restrict__ mat, const float* __restrict__ q,
-      const uint32_t* __restrict__ seats, const float4* __restrict__ force0, const float4* __restrict__ mono0,
-      const float4* __restrict__ force1, const float4* __restrict__ mono1, const float* __restrict__ ppSlot,
-      const float4* __restrict__ ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, SlotsOut cnt) {
-  __shared__ int cand[CAND_ROWS][SIM_BLOCK];   // entry-major: a lane's list is a column, no bank conflicts
+__device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, const Box* __restrict__ bxp,
+                                         float touch2r, const float4* __restrict__ pos,
+                                         const float4* __restrict__ vel, const float* __restrict__ mat,
+                                         const float* __restrict__ q, const uint32_t* __restrict__ seats,
+                                         const float4* __restrict__ force0, const float4* __restrict__ mono0,
+                                         const float4* __restrict__ force1, const float4* __restrict__ mono1,
+                                         const float* __restrict__ ppSlot, const float4* __restrict__ ppForce,
+                                         float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut& cnt,
+                                         int at) {
   int lane = threadIdx.x;
-  const int at = (u.order ? (int)blockIdx.x : xcd_block()) * blockDim.x + lane;   // the place in the visit order
   if (at >= u.N) return;
   const int me = u.visit ? u.visit[at] : at;
   float4 P = pos[me];
@@ -1249,6 +1290,35 @@ k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restri
   }
 }
 
+// (at most 128 registers: four wavefronts a SIMD, as the 10 KiB of LDS allow)
+template <bool COUNT>
+__global__ void __launch_bounds__(SIM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restrict__ pos,
+      const float4* __restrict__ vel, const float* __restrict__ mat, const float* __restrict__ q,
+      const uint32_t* __restrict__ seats, const float4* __restrict__ force0, const float4* __restrict__ mono0,
+      const float4* __restrict__ force1, const float4* __restrict__ mono1, const float* __restrict__ ppSlot,
+      const float4* __restrict__ ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, SlotsOut cnt) {
+  __shared__ int cand[CAND_ROWS][SIM_BLOCK];
+  const int at = (u.order ? (int)blockIdx.x : xcd_block()) * SIM_BLOCK + (int)threadIdx.x;
+  sim_body<COUNT>(cand, u, bxp, touch2r, pos, vel, mat, q, seats, force0, mono0, force1, mono1, ppSlot, ppForce,
+                  oPos, oVel, oAux, oQ, cnt, at);
+}
+
+// The step passes of several instances in one launch: logical work-group L (spread over the XCDs as
+// xcd_block does, so each XCD's L2 holds a contiguous run of some instance's Morton blocks) serves job
+// blockJob[L], its local block L - block0.  A batch whose instances step together launches one per step.
+template <bool COUNT>
+__global__ void __launch_bounds__(SIM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_sim_jobs(const SimJob* __restrict__ jobs, const uint16_t* __restrict__ blockJob, int blocks) {
+  __shared__ int cand[CAND_ROWS][SIM_BLOCK];
+  const int L = xcd_block();
+  if (L >= blocks) return;
+  const SimJob& J = jobs[blockJob[L]];
+  sim_body<COUNT>(cand, J.u, J.bx, J.touch2r, J.pos, J.vel, J.mat, J.q, J.seats, J.force0, J.mono0, J.force1,
+                  J.mono1, J.ppSlot, J.ppForce, J.oPos, J.oVel, J.oAux, J.oQ, J.cnt, (L - J.block0) * SIM_BLOCK +
+                  (int)threadIdx.x);
+}
+
 // the smallest r^2 whose correctly rounded square root is not below the
 // contact distance: r^2 < touch2r is then the same test as touch - sqrt(r^2) > 0
 float touch_r2(float touch) {
@@ -1256,6 +1326,29 @@ float touch_r2(float touch) {
   while (x > 0.0f && std::sqrt(std::nextafter(x, 0.0f)) >= touch) x = std::nextafter(x, 0.0f);
   while (std::sqrt(x) < touch) x = std::nextafter(x, INFINITY);
   return x;
+}
+
+SimJob sim_job(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
+               const uint32_t* seats, const MeshBuf& m0, const MeshBuf& m1, const float* ppSlot,
+               const float4* ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut* count) {
+  SimJob j;
+  j.u = u; j.bx = bx; j.touch2r = touch_r2(u.touch);
+  j.pos = pos; j.vel = vel; j.mat = mat; j.q = q; j.seats = seats;
+  j.force0 = m0.force; j.mono0 = m0.mono; j.force1 = m1.force; j.mono1 = m1.mono;
+  j.ppSlot = ppSlot; j.ppForce = ppForce;
+  j.oPos = oPos; j.oVel = oVel; j.oAux = oAux; j.oQ = oQ;
+  j.cnt = count ? *count : SlotsOut{};
+  j.block0 = 0;
+  return j;
+}
+
+void launch_sim_jobs(const SimJob* jobs, const uint16_t* blockJob, int blocks, bool count, hipStream_t s) {
+  if (blocks <= 0) return;
+  int grid = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
+  if (count)
+    hipLaunchKernelGGL(k_sim_jobs<true>, dim3(grid), dim3(SIM_BLOCK), 0, s, jobs, blockJob, blocks);
+  else
+    hipLaunchKernelGGL(k_sim_jobs<false>, dim3(grid), dim3(SIM_BLOCK), 0, s, jobs, blockJob, blocks);
 }
 
 void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* vel, const float* mat, const float* q,
