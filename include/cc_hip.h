@@ -269,11 +269,15 @@ int cc_ff_stats(cc_ctx* ctx, double* out);
 
 /* Batched sweeps (no reference counterpart: the reference runs one instance
  * per page).  Independent instances, each its own ctx (any N = W*H up to
- * 262144), stepped together: every instance's cycle of meshEvery steps is one
- * branch of one captured HIP graph, so the batch advances a cycle per launch.
- * Results are those of each instance stepped alone with cc_step_n.  The ctxs
- * stay owned by the caller and must outlive the batch; all must share
- * meshEvery and stand at the same step of the mesh cycle. */
+ * 262144), stepped together: a cycle of meshEvery steps of every instance is
+ * one captured HIP graph, so the batch advances a cycle per launch.  Inside
+ * it each step's step pass and seat pass are one kernel launch each for all
+ * instances (a work-group map to per-instance jobs); the mesh step's own
+ * passes run per instance, side by side.  Steps outside a whole cycle run
+ * every instance on its own stream.  Results are those of each instance
+ * stepped alone with cc_step_n.  The ctxs stay owned by the caller and must
+ * outlive the batch; all must share meshEvery and stand at the same step of
+ * the mesh cycle. */
 typedef struct cc_batch cc_batch;
 int cc_batch_create(cc_ctx* const* ctxs, int n, cc_batch** out);
 int cc_batch_destroy(cc_batch* b);
