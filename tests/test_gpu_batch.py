@@ -118,6 +118,48 @@ def test_batch_with_learned_far_field_equals_instances_alone():
         ff.close()
 
 
+def make_crater(N):
+    """A crater instance (constant gravity: no mesh, no pairwise pass) of the first N grains of the reference's
+    crater dump after the coupling."""
+    fx = gu.fixture("c3_crater")
+    h = gu.crater_host(fx, "crater_build")[0]
+    s, k = h["sim"], h["consts"]
+    ctx = ccamd.Context(ccamd.params(N=N, W=s["W"], G=s["G"], m=1.0, a=s["a"], cell=s["cell"], dt=s["dt"], k=s["k"],
+                                     c=s["c"], creep=s["creep"], damp=s["damp"], tK=s["tK"], kappa=s["kappa"],
+                                     field=True, gField=(0.0, -k["G_EARTH"] * k["T_UNIT"] ** 2 / k["L_UNIT"], 0.0),
+                                     frozen=k["MAT_FROZEN"], cpIron=1.0))
+    pos, vel, q = gu.crater_state(fx, "post")
+    ctx.upload(pos[:N], vel[:N], fx["mat"][:N], q[:N])
+    return ctx
+
+
+def test_batch_with_constant_gravity_members_equals_instances_alone():
+    """Two crater instances (constant gravity: every step a seat rebuild or not and the step pass, no mesh) in
+    one batch with an impact instance (its mesh and pairwise passes every eighth step): the fused step and seat
+    passes serve both kinds, and every member is bit-identical to itself stepped alone."""
+    ms = members()[:1]
+    alone = [make_crater(16384), make_crater(128 * 40)] + [make(m) for m in ms]
+    together = [make_crater(16384), make_crater(128 * 40)] + [make(m) for m in ms]
+    batch = ccamd.Batch(together)
+    try:
+        args = ccamd.sim_args()
+        for call, n in enumerate((24, 21, 35)):
+            start = alone[0].step_no
+            print("mixed batch call", call, "steps", n, "from", start, flush=True)
+            bl = [np.zeros((0, 7)), np.zeros((0, 7))] + [boxes(m, start, n) for m in ms]
+            for ctx, b in zip(alone, bl):
+                ctx.step_n(n, args, b if len(b) else None)
+            batch.step_n(n, args, bl)
+            for i, (a, t) in enumerate(zip(alone, together)):
+                assert a.step_no == t.step_no == start + n
+                for x, y, what in zip(a.download(), t.download(), ("pos", "vel", "aux", "q", "mat")):
+                    assert np.array_equal(x, y), (call, i, what)
+    finally:
+        batch.close()
+        for c in alone + together:
+            c.close()
+
+
 def test_batch_rejects_misaligned_members():
     ms = members()[:2]
     ctxs = [make(m) for m in ms]
