@@ -1054,10 +1054,13 @@ __device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int b
   return d2;
 }
 
-// The pass for the particle at place `at` of the visit order, by lane threadIdx.x of a one-wavefront
-// work-group (cand: the work-group's candidate lists, entry-major: a lane's list is a column, no bank
-// conflicts).
-template <bool COUNT>
+// The pass for the particle at place `at` of the visit order, by lane threadIdx.x % 64 (cand: the
+// work-group's candidate lists, entry-major: a lane's list is a column, no bank conflicts).  PLANES 1: a
+// one-wavefront work-group, the wavefront walks the whole neighbourhood.  PLANES 3: three wavefronts for the
+// same 64 particles, wavefront w walks the z-plane dz = w - 1 into list w (rows w * CAND_ROWS on); the
+// lists are joined in plane order behind list 0 and wavefront 0 goes on alone -- the same list, so the same
+// sums, with three times the wavefronts in flight where the particles are too few to fill the GPU.
+template <bool COUNT, int PLANES = 1>
 __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, const Box* __restrict__ bxp,
                                          float touch2r, const float4* __restrict__ pos,
                                          const float4* __restrict__ vel, const float* __restrict__ mat,
@@ -1067,7 +1070,7 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
                                          const float* __restrict__ ppSlot, const float4* __restrict__ ppForce,
                                          float4* oPos, float4* oVel, float4* oAux, float* oQ, const SlotsOut& cnt,
                                          int at) {
-  int lane = threadIdx.x;
+  const int lane = (int)threadIdx.x & (SIM_BLOCK - 1), w = PLANES == 1 ? 0 : (int)threadIdx.x / SIM_BLOCK;
   if (at >= u.N) return;
   const int me = u.visit ? u.visit[at] : at;
   float4 P = pos[me];
@@ -1094,9 +1097,9 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
   // the list's next entry as a byte offset into the LDS array (row stride SIM_BLOCK ints), held at the row
   // past the cap after each cell
   const uint32_t rowB = SIM_BLOCK * 4u, wa0 = (uint32_t)lane * 4u, waEnd = (CAND_CAP + 1) * rowB + wa0;
-  char* const candB = reinterpret_cast<char*>(&cand[0][0]);
+  char* const candB = reinterpret_cast<char*>(&cand[w * CAND_ROWS][0]);
   uint32_t wa = wa0;
-  for (int dz = -1; dz <= 1; dz++) {
+  for (int dz = PLANES == 1 ? -1 : w - 1; dz <= (PLANES == 1 ? 1 : w - 1); dz++) {
     uint4 sa[9], sb[9];
     bool ok[9];
 #pragma unroll
@@ -1132,7 +1135,25 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
       wa = min(wa, waEnd);
     }
   }
-  const int n = (int)((wa - wa0) / rowB);
+  int n = (int)((wa - wa0) / rowB);
+  if (PLANES > 1) {
+    // each list's length in its last row (a slack row, past every write of the walk), then the lists behind
+    // list 0 in plane order, unless they overflow together
+    cand[w * CAND_ROWS + CAND_ROWS - 1][lane] = n;
+    __syncthreads();
+    int total = 0, off = 0;
+#pragma unroll
+    for (int v = 0; v < PLANES; v++) {
+      const int nv = cand[v * CAND_ROWS + CAND_ROWS - 1][lane];
+      off += v < w ? nv : 0;
+      total += nv;
+    }
+    if (w > 0 && total <= CAND_CAP)
+      for (int k = 0; k < n; k++) cand[off + k][lane] = cand[w * CAND_ROWS + k][lane];
+    __syncthreads();
+    if (w > 0) return;
+    n = total;
+  }
   const bool over = n > CAND_CAP;
   // the particle's own velocity, temperature and mass, needed from the contacts on
   float4 V = vel[me];
@@ -1290,18 +1311,19 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
   }
 }
 
-// (at most 128 registers: four wavefronts a SIMD, as the 10 KiB of LDS allow)
-template <bool COUNT>
-__global__ void __launch_bounds__(SIM_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+// (at most 128 registers: four wavefronts a SIMD, as the 10 KiB of LDS a wavefront allow).  u.order 1: the
+// work-groups in launch order, else spread over the XCDs; PLANES: wavefronts per 64 particles (sim_body).
+template <bool COUNT, int PLANES>
+__global__ void __launch_bounds__(SIM_BLOCK * PLANES) __attribute__((amdgpu_waves_per_eu(4)))
 k_sim(SimU u, const Box* __restrict__ bxp, float touch2r, const float4* __restrict__ pos,
       const float4* __restrict__ vel, const float* __restrict__ mat, const float* __restrict__ q,
       const uint32_t* __restrict__ seats, const float4* __restrict__ force0, const float4* __restrict__ mono0,
       const float4* __restrict__ force1, const float4* __restrict__ mono1, const float* __restrict__ ppSlot,
       const float4* __restrict__ ppForce, float4* oPos, float4* oVel, float4* oAux, float* oQ, SlotsOut cnt) {
-  __shared__ int cand[CAND_ROWS][SIM_BLOCK];
-  const int at = (u.order ? (int)blockIdx.x : xcd_block()) * SIM_BLOCK + (int)threadIdx.x;
-  sim_body<COUNT>(cand, u, bxp, touch2r, pos, vel, mat, q, seats, force0, mono0, force1, mono1, ppSlot, ppForce,
-                  oPos, oVel, oAux, oQ, cnt, at);
+  __shared__ int cand[PLANES * CAND_ROWS][SIM_BLOCK];
+  const int at = (u.order == 1 ? (int)blockIdx.x : xcd_block()) * SIM_BLOCK + ((int)threadIdx.x & (SIM_BLOCK - 1));
+  sim_body<COUNT, PLANES>(cand, u, bxp, touch2r, pos, vel, mat, q, seats, force0, mono0, force1, mono1, ppSlot,
+                          ppForce, oPos, oVel, oAux, oQ, cnt, at);
 }
 
 // The step passes of several instances in one launch: logical work-group L (spread over the XCDs as
@@ -1357,12 +1379,22 @@ void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* v
                 hipStream_t s) {
   int blocks = (u.N + SIM_BLOCK - 1) / SIM_BLOCK;
   blocks = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
-  if (count)
-    hipLaunchKernelGGL(k_sim<true>, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, touch_r2(u.touch), pos, vel, mat, q,
-                       seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, *count);
-  else
-    hipLaunchKernelGGL(k_sim<false>, dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, touch_r2(u.touch), pos, vel, mat, q,
-                       seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, SlotsOut{});
+  const SlotsOut o = count ? *count : SlotsOut{};
+  const float t2 = touch_r2(u.touch);
+  if (u.order == 2) {   // three wavefronts per 64 particles, one per z-plane of the neighbourhood
+    if (count)
+      hipLaunchKernelGGL((k_sim<true, 3>), dim3(blocks), dim3(3 * SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q,
+                         seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);
+    else
+      hipLaunchKernelGGL((k_sim<false, 3>), dim3(blocks), dim3(3 * SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q,
+                         seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);
+  } else if (count) {
+    hipLaunchKernelGGL((k_sim<true, 1>), dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q, seats,
+                       m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);
+  } else {
+    hipLaunchKernelGGL((k_sim<false, 1>), dim3(blocks), dim3(SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q, seats,
+                       m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);
+  }
 }
 
 // ----------------------------------------------------------- rigid (A14) --
