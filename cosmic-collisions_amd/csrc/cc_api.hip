@@ -21,6 +21,9 @@ struct CycleGraph {
 };
 
 constexpr int SCHED_CAP = 4096;   // step cycles per schedule upload
+// cycles kept captured: a run's looks alternate the seat buffers' parity and whether the first cycle after
+// a look takes its rebuild's count, and a Morton sort moves the state's buffers, so keys recur in a handful
+constexpr int CYCLE_GRAPHS = 8;
 
 struct cc_ctx {
   cc_params p;
@@ -45,6 +48,7 @@ struct cc_ctx {
   float* ppSlot = nullptr;
   float2* ppTab = nullptr;
   int ppCount = 0;
+  int* ppCountDev = nullptr;   // the loose list's length on the device (what a captured cycle reads)
   float4* diag = nullptr;
   int *mkeys = nullptr, *mslot = nullptr, *mhist = nullptr, *mblock = nullptr, *mperm = nullptr, *minto = nullptr;
   double *bpart = nullptr, *bout = nullptr;
@@ -53,7 +57,7 @@ struct cc_ctx {
   Box* boxSched = nullptr; // a step_n call's boxes, one per step cycle, taken in turn by k_box_next
   int* boxIdx = nullptr;
   Box* boxHost = nullptr;  // pinned staging of the schedule
-  CycleGraph graphs[4];    // captured step cycles, by what their launches bake in
+  CycleGraph graphs[CYCLE_GRAPHS];   // captured step cycles, by what their launches bake in
   int nextGraph = 0;
   int useGraphs = 1;
   // the step pass's visit order (a permutation that only the caches see), re-sorted at a cycle's start once
@@ -320,6 +324,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
       ALLOC(m.occ, (size_t)SCRATCH * 4); ALLOC(m.occN, 4); ALLOC(m.coarseDone, 4);
     }
     ALLOC(c->ppIndex, PP_CAP * 4); ALLOC(c->ppList, PP_CAP * 16); ALLOC(c->ppBox, PP_CAP * 4);
+    ALLOC(c->ppCountDev, 4);
     ALLOC(c->ppBounds, PP_TILES_MAX * 32);
     ALLOC(c->ppForce, 4 * PP_CAP * 16); ALLOC(c->ppSlot, N * 4); ALLOC(c->ppTab, 64 * 8);
     ALLOC(c->diag, N * 16);
@@ -356,6 +361,7 @@ int cc_create(const cc_params* p, cc_ctx** out) {
     HIPCHK(c, hipMemsetAsync(c->ppForce, 0, 4 * PP_CAP * 16, c->s));
     HIPCHK(c, hipMemsetAsync(c->ppList, 0, PP_CAP * 16, c->s));
     HIPCHK(c, hipMemsetAsync(c->ppTab, 0, 64 * 8, c->s));
+    HIPCHK(c, hipMemsetAsync(c->ppCountDev, 0, 4, c->s));
     HIPCHK(c, hipMemsetAsync(c->diag, 0, N * 16, c->s));
     LAUNCHCHK(c);
     HIPCHK(c, hipEventCreate(&c->ev0));
@@ -623,7 +629,7 @@ int cc_set_pp_table(cc_ctx* c, const float* tab) {
 int cc_pp(cc_ctx* c) {
   if (!c) return CC_EINVAL;
   if (!c->ppCount) return 0;
-  launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+  launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCountDev, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
             (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppBounds, c->ppForce, c->s);
   LAUNCHCHK(c);
   return 0;
@@ -639,6 +645,7 @@ int cc_set_pp(cc_ctx* c, const int32_t* list, int count) {
   HIPCHK(c, hipMemcpyAsync(c->ppIndex, idx.data(), PP_CAP * 4, hipMemcpyHostToDevice, c->s));
   launch_pp_slot(c->N, c->ppIndex, count, c->ppSlot, c->s);
   LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(c->ppCountDev, &count, 4, hipMemcpyHostToDevice, c->s));
   c->ppCount = count;
   HIPCHK(c, hipStreamSynchronize(c->s));
   return count ? cc_pp(c) : 0;
@@ -707,7 +714,7 @@ static int one_step(cc_ctx* c, int settle, int kick, const cc_sim_args* a, int p
     if (side) {
       HIPCHK(c, hipEventRecord(c->ppFork, c->s));
       HIPCHK(c, hipStreamWaitEvent(c->s2, c->ppFork, 0));
-      launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
+      launch_pp(c->pos[c->cur], c->mat, c->N, c->boxDev, c->ppIndex, c->ppCountDev, c->ppCount, (float)c->p.m, (float)(5.0 * FH),
                 (float)(c->p.a * c->p.a), c->ppTab, c->ppList, c->ppBox, c->ppBounds, c->ppForce, c->s2);
       LAUNCHCHK(c);
       HIPCHK(c, hipEventRecord(c->ppJoin, c->s2));
@@ -787,7 +794,7 @@ static std::vector<double> cycle_key(const cc_ctx* c, const cc_sim_args* a) {
     k.push_back(p.spin0[i]); k.push_back(p.spin1[i]); k.push_back(p.gField[i]);
     k.push_back(a->bulk0[i]); k.push_back(a->bulk1[i]); k.push_back(a->cen0[i]); k.push_back(a->cen1[i]);
   }
-  k.push_back((double)c->ppCount);
+  k.push_back((double)(c->ppCount > 0));   // the length itself is read on the device
   k.push_back((double)(c->visitEvery > 0));
   k.push_back((double)(c->box.boxB > 0.5f));
   k.push_back((double)c->cur);
@@ -816,7 +823,7 @@ static int cycle_graph(cc_ctx* c, const cc_sim_args* a, CycleGraph** out) {
   for (CycleGraph& g : c->graphs)
     if (g.exec && g.key == key) { *out = &g; return 0; }
   CycleGraph& g = c->graphs[c->nextGraph];
-  c->nextGraph = (c->nextGraph + 1) % 4;
+  c->nextGraph = (c->nextGraph + 1) % CYCLE_GRAPHS;
   if (g.exec) {   // its last launch may still run: let it end first
     HIPCHK(c, hipStreamSynchronize(c->s));
     hipGraphExecDestroy(g.exec);
@@ -1801,7 +1808,7 @@ extern "C" int cc_pp_noalloc(const float* pos, const float* mat, int N, const do
   b.boxB = boxB ? 1.0f : 0.0f;
   Box* bx = reinterpret_cast<Box*>(w + NA_BOX);
   launch_put_box(b, bx, s);
-  launch_pp(reinterpret_cast<const float4*>(pos), mat, N, bx, index, count, (float)m, (float)(5.0 * FH), (float)(a * a),
+  launch_pp(reinterpret_cast<const float4*>(pos), mat, N, bx, index, nullptr, count, (float)m, (float)(5.0 * FH), (float)(a * a),
             reinterpret_cast<const float2*>(tab), reinterpret_cast<float4*>(w + NA_LIST),
             reinterpret_cast<float*>(w + NA_SEATBOX), reinterpret_cast<float4*>(w + NA_BOUNDS),
             reinterpret_cast<float4*>(force), s);

@@ -191,10 +191,11 @@ void launch_centres(const float4* mesh, float4* com, int n, hipStream_t s);
 void launch_cell_only(const MeshBuf& m, const float4* farA, const float4* farT0, const float4* farT1, float4* force,
                       float eps2, float gm, int blocks, bool zeroed, hipStream_t s);
 // the pairwise pass: the loose list's gather, its tiles' boxes (bounds: 2 x PP_TILES_MAX), then each seat's
-// four slices (force[slice * PP_CAP + seat])
-void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
-               float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds, float4* force,
-               hipStream_t s);
+// four slices (force[slice * PP_CAP + seat]); the list's length *countDev where given (a captured cycle reads
+// the present list's), else count
+void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, const int* countDev,
+               int count, float gm, float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds,
+               float4* force, hipStream_t s);
 // the step pass's visit order: the particles sorted by the Morton code of their contact cell (coordinates
 // modulo 128); keys, slot: N ints, hist: KEYS + 1 ints, blockSums: scan scratch
 void launch_visit(const float4* pos, int N, float invCell, int* keys, int* slot, int* hist, int* blockSums,

@@ -713,9 +713,13 @@ void launch_cell(MeshBuf& m, float eps2, float gm, int blocks, bool zeroed, hipS
 // ------------------------------------------------------ pairwise P3M (A11) --
 // The loose list's seats (PPGATHER_FS, src/glsl-sim.js:676-698): seat s < count
 // holds particle index[s]; the rest, and an entry outside [0, N), are empty.
+// (the list's length: *countDev where given -- a captured cycle then reads the length of the present list --
+// else countArg; the same in the three passes)
 __global__ void __launch_bounds__(256) k_pp_gather(const float4* __restrict__ pos, const float* __restrict__ mat,
                                                    int N, const Box* __restrict__ bxp, const int* __restrict__ index,
-                                                   int count, float4* list, float* box) {
+                                                   const int* __restrict__ countDev, int countArg, float4* list,
+                                                   float* box) {
+  const int count = countDev ? *countDev : countArg;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= PP_CAP) return;
   int i = s < count ? index[s] : -1;
@@ -752,8 +756,11 @@ constexpr int PP_CHAIN = PP_COLS + 4;   // floats per (component, seat) row of t
 // gap, and FP32 rounding keeps that order (the differences, squares and sums are monotone in their
 // non-negative operands), so none of the tile's pairs is within the cut and each of its terms would be
 // zero.  A running sum that starts at +0 never becomes -0, so leaving out adds of +0 or -0 changes nothing.
-__global__ void __launch_bounds__(64) k_pp_bounds(const float4* __restrict__ list, int count, float4* bounds) {
+__global__ void __launch_bounds__(64) k_pp_bounds(const float4* __restrict__ list, const int* __restrict__ countDev,
+                                                  int countArg, float4* bounds) {
+  const int count = countDev ? *countDev : countArg;
   const int lane = threadIdx.x, share = (count + 3) >> 2, tps = (share + PP_COLS - 1) / PP_COLS;
+  if ((int)blockIdx.x >= 4 * tps) return;   // a grid for the cap: the tiles past this list's
   const int slice = blockIdx.x / tps, t = blockIdx.x % tps;
   const int j0 = slice * share + t * PP_COLS, j1 = min(min(j0 + PP_COLS, (slice + 1) * share), count);
   const int j = j0 + lane;
@@ -805,8 +812,10 @@ __device__ __forceinline__ float4 pp_term(float4 P, float bi, float4 Q, float bj
 }
 
 __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list, const float* __restrict__ box,
-                                                 const float4* __restrict__ bounds, int count, float gm, float cut,
-                                                 float a2, const float2* __restrict__ tab, float4* force) {
+                                                 const float4* __restrict__ bounds, const int* __restrict__ countDev,
+                                                 int countArg, float gm, float cut, float a2,
+                                                 const float2* __restrict__ tab, float4* force) {
+  const int count = countDev ? *countDev : countArg;
   __shared__ __align__(16) float term[4 * PP_ROWS * PP_CHAIN];
   __shared__ float4 lP[PP_ROWS], lQ[PP_COLS];
   __shared__ float bP[PP_ROWS], bQ[PP_COLS];
@@ -905,14 +914,16 @@ __global__ void __launch_bounds__(64) k_pp_force(const float4* __restrict__ list
   }
 }
 
-void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, int count, float gm,
-               float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds, float4* force,
-               hipStream_t s) {
-  hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, N, b, index, count, list, box);
-  const int tiles = 4 * (((count + 3) / 4 + PP_COLS - 1) / PP_COLS);
-  if (tiles > 0) hipLaunchKernelGGL(k_pp_bounds, dim3(tiles), dim3(64), 0, s, list, count, bounds);
-  hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / PP_ROWS, 4), dim3(64), 0, s, list, box, bounds, count, gm, cut, a2, tab,
-                     force);
+void launch_pp(const float4* pos, const float* mat, int N, const Box* b, const int* index, const int* countDev,
+               int count, float gm, float cut, float a2, const float2* tab, float4* list, float* box, float4* bounds,
+               float4* force, hipStream_t s) {
+  hipLaunchKernelGGL(k_pp_gather, dim3(PP_CAP / 256), dim3(256), 0, s, pos, mat, N, b, index, countDev, count, list,
+                     box);
+  // with the length on the device the tiles' grid is the cap's, the work-groups past the list's tiles leaving
+  const int tiles = countDev ? PP_TILES_MAX : 4 * (((count + 3) / 4 + PP_COLS - 1) / PP_COLS);
+  if (tiles > 0) hipLaunchKernelGGL(k_pp_bounds, dim3(tiles), dim3(64), 0, s, list, countDev, count, bounds);
+  hipLaunchKernelGGL(k_pp_force, dim3(PP_CAP / PP_ROWS, 4), dim3(64), 0, s, list, box, bounds, countDev, count, gm, cut,
+                     a2, tab, force);
 }
 
 __global__ void k_pp_slot(int N, const int* __restrict__ index, int count, float* slot) {
