@@ -67,6 +67,35 @@ def test_graph_replays_equal_eager(name):
         e.close()
 
 
+@pytest.mark.parametrize("name", ["impact", "twobox"])
+def test_graph_replays_follow_the_loose_list(name):
+    """A captured cycle's pairwise pass reads the loose list's length on the device, so a cycle captured with
+    one list replays with the next (a look's new list, longer or shorter): bit-identical to eager steps."""
+    fx = gu.fixture("c1_" + name)
+    g, e = make(fx, True), make(fx, False)
+    try:
+        u = gu.draws(fx, "dep")[0]
+        LA = np.array([float(x) for x in u["uLA"]])
+        LB = np.array([float(x) for x in u["uLB"]])
+        boxB = float(u["uBoxB"][0])
+        pp = np.asarray(fx["pp_index"])
+        args = ccamd.sim_args()
+        for call, keep in enumerate((len(pp), len(pp) // 2, len(pp), 7, len(pp) // 3)):
+            print(name, "call", call, "loose list", keep, flush=True)
+            for ctx in (g, e):
+                ctx.set_pp(pp[:keep])
+                n_mesh = sum(1 for i in range(16) if (ctx.step_no + i) % 8 == 0)
+                boxL = np.array([np.concatenate([LA + 1e-3 * k, LB - 1e-3 * k, [boxB]]) for k in range(n_mesh)])
+                ctx.step_n(16, args, boxL)
+            for a, b, what in zip(g.download(), e.download(), ("pos", "vel", "aux", "q", "mat")):
+                assert np.array_equal(a, b), (call, keep, what)
+            for a, b in zip(g.read_pp(), e.read_pp()):
+                assert np.array_equal(a, b), (call, keep, "pairwise")
+    finally:
+        g.close()
+        e.close()
+
+
 def test_odd_mesh_cadence_across_a_morton_sort():
     """meshEvery = 3: a cycle starts on an odd step as often as on an even one,
     and which of its steps rebuild the seats differs; a Morton sort between
