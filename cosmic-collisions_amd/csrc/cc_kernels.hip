@@ -1146,6 +1146,54 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
       wa = min(wa, waEnd);
     }
   }
+  // gravity at the particle: the constant field, or the mesh of its box (the monopoles outside both) and
+  // the pairwise pass's four slices
+  auto gravity = [&](float& gvx, float& gvy, float& gvz) {
+    gvx = 0; gvy = 0; gvz = 0;
+    if (u.field) {
+      gvx = u.gField[0]; gvy = u.gField[1]; gvz = u.gField[2];
+    } else {
+      float gx, gy, gz;
+      const Box B = *bxp;
+      int box = box_of(B, px, py, pz, gx, gy, gz);
+      float4 m0 = mono0[0], m1 = mono1[0];
+      if (box >= 0) {
+        const float4* F = box == 1 ? force1 : force0;
+        float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
+        float tx = gx - fx, ty = gy - fy, tz = gz - fz;
+        int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          int o0 = k & 1, o1 = (k >> 1) & 1, o2 = k >> 2;
+          float wt = gmix(1.0f - tx, tx, (float)o0) * gmix(1.0f - ty, ty, (float)o1) * gmix(1.0f - tz, tz, (float)o2);
+          float4 f = F[fine_at(min(ix + o0, 63), min(iy + o1, 63), min(iz + o2, 63))];
+          gvx += f.x * wt; gvy += f.y * wt; gvz += f.z * wt;
+        }
+        float4 sc = F[SCRATCH];
+        gvx -= sc.x * u.gm; gvy -= sc.y * u.gm; gvz -= sc.z * u.gm;
+        if (B.boxB > 0.5f) {
+          float ox, oy, oz;
+          if (box == 0) cross_at(m1, m0, u.gm, u.eps2, ox, oy, oz); else cross_at(m0, m1, u.gm, u.eps2, ox, oy, oz);
+          gvx += ox; gvy += oy; gvz += oz;
+        }
+      } else {
+        mass_at(px, py, pz, m0, u.gm, u.eps2, gvx, gvy, gvz);
+        if (B.boxB > 0.5f) {
+          float ox, oy, oz;
+          mass_at(px, py, pz, m1, u.gm, u.eps2, ox, oy, oz);
+          gvx += ox; gvy += oy; gvz += oz;
+        }
+      }
+      float seat = ppSlot[me];
+      if (seat >= 0.0f) {
+        int si = (int)seat;
+        for (int s = 0; s < 4; s++) {
+          float4 pf = ppForce[s * PP_CAP + si];
+          gvx += pf.x; gvy += pf.y; gvz += pf.z;
+        }
+      }
+    }
+  };
   int n = (int)((wa - wa0) / rowB);
   if (PLANES > 1) {
     // each list's length in its last row (a slack row, past every write of the walk), then the lists behind
@@ -1162,7 +1210,17 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
     if (w > 0 && total <= CAND_CAP)
       for (int k = 0; k < n; k++) cand[off + k][lane] = cand[w * CAND_ROWS + k][lane];
     __syncthreads();
-    if (w > 0) return;
+    if (w > 0) {   // wavefront 1 takes gravity into list 1's first rows (free now) while wavefront 0 goes on
+      if (w == 1) {
+        float gx, gy, gz;
+        gravity(gx, gy, gz);
+        cand[CAND_ROWS][lane] = __float_as_int(gx);
+        cand[CAND_ROWS + 1][lane] = __float_as_int(gy);
+        cand[CAND_ROWS + 2][lane] = __float_as_int(gz);
+      }
+      __syncthreads();
+      return;
+    }
     n = total;
   }
   const bool over = n > CAND_CAP;
@@ -1243,49 +1301,14 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
   }
   float pushx = acc.pushx, pushy = acc.pushy, pushz = acc.pushz, nb = acc.nb, lost = acc.lost, flow = acc.flow;
   float body = P.w;
-  float gvx = 0, gvy = 0, gvz = 0;
-  if (u.field) {
-    gvx = u.gField[0]; gvy = u.gField[1]; gvz = u.gField[2];
+  float gvx, gvy, gvz;
+  if (PLANES > 1) {   // wavefront 1's, beside the contacts
+    __syncthreads();
+    gvx = __int_as_float(cand[CAND_ROWS][lane]);
+    gvy = __int_as_float(cand[CAND_ROWS + 1][lane]);
+    gvz = __int_as_float(cand[CAND_ROWS + 2][lane]);
   } else {
-    float gx, gy, gz;
-    const Box B = *bxp;
-    int box = box_of(B, px, py, pz, gx, gy, gz);
-    float4 m0 = mono0[0], m1 = mono1[0];
-    if (box >= 0) {
-      const float4* F = box == 1 ? force1 : force0;
-      float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
-      float tx = gx - fx, ty = gy - fy, tz = gz - fz;
-      int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        int o0 = k & 1, o1 = (k >> 1) & 1, o2 = k >> 2;
-        float w = gmix(1.0f - tx, tx, (float)o0) * gmix(1.0f - ty, ty, (float)o1) * gmix(1.0f - tz, tz, (float)o2);
-        float4 f = F[fine_at(min(ix + o0, 63), min(iy + o1, 63), min(iz + o2, 63))];
-        gvx += f.x * w; gvy += f.y * w; gvz += f.z * w;
-      }
-      float4 sc = F[SCRATCH];
-      gvx -= sc.x * u.gm; gvy -= sc.y * u.gm; gvz -= sc.z * u.gm;
-      if (B.boxB > 0.5f) {
-        float ox, oy, oz;
-        if (box == 0) cross_at(m1, m0, u.gm, u.eps2, ox, oy, oz); else cross_at(m0, m1, u.gm, u.eps2, ox, oy, oz);
-        gvx += ox; gvy += oy; gvz += oz;
-      }
-    } else {
-      mass_at(px, py, pz, m0, u.gm, u.eps2, gvx, gvy, gvz);
-      if (B.boxB > 0.5f) {
-        float ox, oy, oz;
-        mass_at(px, py, pz, m1, u.gm, u.eps2, ox, oy, oz);
-        gvx += ox; gvy += oy; gvz += oz;
-      }
-    }
-    float seat = ppSlot[me];
-    if (seat >= 0.0f) {
-      int si = (int)seat;
-      for (int s = 0; s < 4; s++) {
-        float4 pf = ppForce[s * PP_CAP + si];
-        gvx += pf.x; gvy += pf.y; gvz += pf.z;
-      }
-    }
+    gravity(gvx, gvy, gvz);
   }
   float sc = u.invM / ri;
   vx += pushx * sc * u.dt + gvx * u.gdt;
