@@ -57,7 +57,7 @@ class CraterReport(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("cellBlocks", C.c_int32), ("simOrder", C.c_int32), ("cellMs", C.c_double * 4),
-                ("simMs", C.c_double * 3)]
+                ("simMs", C.c_double * 4)]
 
 
 class Pile(C.Structure):

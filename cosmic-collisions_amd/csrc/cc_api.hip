@@ -107,7 +107,8 @@ struct cc_ctx {
 };
 
 static std::string g_err;
-static const int SIM_PLANES_MAX_N = 98304;   // the default step-pass shape's crossover (cc_create)
+static const int SIM_PLANES_MAX_N = 98304;   // the default step-pass shapes' crossovers (cc_create)
+static const int SIM_ROWS_MAX_N = 16384;     // 256 work-groups of 64 particles: one a CU
 
 #define FAIL(ctx, code, ...)                                      \
   do {                                                            \
@@ -293,8 +294,9 @@ int cc_create(const cc_params* p, cc_ctx** out) {
   c->p = *p;
   c->N = p->N;
   // below about 1.5 wavefronts a SIMD of step pass the GPU is not full: a wavefront per z-plane is faster
-  // there (16,384: 12 against 20 us), above it the one-wavefront walk (262,144: 36 against 51 us)
-  c->simOrder = p->N <= SIM_PLANES_MAX_N ? 2 : 0;
+  // there (57,856: 15 against 21 us), above it the one-wavefront walk (262,144: 36 against 51 us); while the
+  // work-groups of nine wavefronts, one per row of three cells, fit the CUs at once, those (16,384: 11 us)
+  c->simOrder = p->N <= SIM_ROWS_MAX_N ? 3 : p->N <= SIM_PLANES_MAX_N ? 2 : 0;
   grid_make(p->G, c->grid);
   c->grid.IB = p->N > (1 << 18) ? 19 : 18;
   c->cells = (size_t)c->grid.AW * c->grid.AH;
@@ -942,7 +944,7 @@ int cc_set_tuning(cc_ctx* c, int cellBlocks, int simOrder, int fuseCount) {
   if (!c) return CC_EINVAL;
   bool known = false;
   for (int b : CELL_BLOCKS) known = known || b == cellBlocks;
-  if (!known || simOrder < 0 || simOrder > 2 || (fuseCount != 0 && fuseCount != 1))
+  if (!known || simOrder < 0 || simOrder > 3 || (fuseCount != 0 && fuseCount != 1))
     FAIL(c, CC_EINVAL, "unknown tuning (%d, %d, %d)", cellBlocks, simOrder, fuseCount);
   c->cellBlocks = cellBlocks;
   c->simOrder = simOrder;
@@ -995,7 +997,7 @@ int cc_autotune(cc_ctx* c, int reps, cc_tune_report* out) {
   c->cellBlocks = CELL_BLOCKS[best];
   cc_sim_args a;
   memset(&a, 0, sizeof a);
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < 4; k++) {
     c->simOrder = k;
     auto once = [&]() -> int {
       SimU u = sim_uniforms(c, &a);
@@ -1007,10 +1009,12 @@ int cc_autotune(cc_ctx* c, int reps, cc_tune_report* out) {
     };
     if ((r = timed(once, &out->simMs[k]))) return r;
   }
-  // a wavefront per z-plane must win by 2 % (its lists' join is work the one-wavefront walk does not do);
-  // launch order fetches about four times the HBM bytes (neighbours' data spread over every XCD's L2), which
-  // the pass, bound by issue, hardly feels alone but concurrent work does: it must win by 5 %
-  c->simOrder = out->simMs[2] < 0.98 * out->simMs[0] ? 2 : 0;
+  // a wavefront per z-plane or per row must win by 2 % (the lists' join is work the one-wavefront walk does
+  // not do); launch order fetches about four times the HBM bytes (neighbours' data spread over every XCD's
+  // L2), which the pass, bound by issue, hardly feels alone but concurrent work does: it must win by 5 %
+  c->simOrder = 0;
+  for (int k = 2; k < 4; k++)
+    if (out->simMs[k] < 0.98 * out->simMs[c->simOrder]) c->simOrder = k;
   if (out->simMs[1] < 0.95 * out->simMs[c->simOrder]) c->simOrder = 1;
   out->cellBlocks = c->cellBlocks;
   out->simOrder = c->simOrder;

@@ -1068,9 +1068,11 @@ __device__ __forceinline__ int seat_d2(uint32_t sv, int B, int bx, int by, int b
 // The pass for the particle at place `at` of the visit order, by lane threadIdx.x % 64 (cand: the
 // work-group's candidate lists, entry-major: a lane's list is a column, no bank conflicts).  PLANES 1: a
 // one-wavefront work-group, the wavefront walks the whole neighbourhood.  PLANES 3: three wavefronts for the
-// same 64 particles, wavefront w walks the z-plane dz = w - 1 into list w (rows w * CAND_ROWS on); the
-// lists are joined in plane order behind list 0 and wavefront 0 goes on alone -- the same list, so the same
-// sums, with three times the wavefronts in flight where the particles are too few to fill the GPU.
+// same 64 particles, wavefront w walks the z-plane dz = w - 1 into list w (rows w * CAND_ROWS on); PLANES 9:
+// nine, wavefront w the row dz = w / 3 - 1, dy = w % 3 - 1.  The lists are joined in that order (the
+// reference's cell order) behind list 0 -- the same list, so the same sums -- and wavefront 1 takes the
+// gravity while wavefront 0 takes the contacts: more wavefronts in flight where the particles are too few
+// to fill the GPU.
 template <bool COUNT, int PLANES = 1>
 __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, const Box* __restrict__ bxp,
                                          float touch2r, const float4* __restrict__ pos,
@@ -1110,12 +1112,15 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
   const uint32_t rowB = SIM_BLOCK * 4u, wa0 = (uint32_t)lane * 4u, waEnd = (CAND_CAP + 1) * rowB + wa0;
   char* const candB = reinterpret_cast<char*>(&cand[w * CAND_ROWS][0]);
   uint32_t wa = wa0;
-  for (int dz = PLANES == 1 ? -1 : w - 1; dz <= (PLANES == 1 ? 1 : w - 1); dz++) {
-    uint4 sa[9], sb[9];
-    bool ok[9];
+  // the cells a wavefront walks in a plane: all nine, or (PLANES 9) its row of three, dy = w % 3 - 1
+  constexpr int NC = PLANES == 9 ? 3 : 9;
+  const int dzw = PLANES == 9 ? w / 3 - 1 : w - 1;
+  for (int dz = PLANES == 1 ? -1 : dzw; dz <= (PLANES == 1 ? 1 : dzw); dz++) {
+    uint4 sa[NC], sb[NC];
+    bool ok[NC];
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
-      int dy = i / 3 - 1, dx = i % 3 - 1;
+    for (int i = 0; i < NC; i++) {
+      int dy = PLANES == 9 ? w % 3 - 1 : i / 3 - 1, dx = PLANES == 9 ? i - 1 : i % 3 - 1;
       float lox = c0x + (float)dx * u.cell, loy = c0y + (float)dy * u.cell, loz = c0z + (float)dz * u.cell;
       float qx = gclamp(px, lox, lox + u.cell) - px, qy = gclamp(py, loy, loy + u.cell) - py,
             qz = gclamp(pz, loz, loz + u.cell) - pz;
@@ -1128,10 +1133,11 @@ __device__ __forceinline__ void sim_body(int (*cand)[SIM_BLOCK], const SimU& u, 
     int bz = SEAT_Q * dz + 16 - pqz;
     asm volatile("" : "+v"(bz));   // held in a register: one shift-add per seat, not two
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
+    for (int i = 0; i < NC; i++) {
       const unsigned long long okm = __ballot(ok[i]);
       if (okm == 0ull) continue;
-      const int bx = SEAT_Q * (i % 3 - 1) + 16 - pqx, by = SEAT_Q * (i / 3 - 1) + 16 - pqy;
+      const int bx = SEAT_Q * ((PLANES == 9 ? i : i % 3) - 1) + 16 - pqx,
+                by = SEAT_Q * ((PLANES == 9 ? w % 3 : i / 3) - 1) + 16 - pqy;
       uint32_t sv8[8] = { sa[i].x, sb[i].x, sa[i].y, sb[i].y, sa[i].z, sb[i].z, sa[i].w, sb[i].w };
 #pragma unroll
       for (int k = 0; k < SLOTS; k++) {
@@ -1415,7 +1421,14 @@ void launch_sim(const SimU& u, const Box* bx, const float4* pos, const float4* v
   blocks = (blocks + 7) & ~7;   // a multiple of the 8 XCDs, so the block remap applies
   const SlotsOut o = count ? *count : SlotsOut{};
   const float t2 = touch_r2(u.touch);
-  if (u.order == 2) {   // three wavefronts per 64 particles, one per z-plane of the neighbourhood
+  if (u.order == 3) {   // nine wavefronts per 64 particles, one per row of three cells of the neighbourhood
+    if (count)
+      hipLaunchKernelGGL((k_sim<true, 9>), dim3(blocks), dim3(9 * SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q,
+                         seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);
+    else
+      hipLaunchKernelGGL((k_sim<false, 9>), dim3(blocks), dim3(9 * SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q,
+                         seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);
+  } else if (u.order == 2) {   // three wavefronts per 64 particles, one per z-plane of the neighbourhood
     if (count)
       hipLaunchKernelGGL((k_sim<true, 3>), dim3(blocks), dim3(3 * SIM_BLOCK), 0, s, u, bx, t2, pos, vel, mat, q,
                          seats, m0.force, m0.mono, m1.force, m1.mono, ppSlot, ppForce, oPos, oVel, oAux, oQ, o);

@@ -294,8 +294,10 @@ int cc_batch_last_ms(cc_batch* b, double* ms);
  * wavefront per 64 particles, consecutive work-groups spread over the XCDs;
  * 1: the same in launch order; 2: spread over the XCDs, three wavefronts per
  * 64 particles, one per z-plane of the neighbourhood, their candidate lists
- * joined in order -- for states too small to fill the GPU; each particle's
- * update is its own) and where a seat rebuild's count is taken
+ * joined in order -- for states too small to fill the GPU; 3: nine
+ * wavefronts, one per row of three cells -- while the work-groups fit the
+ * CUs at once; each particle's update is its own) and where a seat
+ * rebuild's count is taken
  * (fuseCount 1: by the step pass before it, on the positions that pass
  * writes; 0: by a pass of its own at the rebuild; the same counts).  cc_autotune times every candidate on the
  * present state and keeps the fastest (a candidate must beat the current
@@ -303,13 +305,13 @@ int cc_batch_last_ms(cc_batch* b, double* ms);
  * HBM bytes), the state as it was; it is called where the next step is a
  * mesh step, without constant gravity or a learned far field attached
  * (CC_ESTATE otherwise); it leaves fuseCount as it is.  cc_set_tuning sets all
- * three outright.  Defaults 2048, shape 2 up to 98,304 particles and 0 above,
- * and 1.  Captured step cycles are
+ * three outright.  Defaults 2048, shape 3 up to 16,384 particles, 2 up to
+ * 98,304 and 0 above, and 1.  Captured step cycles are
  * re-captured for a new choice. */
 typedef struct {
   int32_t cellBlocks, simOrder;   /* the choice */
   double cellMs[4];               /* mesh pass per cell grid (512, 1024, 2048, 4096), ms */
-  double simMs[3];                /* step pass per shape (0, 1, 2), ms */
+  double simMs[4];                /* step pass per shape (0-3), ms */
 } cc_tune_report;
 int cc_autotune(cc_ctx* ctx, int reps, cc_tune_report* out);
 int cc_set_tuning(cc_ctx* ctx, int cellBlocks, int simOrder, int fuseCount);
